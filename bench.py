@@ -1,0 +1,218 @@
+"""Benchmark of the force-and-integrate step on one MI355X.
+
+Headline: direct-sum Ginteractions/s on the 50,000-particle exponential disk
+(scale 20, radius 100; G=1, softening 0.2, dt=1e-3).  The same JSON line also
+carries the Barnes-Hut step at 500,000 particles (two exponential disks,
+theta=1, W=64) because the headline metric names both.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--bh-steps K2] [--no-cpu]
+
+Inputs are generated on the host and uploaded before the timed region; the
+timed region covers K steps bracketed by device synchronisation.  Kernel times
+come from HIP events recorded by libgsim on the simulation's own stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gravity-sim_amd", "python"))
+
+import numpy as np  # noqa: E402
+
+import gsim  # noqa: E402
+
+# MI355X published peaks
+PEAK_FP32_TFLOPS = 157.3          # vector FP32 (packed FMA), TFLOP/s
+PEAK_HBM_GBS = 8000.0             # HBM3E, GB/s
+VALU_LANE_OPS = 256 * 64 * 2.4e9  # one FP32 op per lane per clock, lane-ops/s
+DS_FLOP_PER_PAIR = 14             # 2 sub, 2 FMA, rsqrt, 3 mul, 2 FMA
+DS_LANEOPS_PER_PAIR = 10
+BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
+BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
+REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
+REF_BH_MS_500K = 2.853            # reference README.md:19, ms/step at 500,000
+
+DS_N = 50000
+BH_N = 500000
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_direct_sum(steps: int, warmup: int) -> dict:
+    p = gsim.generate("exponential-disk", DS_N, 100.0, 1.0, 10.0, 1.0, seed=2)
+    sim = gsim.Simulation(p, "direct-sum")
+    sim.step(max(warmup, 1))
+    sim.synchronize()
+    sim.set_timing(True)
+    sim.timing(reset=True)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    t = sim.timing(reset=True)
+    sim.close()
+    pairs = float(DS_N) * DS_N
+    ms_step = elapsed * 1e3 / steps
+    kernel_ms = t["force_ms"] / max(t["force_launches"], 1)
+    achieved = DS_FLOP_PER_PAIR * pairs / (kernel_ms * 1e-3) / 1e12
+    return {
+        "value": pairs * steps / elapsed / 1e9,
+        "ms_per_step": ms_step,
+        "kernel_ms": kernel_ms,
+        "combine_ms": t["integrate_ms"] / max(t["steps"], 1),
+        "roofline": {
+            "bound": "valu",
+            "kernel": "ds_partial_kernel",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "valu_lane_op_frac": DS_LANEOPS_PER_PAIR * pairs / (kernel_ms * 1e-3) / VALU_LANE_OPS,
+        },
+    }
+
+
+def bench_barnes_hut(steps: int, warmup: int) -> dict:
+    p = gsim.generate("two-exponential-disks", BH_N, 33.0, 1.0, 10.0, 1.0, seed=3)
+    sim = gsim.Simulation(p, "barnes-hut", accuracy_parameter=1.0, group_width=64)
+    sim.step(max(warmup, 1))
+    sim.synchronize()
+    sim.set_timing(True)
+    sim.timing(reset=True)
+    V = A = 0
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    V, A = (int(x) for x in sim.dump(gsim.DUMP_COUNTERS))  # last step's counts
+    t = sim.timing(reset=True)
+    sim.close()
+    ms_step = elapsed * 1e3 / steps
+    trav_ms = t["force_ms"] / max(t["steps"], 1)
+    build_ms = t["build_ms"] / max(t["steps"], 1)
+    flops = BH_FLOP_VISIT * V + BH_FLOP_ACCEPT * A
+    achieved = flops / (trav_ms * 1e-3) / 1e12
+    lane_ops = BH_LANEOPS_VISIT * V + BH_LANEOPS_ACCEPT * A
+    # step roofline: build bytes at HBM peak + traversal lane-ops at VALU peak
+    na = (BH_N + 63) // 64 * 64
+    t_star = (64 * na) / (PEAK_HBM_GBS * 1e9) + lane_ops / VALU_LANE_OPS
+    return {
+        "metric": "BH step ms @500k",
+        "value": ms_step,
+        "unit": "ms/step",
+        "higher_is_better": False,
+        "vs_baseline": REF_BH_MS_500K / ms_step,
+        "particle_steps_per_s": BH_N / (ms_step * 1e-3),
+        "build_ms": build_ms,
+        "traverse_ms": trav_ms,
+        "visits": V,
+        "accepts": A,
+        "config": {"workload": "barnes-hut, two exponential disks of 250,000 (radius 33, "
+                               "scale 7, centres +-40, bulk vx -+2), theta=1, W=64, "
+                               "G=1, softening 0.2, dt=1e-3", "particles": BH_N,
+                   "theta": 1.0, "group_width": 64},
+        "roofline": {
+            "bound": "valu",
+            "kernel": "bh_traverse64_kernel",
+            "achieved": achieved,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "valu_lane_op_frac": lane_ops / (trav_ms * 1e-3) / VALU_LANE_OPS,
+            "step_frac": t_star / (ms_step * 1e-3),
+        },
+    }
+
+
+def cpu_baseline(budget_s: float = 12.0) -> dict:
+    """The oracle's FP32 direct sum (scalar C, one thread per core slice) on
+    the same 50k workload, timed on this host: a bounded sample of whole
+    force passes over all 50,000 x 50,000 pairs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker, used here only as the CPU baseline
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    p = gsim.generate("exponential-disk", DS_N, 100.0, 1.0, 10.0, 1.0, seed=2)
+    eps2 = oracle.square_f32(0.2)
+    # full passes over a target subset, sized to fit the time budget
+    t0 = time.perf_counter()
+    oracle.ds_accel_f32(p.pos, p.mass, eps2, nthreads=threads)
+    one = time.perf_counter() - t0
+    reps = max(1, int(budget_s / max(one, 1e-3)))
+    reps = min(reps, 8)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        oracle.ds_accel_f32(p.pos, p.mass, eps2, nthreads=threads)
+    el = time.perf_counter() - t0
+    pairs = float(DS_N) * DS_N * reps
+    return {"value": pairs / el / 1e9, "unit": "Ginteractions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{reps + 1} full 50,000x50,000 FP32 force passes of the oracle "
+                      f"({threads} threads), integrate excluded"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--bh-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bh", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the simulation step has no multi-GPU path "
+              "in this build (see DESIGN.md for how it would shard); run with --gpus 1.",
+              file=sys.stderr)
+        return 2
+    log(f"direct-sum 50k: warmup {args.warmup}, steps {args.steps}")
+    ds = bench_direct_sum(args.steps, args.warmup)
+    log(f"  {ds['value']:.1f} Ginteractions/s, {ds['ms_per_step']:.4f} ms/step")
+    bh = None
+    if not args.no_bh:
+        log(f"barnes-hut 500k: steps {args.bh_steps}")
+        bh = bench_barnes_hut(args.bh_steps, max(args.warmup, 1))
+        log(f"  {bh['value']:.4f} ms/step (build {bh['build_ms']:.4f}, traverse "
+            f"{bh['traverse_ms']:.4f})")
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle)")
+        cpu = cpu_baseline()
+        log(f"  {cpu['value']:.2f} Ginteractions/s on {cpu['cores']} threads")
+    ref_ginter = DS_N * DS_N / (REF_DS_MS_50K * 1e-3) / 1e9
+    line = {
+        "metric": "direct-sum Ginteractions/s @50k",
+        "value": ds["value"],
+        "unit": "Ginteractions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ds["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": ds["value"] / ref_ginter,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "direct-sum, 50,000-particle exponential disk (radius 100, "
+                               "scale 20, circular velocities + 5% dispersion, masses "
+                               "U[1,10]), G=1, softening 0.2, dt=1e-3",
+                   "particles": DS_N, "global_batch": 1, "parallelism": "single-gpu"},
+        "roofline": ds["roofline"],
+        "cpu_baseline": cpu,
+        "secondary": [bh] if bh else [],
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,98 @@
+"""Builds libgsim.so (HIP kernels for gfx950 + the C-ABI host runtime) and the
+gravity-sim command-line tool, in-tree under gravity-sim_amd/lib and
+gravity-sim_amd/bin.
+
+Usage: python gravity-sim_amd/build.py [--jobs N]
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+INCLUDE = os.path.join(ROOT, "include")
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+BUILD = os.path.join(PKG, "_build")
+LIBDIR = os.path.join(PKG, "lib")
+BINDIR = os.path.join(PKG, "bin")
+LIBGSIM = os.path.join(LIBDIR, "libgsim.so")
+CLI = os.path.join(BINDIR, "gravity-sim")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+ARCH = "gfx950"
+
+# -ffp-contract=off: every FP32 operation rounds as written (explicit fmaf is
+# still an FMA).  Packed FP32 instructions are disabled for device code.
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-I", INCLUDE, "-I", CSRC,
+          "-Wall", "-Wno-unused-function"]
+DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-munsafe-fp-atomics"]
+
+SOURCES = [
+    (os.path.join(CSRC, "ds_kernels.hip"), "hip"),
+    (os.path.join(CSRC, "bh_kernels.hip"), "hip"),
+    (os.path.join(CSRC, "api.cpp"), "host"),
+    (os.path.join(HOST, "particles.cpp"), "host"),
+]
+
+
+def _run(cmd: list[str]) -> None:
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(line for line in proc.stdout.splitlines()
+                    if "packed-fp32-ops" not in line)  # host pass: unknown feature
+    if proc.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
+    if out.strip():
+        print(out)
+
+
+def _compile(src: str, kind: str) -> str:
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if kind == "hip":
+        cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
+    _run(cmd)
+    return obj
+
+
+def build_lib(jobs: int = 4) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(lambda s: _compile(*s), SOURCES))
+    tmp = LIBGSIM + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIBGSIM)
+    return LIBGSIM
+
+
+def build_cli() -> str:
+    os.makedirs(BINDIR, exist_ok=True)
+    src = os.path.join(HOST, "main.cpp")
+    if not os.path.exists(src):
+        return ""
+    _run(["g++", "-O2", "-std=c++17", "-I", INCLUDE, "-I", HOST, src, "-o", CLI,
+          f"-L{LIBDIR}", "-lgsim", "-Wl,-rpath,$ORIGIN/../lib"])
+    return CLI
+
+
+def main(argv: list[str] | None = None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=4)
+    args = ap.parse_args(argv)
+    print(build_lib(args.jobs))
+    cli = build_cli()
+    if cli:
+        print(cli)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,441 @@
+// C-ABI entry points of libgsim: handle lifetime, stepping, readback, debug
+// dumps and HIP-event timing.  HIP kernels live in ds_kernels.hip and
+// bh_kernels.hip; this file is host code built by hipcc.
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace gsim {
+
+namespace {
+thread_local char g_error[512] = {0};
+}
+
+gsim_status fail(gsim_status st, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(g_error, sizeof(g_error), fmt, ap);
+    va_end(ap);
+    return st;
+}
+
+void clear_error() { g_error[0] = 0; }
+
+}  // namespace gsim
+
+using gsim::fail;
+
+namespace {
+
+gsim_status hip_fail(hipError_t e, const char *what) {
+    return fail(GSIM_ERROR_DEVICE, "%s failed: %s", what, hipGetErrorString(e));
+}
+
+#define GSIM_HIP(call)                                       \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);    \
+    } while (0)
+
+size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
+
+void free_sim(gsim_sim *s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (int k = 0; k < 2; ++k) {
+        (void)hipFree(s->pos[k]);
+        (void)hipFree(s->vel[k]);
+    }
+    (void)hipFree(s->mass);
+    (void)hipFree(s->partial);
+    (void)hipFree(s->accel);
+    gsim::bh_free(&s->bh);
+    for (auto &p : s->pending)
+        for (hipEvent_t e : p.e) s->event_pool.push_back(e);
+    for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+hipEvent_t take_event(gsim_sim *s) {
+    if (!s->event_pool.empty()) {
+        hipEvent_t e = s->event_pool.back();
+        s->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Converts recorded event pairs into accumulated milliseconds.
+gsim_status drain_timing(gsim_sim *s) {
+    if (s->pending.empty()) return GSIM_OK;
+    GSIM_HIP(hipStreamSynchronize(s->stream));
+    for (auto &p : s->pending) {
+        float first = 0.0f, second = 0.0f, whole = 0.0f;
+        GSIM_HIP(hipEventElapsedTime(&first, p.e[0], p.e[1]));
+        GSIM_HIP(hipEventElapsedTime(&second, p.e[1], p.e[2]));
+        GSIM_HIP(hipEventElapsedTime(&whole, p.e[0], p.e[2]));
+        if (p.bh) {
+            s->acc.build_ms += first;
+            s->acc.force_ms += second;
+        } else {
+            s->acc.force_ms += first;
+            s->acc.integrate_ms += second;
+        }
+        s->acc.step_ms += whole;
+        s->acc.steps += 1;
+        s->acc.force_launches += 1;
+        for (hipEvent_t e : p.e) s->event_pool.push_back(e);
+    }
+    s->pending.clear();
+    return GSIM_OK;
+}
+
+gsim_status ds_alloc_partial(gsim_sim *s) {
+    (void)hipFree(s->partial);
+    s->partial = nullptr;
+    GSIM_HIP(hipMalloc((void **)&s->partial, sizeof(float2) * s->na * s->ds_S));
+    s->ds_S_cap = s->ds_S;
+    return GSIM_OK;
+}
+
+gsim_status run_steps(gsim_sim *s, uint32_t count) {
+    const bool bh = s->params.algorithm == GSIM_BARNES_HUT;
+    for (uint32_t k = 0; k < count; ++k) {
+        const int in = s->cur, out = s->cur ^ 1;
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        if (s->timing) {
+            for (auto &e : ev) {
+                e = take_event(s);
+                if (!e) return fail(GSIM_ERROR_DEVICE, "hipEventCreate failed");
+            }
+            GSIM_HIP(hipEventRecord(ev[0], s->stream));
+        }
+        if (bh) {
+            GSIM_HIP(gsim::bh_launch_build(s, s->pos[in], s->mass, s->pos[out], s->vel[in],
+                                           s->vel[out]));
+            if (s->timing) GSIM_HIP(hipEventRecord(ev[1], s->stream));
+            GSIM_HIP(gsim::bh_launch_traverse(s, s->pos[in], s->vel[in], s->pos[out],
+                                              s->vel[out]));
+            s->has_tree = true;
+        } else {
+            GSIM_HIP(gsim::ds_launch_partial(s, s->pos[in], s->mass));
+            if (s->timing) GSIM_HIP(hipEventRecord(ev[1], s->stream));
+            GSIM_HIP(gsim::ds_launch_combine(s, s->pos[in], s->vel[in], s->pos[out],
+                                             s->vel[out]));
+        }
+        if (s->timing) {
+            GSIM_HIP(hipEventRecord(ev[2], s->stream));
+            s->pending.push_back({{ev[0], ev[1], ev[2]}, bh});
+            if (s->pending.size() > 100000) {
+                gsim_status st = drain_timing(s);
+                if (st != GSIM_OK) return st;
+            }
+        }
+        s->cur = out;
+        ++s->steps;
+    }
+    return GSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gsim_last_error(void) { return gsim::g_error; }
+
+const char *gsim_version(void) { return "gsim-mi355x 0.1.0 (gfx950)"; }
+
+void gsim_default_params(gsim_params *p) {
+    if (!p) return;
+    // ProgramInfo defaults, Main.cpp:59-63
+    p->gravitational_const = 1.0f;
+    p->simulation_time = 0.001f;
+    p->simulation_speed = 1.0f;
+    p->softening_len = 0.2f;
+    p->accuracy_parameter = 1.0f;
+    p->algorithm = GSIM_DIRECT_SUM;
+    p->group_width = 64;
+    p->device = 0;
+}
+
+size_t gsim_required_alignment(int32_t algorithm) {
+    // DirectSimulation.cpp:26-28 (WORKGROUP_SIZE) and BarnesHutSimulation.cpp:834-836
+    return (algorithm == GSIM_DIRECT_SUM || algorithm == GSIM_BARNES_HUT) ? 64 : 0;
+}
+
+gsim_status gsim_create(const gsim_params *params, const float *pos_xy, const float *vel_xy,
+                        const float *mass, size_t n, gsim_sim **out) {
+    gsim::clear_error();
+    if (!out) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_create: out is NULL");
+    *out = nullptr;
+    if (!params || !pos_xy || !vel_xy || !mass)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_create: NULL argument");
+    if (n == 0)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "The simulation must contain at least one particle!");
+    if (params->algorithm != GSIM_DIRECT_SUM && params->algorithm != GSIM_BARNES_HUT)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "A valid simulation algorithm must be given!");
+    const uint32_t W = params->group_width ? params->group_width : 64;
+    if (W != 32 && W != 64)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "group_width must be 32 or 64 (got %u)", W);
+    const float vals[] = {params->gravitational_const, params->simulation_time,
+                          params->simulation_speed, params->softening_len,
+                          params->accuracy_parameter};
+    for (float v : vals)
+        if (!std::isfinite(v))
+            return fail(GSIM_ERROR_INVALID_ARGUMENT, "simulation parameters must be finite");
+    const size_t na = align_up(n, 64);
+    if (na > 0x7fffffffull / 4)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "particle count %zu too large", n);
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(GSIM_ERROR_NO_DEVICE, "no HIP device visible");
+    if (params->device < 0 || params->device >= ndev)
+        return fail(GSIM_ERROR_NO_DEVICE, "device %d not present (%d visible)", params->device, ndev);
+    hipDeviceProp_t prop;
+    GSIM_HIP(hipGetDeviceProperties(&prop, params->device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(GSIM_ERROR_NO_DEVICE, "device %d is %s; libgsim is built for gfx950 only",
+                    params->device, prop.gcnArchName);
+    GSIM_HIP(hipSetDevice(params->device));
+
+    gsim_sim *s = new (std::nothrow) gsim_sim();
+    if (!s) return fail(GSIM_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    s->params = *params;
+    s->params.group_width = W;
+    s->device = params->device;
+    s->n = n;
+    s->na = na;
+    s->W = W;
+    const float G = params->gravitational_const;
+    const float soft = params->softening_len, acc = params->accuracy_parameter;
+    s->eps2 = soft * soft;  // DirectSimulation.cpp:179, BarnesHutSimulation.cpp:577
+    s->theta2 = acc * acc;  // BarnesHutSimulation.cpp:578
+    if (params->algorithm == GSIM_BARNES_HUT) {
+        s->dt = params->simulation_time * params->simulation_speed;  // :575
+        s->kv = G * s->dt;   // GRAVITATIONAL_CONST * SIMULATION_TIME, ForceShader :145
+        s->kp = 0.5f * s->dt;
+    } else {
+        s->dt = params->simulation_time;  // SimShader ignores --simulation-speed
+        s->kv = s->dt * G;                // SIMULATION_TIME * GRAVITATIONAL_CONST, :68
+        s->kp = 0.5f * s->dt;
+    }
+
+    auto bail = [&](gsim_status st) {
+        free_sim(s);
+        return st;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return bail(hip_fail(e, "hipStreamCreateWithFlags"));
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipMalloc((void **)&s->pos[k], sizeof(float2) * na);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->vel[k], sizeof(float2) * na);
+    }
+    if (e == hipSuccess) e = hipMalloc((void **)&s->mass, sizeof(float) * na);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->accel, sizeof(float2) * na);
+    if (e != hipSuccess) return bail(fail(GSIM_ERROR_OUT_OF_MEMORY, "device allocation failed: %s",
+                                          hipGetErrorString(e)));
+
+    // Padding particles are {0,0,0,0,0} (ParticleSystem.cpp:487-491).
+    std::vector<float> hp(2 * na, 0.0f), hv(2 * na, 0.0f), hm(na, 0.0f);
+    std::memcpy(hp.data(), pos_xy, sizeof(float) * 2 * n);
+    std::memcpy(hv.data(), vel_xy, sizeof(float) * 2 * n);
+    std::memcpy(hm.data(), mass, sizeof(float) * n);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipMemcpy(s->pos[k], hp.data(), sizeof(float) * 2 * na, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(s->vel[k], hv.data(), sizeof(float) * 2 * na, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(s->mass, hm.data(), sizeof(float) * na, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(s->accel, 0, sizeof(float2) * na);
+    if (e != hipSuccess) return bail(hip_fail(e, "upload"));
+
+    if (params->algorithm == GSIM_DIRECT_SUM) {
+        gsim::ds_pick_shape(na, &s->ds_R, &s->ds_S);
+        gsim_status st = ds_alloc_partial(s);
+        if (st != GSIM_OK) return bail(st);
+    } else {
+        e = gsim::bh_alloc(&s->bh, na);
+        if (e != hipSuccess)
+            return bail(fail(GSIM_ERROR_OUT_OF_MEMORY, "Barnes-Hut buffers: %s", hipGetErrorString(e)));
+    }
+    *out = s;
+    return GSIM_OK;
+}
+
+gsim_status gsim_step(gsim_sim *s, uint32_t count) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_step: NULL handle");
+    if (count == 0) return GSIM_OK;  // BarnesHutSimulation.cpp:851-852
+    GSIM_HIP(hipSetDevice(s->device));
+    return run_steps(s, count);
+}
+
+gsim_status gsim_synchronize(gsim_sim *s) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_synchronize: NULL handle");
+    GSIM_HIP(hipStreamSynchronize(s->stream));
+    return GSIM_OK;
+}
+
+gsim_status gsim_read(gsim_sim *s, float *pos_xy, float *vel_xy, float *mass) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_read: NULL handle");
+    GSIM_HIP(hipSetDevice(s->device));
+    GSIM_HIP(hipStreamSynchronize(s->stream));
+    if (pos_xy) GSIM_HIP(hipMemcpy(pos_xy, s->pos[s->cur], sizeof(float2) * s->n, hipMemcpyDeviceToHost));
+    if (vel_xy) GSIM_HIP(hipMemcpy(vel_xy, s->vel[s->cur], sizeof(float2) * s->n, hipMemcpyDeviceToHost));
+    if (mass) GSIM_HIP(hipMemcpy(mass, s->mass, sizeof(float) * s->n, hipMemcpyDeviceToHost));
+    return GSIM_OK;
+}
+
+gsim_status gsim_destroy(gsim_sim *s) {
+    gsim::clear_error();
+    if (!s) return GSIM_OK;
+    (void)hipSetDevice(s->device);
+    free_sim(s);
+    return GSIM_OK;
+}
+
+size_t gsim_particle_count(const gsim_sim *s) { return s ? s->n : 0; }
+size_t gsim_aligned_particle_count(const gsim_sim *s) { return s ? s->na : 0; }
+uint64_t gsim_steps_done(const gsim_sim *s) { return s ? s->steps : 0; }
+
+gsim_status gsim_set_debug(gsim_sim *s, uint32_t flags) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_debug: NULL handle");
+    s->debug_flags = flags;
+    return GSIM_OK;
+}
+
+gsim_status gsim_set_ds_shape(gsim_sim *s, uint32_t R, uint32_t S) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_ds_shape: NULL handle");
+    if (s->params.algorithm != GSIM_DIRECT_SUM)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_ds_shape: not a direct-sum handle");
+    if (R != 1 && R != 2 && R != 4)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "targets per lane must be 1, 2 or 4");
+    if (S == 0) {
+        uint32_t r;
+        gsim::ds_pick_shape(s->na, &r, &S);
+    }
+    if (S > 256 || (size_t)S * 64 > s->na)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "slices must be <= 256 and <= Na/64");
+    GSIM_HIP(hipSetDevice(s->device));
+    GSIM_HIP(hipStreamSynchronize(s->stream));
+    s->ds_R = R;
+    s->ds_S = S;
+    if (S > s->ds_S_cap) return ds_alloc_partial(s);
+    return GSIM_OK;
+}
+
+gsim_status gsim_set_timing(gsim_sim *s, int32_t enable) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_timing: NULL handle");
+    s->timing = enable != 0;
+    return GSIM_OK;
+}
+
+gsim_status gsim_get_timing(gsim_sim *s, gsim_timing *out, int32_t reset) {
+    gsim::clear_error();
+    if (!s || !out) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_get_timing: NULL argument");
+    GSIM_HIP(hipSetDevice(s->device));
+    gsim_status st = drain_timing(s);
+    if (st != GSIM_OK) return st;
+    *out = s->acc;
+    if (reset) s->acc = gsim_timing{};
+    return GSIM_OK;
+}
+
+gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, size_t *written) {
+    gsim::clear_error();
+    if (!s || !written) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_debug_dump: NULL argument");
+    GSIM_HIP(hipSetDevice(s->device));
+    GSIM_HIP(hipStreamSynchronize(s->stream));
+    const size_t na = s->na;
+    if (what == GSIM_DUMP_ACCEL) {
+        *written = sizeof(float2) * na;
+        if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+        GSIM_HIP(hipMemcpy(buf, s->accel, *written, hipMemcpyDeviceToHost));
+        return GSIM_OK;
+    }
+    if (s->params.algorithm != GSIM_BARNES_HUT)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "dump %d needs a Barnes-Hut handle", what);
+    gsim::BhCounters ctr{};
+    GSIM_HIP(hipMemcpy(&ctr, s->bh.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    const size_t nodes = s->has_tree ? ctr.n_nodes : 0;
+    switch (what) {
+        case GSIM_DUMP_CELLS: {
+            *written = sizeof(uint32_t) * na;
+            if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            GSIM_HIP(hipMemcpy(buf, s->bh.keys[0], *written, hipMemcpyDeviceToHost));
+            uint32_t *u = (uint32_t *)buf;
+            for (size_t i = 0; i < na; ++i)
+                if (u[i] >= gsim::kKilledKey) u[i] = 0xffffffffu;
+            return GSIM_OK;
+        }
+        case GSIM_DUMP_MORTON30: {
+            *written = sizeof(uint32_t) * na;
+            if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            // positions the last step read (its input set)
+            const int in = s->steps ? (s->cur ^ 1) : s->cur;
+            GSIM_HIP(gsim::bh_launch_morton30(s, s->pos[in], s->bh.keys[2]));
+            GSIM_HIP(hipStreamSynchronize(s->stream));
+            GSIM_HIP(hipMemcpy(buf, s->bh.keys[2], *written, hipMemcpyDeviceToHost));
+            return GSIM_OK;
+        }
+        case GSIM_DUMP_SORTED_SRC:
+            *written = sizeof(uint32_t) * na;
+            if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            GSIM_HIP(hipMemcpy(buf, s->bh.sorted_src, *written, hipMemcpyDeviceToHost));
+            return GSIM_OK;
+        case GSIM_DUMP_NODE_TOTAL:
+            *written = sizeof(uint32_t);
+            if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            *(uint32_t *)buf = (uint32_t)nodes;
+            return GSIM_OK;
+        case GSIM_DUMP_COUNTERS:
+            *written = 2 * sizeof(uint64_t);
+            if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            ((uint64_t *)buf)[0] = ctr.visits;
+            ((uint64_t *)buf)[1] = ctr.accepts;
+            return GSIM_OK;
+        case GSIM_DUMP_COUNTS:
+            *written = sizeof(uint32_t) * nodes;
+            if (bytes < *written || (!buf && nodes)) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            if (nodes) GSIM_HIP(hipMemcpy(buf, s->bh.ncount, *written, hipMemcpyDeviceToHost));
+            return GSIM_OK;
+        case GSIM_DUMP_RADII:
+        case GSIM_DUMP_NODE_POS:
+        case GSIM_DUMP_NODE_MASS: {
+            const size_t per = what == GSIM_DUMP_NODE_POS ? 2 : 1;
+            *written = sizeof(float) * per * nodes;
+            if (bytes < *written || (!buf && nodes)) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            if (!nodes) return GSIM_OK;
+            std::vector<float4> tmp(nodes);
+            GSIM_HIP(hipMemcpy(tmp.data(), s->bh.node4, sizeof(float4) * nodes, hipMemcpyDeviceToHost));
+            float *f = (float *)buf;
+            for (size_t k = 0; k < nodes; ++k) {
+                if (what == GSIM_DUMP_RADII) f[k] = tmp[k].w;
+                else if (what == GSIM_DUMP_NODE_MASS) f[k] = tmp[k].z;
+                else { f[2 * k] = tmp[k].x; f[2 * k + 1] = tmp[k].y; }
+            }
+            return GSIM_OK;
+        }
+        default:
+            return fail(GSIM_ERROR_INVALID_ARGUMENT, "unknown dump selector %d", what);
+    }
+}
+
+}  // extern "C"

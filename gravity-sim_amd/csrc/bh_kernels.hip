@@ -1,0 +1,740 @@
+// Barnes-Hut build and traversal for gfx950.
+//
+// Tree semantics follow the reference's fixed-domain quadtree
+// (BarnesHutSimulation.cpp:885-918 and its seven shaders): domain [-500,500)^2,
+// 1024^2 leaf grid, 11 levels, a node for every particle and for every cell
+// holding >= 2 particles, flattened in pre-order with subtree sizes.  The
+// build here is sparse instead of the reference's dense 1.4M-cell pyramid:
+//
+//   1. keys      20-bit Morton cell per particle (InitShader.comp:80-98); the
+//                out-of-domain kill of InitShader.comp:110-117.
+//   2. sort      stable LSD radix sort of (cell, index), 3 passes of 8 bits,
+//                so a leaf's particles are in ascending index order.
+//   3. prep/scan s(k) = depth shared by sorted keys k-1 and k; the cell nodes
+//                that start at k are the depths in (s(k), s(k+1)], so an
+//                exclusive scan C(k) of their number gives every pre-order
+//                slot: particle k -> k + C(k+1); the depth-d cell starting at
+//                k -> k + C(k) + (d - s(k) - 1).
+//   4. emit      particle nodes, sorted_src, m*p products, leaf records.
+//   5. leafsum   leaf sums in ascending particle index (the canonical order of
+//                the reference's float atomics, InitShader.comp:107-109).
+//   6. tile      one workgroup per depth-5 cell: the 32x32 leaf tile is
+//                reduced in LDS level by level with ((c0+c1)+c2)+c3 exactly as
+//                TreeInitShader.comp:78-83, counts y = sum + [sum > 1]; cells
+//                with y > 1 are written as nodes (TreeMoveShader.comp:60-77).
+//   7. top       one workgroup reduces the 1024 depth-5 cells to the root.
+//   8. traverse  groups of W sorted slots walk one path; a node is accepted
+//                iff all W lanes accept it (ForceShader.comp:111-141), then
+//                integrate (ForceShader.comp:145-150).
+#include "internal.hpp"
+
+namespace gsim {
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 8;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+
+__device__ __forceinline__ uint32_t part1by1(uint32_t v) {
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// (p / S + 1) * 0.5, the normalised coordinate of InitShader.comp:80.
+// Division is IEEE (no reciprocal contraction); the file is built with
+// -ffp-contract=off, so each operation rounds as written.
+__device__ __forceinline__ float unit_coord(float p) { return ((p / kSimSize) + 1.0f) * 0.5f; }
+
+__device__ __forceinline__ uint32_t cell20(float x, float y) {
+    const float qx = unit_coord(x) * (float)kTreeSize;
+    const float qy = unit_coord(y) * (float)kTreeSize;
+    if (qx >= 0.0f && qx < (float)kTreeSize && qy >= 0.0f && qy < (float)kTreeSize)
+        return part1by1((uint32_t)qx) | (part1by1((uint32_t)qy) << 1);
+    return 0xffffffffu;
+}
+
+// Depth of the deepest cell shared by two in-domain cells (0..10).
+__device__ __forceinline__ int shared_depth(uint32_t a, uint32_t b) {
+    const uint32_t x = a ^ b;
+    if (x == 0) return 10;
+    return 9 - ((31 - __clz(x)) >> 1);
+}
+
+__device__ __forceinline__ float cell_radius(int d) {
+    // TreeMoveShader.comp:70-71: S*S*2 / 4^d, exact in FP32
+    return (kSimSize * kSimSize * 2.0f) / (float)(1u << (2 * d));
+}
+
+// ---- 1. keys ----------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bh_keys_kernel(
+    float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
+    const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
+    BhCounters *__restrict__ ctr) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < 2 * kTiles) tile_range[i] = 0;
+    if (i == 0) {
+        ctr->visits = 0;
+        ctr->accepts = 0;
+    }
+    if (i >= na) return;
+    const float2 p = pos[i];
+    uint32_t c = cell20(p.x, p.y);
+    if (c == 0xffffffffu) {
+        // InitShader.comp:110-117; the reference leaves vel_out stale, here it
+        // keeps the input velocity.
+        const float k = -kSimSize * 2.0f;
+        pos[i] = make_float2(k, k);
+        pos_out[i] = make_float2(k, k);
+        mass[i] = 0.0f;
+        vel_out[i] = vel[i];
+        c = kKilledKey;
+    }
+    keys[i] = c;
+    idx[i] = i;
+}
+
+// ---- 2. stable LSD radix sort ------------------------------------------------
+
+__global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
+    uint32_t *__restrict__ hist, uint32_t nblk) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of one array per workgroup (blockIdx.x selects a or b), in
+// place; each thread scans a contiguous chunk.
+__global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t *__restrict__ a,
+                                                               uint32_t *__restrict__ b,
+                                                               uint32_t count) {
+    __shared__ uint32_t part[1024];
+    uint32_t *data = blockIdx.x == 0 ? a : b;
+    if (!data) return;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t lo = min(count, t * per), hi = min(count, lo + per);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += data[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
+    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
+    uint32_t shift, const uint32_t *__restrict__ hist, uint32_t nblk) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[kSortThreads / 64][256];
+    __shared__ uint32_t base_off[256];
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    run[t] = 0;
+#pragma unroll
+    for (int w = 0; w < kSortThreads / 64; ++w) wcnt[w][t] = 0;
+    base_off[t] = hist[t * nblk + blockIdx.x];
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = blockIdx.x * kSortTile + r * kSortThreads + t;
+        const bool valid = i < n;
+        const uint32_t key = valid ? keys_in[i] : 0u;
+        const uint32_t val = valid ? idx_in[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        // lanes of this wave holding the same digit
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(set);
+            m &= set ? bb : ~bb;
+        }
+        const uint32_t rank = (uint32_t)__popcll(m & below);
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = base_off[d] + run[d] + rank;
+            for (uint32_t w = 0; w < wave; ++w) off += wcnt[w][d];
+            keys_out[off] = key;
+            idx_out[off] = val;
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w = 0; w < kSortThreads / 64; ++w) {
+            tot += wcnt[w][t];
+            wcnt[w][t] = 0;
+        }
+        run[t] += tot;
+        __syncthreads();
+    }
+}
+
+// ---- 3./4. slot scan and emission --------------------------------------------
+
+// Per sorted position k: s(k), s(k+1) and the two scanned quantities.
+struct PosInfo {
+    int sk, sk1;
+    uint32_t ns, lf;
+    bool inside;
+    uint32_t key;
+};
+
+__device__ __forceinline__ PosInfo pos_info(const uint32_t *__restrict__ keys, uint32_t na,
+                                            uint32_t k) {
+    PosInfo q{-1, -1, 0u, 0u, false, 0u};
+    if (k >= na) return q;
+    q.key = keys[k];
+    if (q.key >= kKilledKey) return q;
+    q.inside = true;
+    q.sk = k == 0 ? -1 : shared_depth(keys[k - 1], q.key);
+    if (k + 1 < na) {
+        const uint32_t nk = keys[k + 1];
+        if (nk < kKilledKey) q.sk1 = shared_depth(q.key, nk);
+    }
+    q.ns = q.sk1 > q.sk ? (uint32_t)(q.sk1 - q.sk) : 0u;
+    q.lf = q.sk < 10 ? 1u : 0u;
+    return q;
+}
+
+// Inclusive wave scan of two counters.
+__device__ __forceinline__ void wave_scan2(uint32_t &a, uint32_t &b) {
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t ua = __shfl_up(a, off, 64);
+        const uint32_t ub = __shfl_up(b, off, 64);
+        if (lane >= (uint32_t)off) {
+            a += ua;
+            b += ub;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict__ keys,
+                                                      uint32_t na,
+                                                      uint32_t *__restrict__ block_cells,
+                                                      uint32_t *__restrict__ block_leaves) {
+    __shared__ uint32_t wa[4], wb[4];
+    const PosInfo q = pos_info(keys, na, blockIdx.x * 256 + threadIdx.x);
+    uint32_t a = q.ns, b = q.lf;
+    wave_scan2(a, b);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 63) {
+        wa[wave] = a;
+        wb[wave] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        block_cells[blockIdx.x] = wa[0] + wa[1] + wa[2] + wa[3];
+        block_leaves[blockIdx.x] = wb[0] + wb[1] + wb[2] + wb[3];
+    }
+}
+
+__global__ __launch_bounds__(256) void bh_emit_kernel(
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ sidx,
+    const float2 *__restrict__ pos, const float *__restrict__ mass, uint32_t na,
+    const uint32_t *__restrict__ block_cells, const uint32_t *__restrict__ block_leaves,
+    int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
+    uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
+    float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
+    uint32_t *__restrict__ sorted_src, BhCounters *__restrict__ ctr) {
+    __shared__ uint32_t wa[4], wb[4];
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    const PosInfo q = pos_info(keys, na, k);
+    uint32_t a = q.ns, b = q.lf;
+    wave_scan2(a, b);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 63) {
+        wa[wave] = a;
+        wb[wave] = b;
+    }
+    __syncthreads();
+    uint32_t ea = a - q.ns + block_cells[blockIdx.x];
+    uint32_t eb = b - q.lf + block_leaves[blockIdx.x];
+    for (uint32_t w = 0; w < wave; ++w) {
+        ea += wa[w];
+        eb += wb[w];
+    }
+    if (k >= na) return;
+    if (!q.inside) {
+        sorted_src[k] = na;  // ClearShader.comp:81-82 sentinel
+        if (k == 0) {        // nothing inside the domain
+            ctr->n_in = 0;
+            ctr->n_cells = 0;
+            ctr->n_nodes = 0;
+            ctr->n_leaves = 0;
+            leaf_first[0] = 0;
+        }
+        return;
+    }
+    const uint32_t C = ea, L = eb;
+    const uint32_t i = sidx[k];
+    const float2 p = pos[i];
+    const float m = mass[i];
+    const uint32_t slot = k + C + q.ns;
+    // ParticleSortShader.comp:99-107 (canonical order)
+    node4[slot] = make_float4(p.x, p.y, m, 0.0f);
+    ncount[slot] = 1u;
+    sorted_src[k] = i;
+    sdep[k] = (int8_t)q.sk;
+    cpos[k] = C;
+    prod[k] = make_float4(p.x * m, p.y * m, m, 0.0f);  // pos *= mass, InitShader.comp:102
+    if (q.lf) {
+        leaf_first[L] = k;
+        leaf_key[L] = q.key;
+    }
+    if (q.sk1 < 0) {  // last in-domain position
+        ctr->n_in = k + 1;
+        ctr->n_cells = C + q.ns;
+        ctr->n_nodes = k + 1 + C + q.ns;
+        ctr->n_leaves = L + q.lf;
+        leaf_first[L + q.lf] = k + 1;
+    }
+}
+
+// ---- 5. leaf sums -------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bh_leafsum_kernel(
+    const uint32_t *__restrict__ leaf_first, const uint32_t *__restrict__ leaf_key,
+    const float4 *__restrict__ prod, const BhCounters *__restrict__ ctr,
+    float4 *__restrict__ leaf_sum, uint32_t *__restrict__ tile_range) {
+    const uint32_t L = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t nl = ctr->n_leaves;
+    if (L >= nl) return;
+    const uint32_t n_in = ctr->n_in;
+    const uint32_t e = min(leaf_first[L + 1], n_in);
+    const uint32_t f = min(leaf_first[L], e);
+    float sx = 0.0f, sy = 0.0f, sm = 0.0f;
+#pragma unroll 8
+    for (uint32_t j = f; j < e; ++j) {
+        const float4 v = prod[j];
+        sx = sx + v.x;
+        sy = sy + v.y;
+        sm = sm + v.z;
+    }
+    leaf_sum[L] = make_float4(sx, sy, sm, __uint_as_float(e - f));
+    const uint32_t T = leaf_key[L] >> 10;
+    if (L == 0 || (leaf_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
+    if (L + 1 == nl || (leaf_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
+}
+
+// ---- 6./7. dense reduction per tile, then the top ------------------------------
+
+__device__ __forceinline__ TileCell reduce4(const TileCell &a, const TileCell &b,
+                                            const TileCell &c, const TileCell &d) {
+    TileCell r;
+    r.n = a.n + b.n + c.n + d.n;
+    r.y = a.y + b.y + c.y + d.y;
+    r.y += r.y > 1u ? 1u : 0u;  // TreeInitShader.comp:83
+    r.f = min(min(a.f, b.f), min(c.f, d.f));
+    r.pad = 0;
+    r.sx = ((a.sx + b.sx) + c.sx) + d.sx;  // TreeInitShader.comp:79-80
+    r.sy = ((a.sy + b.sy) + c.sy) + d.sy;
+    r.sm = ((a.sm + b.sm) + c.sm) + d.sm;
+    r.pad2 = 0.0f;
+    return r;
+}
+
+__device__ __forceinline__ void emit_cell(const TileCell &c, int d,
+                                          const int8_t *__restrict__ sdep,
+                                          const uint32_t *__restrict__ cpos,
+                                          float4 *__restrict__ node4,
+                                          uint32_t *__restrict__ ncount) {
+    if (c.y <= 1u) return;
+    const uint32_t f = c.f;
+    const uint32_t slot = f + cpos[f] + (uint32_t)(d - (int)sdep[f] - 1);
+    float px = c.sx, py = c.sy;
+    if (c.sm != 0.0f) {  // TreeMoveShader.comp:66-67
+        px = px / c.sm;
+        py = py / c.sm;
+    }
+    ncount[slot] = c.y;
+    node4[slot] = make_float4(px, py, c.sm, cell_radius(d));
+}
+
+// Level offsets inside `up`: depth 9 at 0 (256 cells), 8 at 256 (64),
+// 7 at 320 (16), 6 at 336 (4), 5 at 340 (1).
+__device__ __forceinline__ int tile_level_offset(int d) {
+    return d == 9 ? 0 : d == 8 ? 256 : d == 7 ? 320 : d == 6 ? 336 : 340;
+}
+
+__global__ __launch_bounds__(256) void bh_tile_kernel(
+    const uint32_t *__restrict__ tile_range, const uint32_t *__restrict__ leaf_first,
+    const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
+    const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
+    float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top) {
+    __shared__ TileCell l10[1024];
+    __shared__ TileCell up[341];
+    const uint32_t T = blockIdx.x, t = threadIdx.x;
+    const uint32_t lb = tile_range[2 * T], le = tile_range[2 * T + 1];
+    if (lb == le) {
+        if (t == 0) top[T] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
+        return;
+    }
+    for (uint32_t c = t; c < 1024; c += 256)
+        l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
+    __syncthreads();
+    const float r10 = cell_radius(10);
+    for (uint32_t L = lb + t; L < le; L += 256) {
+        const uint32_t c = leaf_key[L] & 1023u;
+        const float4 s = leaf_sum[L];
+        const uint32_t n = __float_as_uint(s.w);
+        TileCell cell{n, n + (n >= 2u ? 1u : 0u), leaf_first[L], 0u, s.x, s.y, s.z, 0.0f};
+        l10[c] = cell;
+        (void)r10;
+        emit_cell(cell, 10, sdep, cpos, node4, ncount);
+    }
+    __syncthreads();
+    {
+        const uint32_t c = t;  // 256 depth-9 cells
+        up[c] = reduce4(l10[4 * c], l10[4 * c + 1], l10[4 * c + 2], l10[4 * c + 3]);
+    }
+    __syncthreads();
+    for (int d = 8; d >= 5; --d) {
+        const uint32_t cnt = 1u << (2 * (d - 5));
+        const int o = tile_level_offset(d), oc = tile_level_offset(d + 1);
+        if (t < cnt)
+            up[o + t] = reduce4(up[oc + 4 * t], up[oc + 4 * t + 1], up[oc + 4 * t + 2],
+                                up[oc + 4 * t + 3]);
+        __syncthreads();
+    }
+    for (uint32_t c = t; c < 341; c += 256) {
+        const int d = c < 256 ? 9 : c < 320 ? 8 : c < 336 ? 7 : c < 340 ? 6 : 5;
+        emit_cell(up[c], d, sdep, cpos, node4, ncount);
+    }
+    if (t == 0) top[T] = up[340];
+}
+
+// Level offsets inside the top workgroup's LDS: depth 5 at 0 (1024 cells),
+// 4 at 1024 (256), 3 at 1280 (64), 2 at 1344 (16), 1 at 1360 (4), 0 at 1364.
+__device__ __forceinline__ int top_level_offset(int d) {
+    return d == 5 ? 0 : d == 4 ? 1024 : d == 3 ? 1280 : d == 2 ? 1344 : d == 1 ? 1360 : 1364;
+}
+
+__global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict__ top,
+                                                      const int8_t *__restrict__ sdep,
+                                                      const uint32_t *__restrict__ cpos,
+                                                      float4 *__restrict__ node4,
+                                                      uint32_t *__restrict__ ncount) {
+    __shared__ TileCell lv[1365];
+    const uint32_t t = threadIdx.x;
+    lv[t] = top[t];
+    __syncthreads();
+    for (int d = 4; d >= 0; --d) {
+        const uint32_t cnt = 1u << (2 * d);
+        const int o = top_level_offset(d), oc = top_level_offset(d + 1);
+        if (t < cnt)
+            lv[o + t] = reduce4(lv[oc + 4 * t], lv[oc + 4 * t + 1], lv[oc + 4 * t + 2],
+                                lv[oc + 4 * t + 3]);
+        __syncthreads();
+    }
+    if (t < 341) {
+        const uint32_t c = 1024 + t;
+        const int d = c < 1280 ? 4 : c < 1344 ? 3 : c < 1360 ? 2 : c < 1364 ? 1 : 0;
+        emit_cell(lv[c], d, sdep, cpos, node4, ncount);
+    }
+}
+
+// ---- 8. traversal + integrate ----------------------------------------------------
+
+// W = 64: one wave is one group, so the node index is wave-uniform and node
+// records arrive through scalar loads.
+template <bool kDebugAccel>
+__global__ __launch_bounds__(256) void bh_traverse64_kernel(
+    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, BhCounters *__restrict__ ctr,
+    float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
+    uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp) {
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if ((slot & ~63u) >= na) return;  // whole wave beyond the last group
+    const uint32_t src = sorted_src[slot];
+    const bool real = src < na;  // ForceShader.comp:80-92
+    float px = 0.0f, py = 0.0f;
+    if (real) {
+        const float2 p = pos[src];
+        px = p.x;
+        py = p.y;
+    }
+    // node count is bounded by the node arrays; every iteration advances ind by
+    // at least one, so the walk always ends
+    const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
+    const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t ind = 0;
+    unsigned long long vis = 0, acc = 0;
+    while (ind < total) {
+        const float4 nd = node4[ind];
+        const uint32_t cnt = ncount[ind];
+        const float dx = nd.x - px;
+        const float dy = nd.y - py;
+        const float r2 = (dx * dx + dy * dy) + eps2;
+        const bool ok = (nd.w <= r2 * theta2) || !real;
+        vis += nreal;
+        if (__ballot(ok) == ~0ull) {
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float s = ((nd.z * inv) * inv) * inv;
+            ax = ax + dx * s;
+            ay = ay + dy * s;
+            ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
+            acc += nreal;
+        } else {
+            ind = __builtin_amdgcn_readfirstlane(ind + 1u);
+        }
+    }
+    if (real) {
+        const float2 v = vel[src];
+        const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
+        pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+        vel_out[src] = make_float2(nvx, nvy);
+        if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+    }
+    if ((threadIdx.x & 63) == 0 && nreal) {
+        atomicAdd(&ctr->visits, vis);
+        atomicAdd(&ctr->accepts, acc);
+    }
+}
+
+// W = 32: each half-wave is one group with its own node index (the published
+// runs' subgroup width).  Both halves advance together; a finished half idles.
+template <bool kDebugAccel>
+__global__ __launch_bounds__(256) void bh_traverse32_kernel(
+    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, BhCounters *__restrict__ ctr,
+    float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
+    uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp) {
+    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
+    if ((slot & ~63u) >= na) return;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t hm = lane < 32 ? 0x00000000ffffffffull : 0xffffffff00000000ull;
+    const uint32_t src = sorted_src[slot];
+    const bool real = src < na;
+    float px = 0.0f, py = 0.0f;
+    if (real) {
+        const float2 p = pos[src];
+        px = p.x;
+        py = p.y;
+    }
+    const uint32_t total = min(ctr->n_nodes, cap);
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t ind = 0;
+    unsigned long long vis = 0, acc = 0;
+    while (true) {
+        const bool active = ind < total;
+        if (__ballot(active) == 0ull) break;
+        float4 nd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        uint32_t cnt = 1u;
+        if (active) {
+            nd = node4[ind];
+            cnt = ncount[ind];
+        }
+        const float dx = nd.x - px;
+        const float dy = nd.y - py;
+        const float r2 = (dx * dx + dy * dy) + eps2;
+        const bool ok = !active || !real || (nd.w <= r2 * theta2);
+        const bool all = (__ballot(ok) & hm) == hm;
+        if (active) {
+            vis += real ? 1u : 0u;
+            if (all) {
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float s = ((nd.z * inv) * inv) * inv;
+                ax = ax + dx * s;
+                ay = ay + dy * s;
+                ind += max(cnt, 1u);
+                acc += real ? 1u : 0u;
+            } else {
+                ind += 1u;
+            }
+        }
+    }
+    if (real) {
+        const float2 v = vel[src];
+        const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
+        pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+        vel_out[src] = make_float2(nvx, nvy);
+        if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        vis += __shfl_xor(vis, off, 64);
+        acc += __shfl_xor(acc, off, 64);
+    }
+    if (lane == 0 && vis) {
+        atomicAdd(&ctr->visits, vis);
+        atomicAdd(&ctr->accepts, acc);
+    }
+}
+
+// 30-bit Morton codes (15 bits per axis) of the same normalised coordinate;
+// code >> 10 equals the 20-bit cell because scaling by 32 is exact.
+__global__ __launch_bounds__(256) void bh_morton30_kernel(const float2 *__restrict__ pos,
+                                                          uint32_t na, uint32_t *__restrict__ out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= na) return;
+    const float2 p = pos[i];
+    const float qx = unit_coord(p.x) * 32768.0f, qy = unit_coord(p.y) * 32768.0f;
+    uint32_t c = 0xffffffffu;
+    if (qx >= 0.0f && qx < 32768.0f && qy >= 0.0f && qy < 32768.0f)
+        c = part1by1((uint32_t)qx) | (part1by1((uint32_t)qy) << 1);
+    out[i] = c;
+}
+
+}  // namespace
+
+size_t bh_capacity(size_t na) {
+    // every particle plus at most min(4^d, na/2) cells with >= 2 particles per level
+    size_t cells = 0;
+    for (int d = 0; d <= (int)kTreeDepth; ++d) {
+        const size_t lvl = (size_t)1 << (2 * d);
+        cells += lvl < na / 2 ? lvl : na / 2;
+    }
+    return na + cells + 64;
+}
+
+hipError_t bh_alloc(BhBuffers *b, size_t na) {
+    b->na = na;
+    b->cap = bh_capacity(na);
+    b->sort_blocks = (na + kSortTile - 1) / kSortTile;
+    b->prep_blocks = (na + 255) / 256;
+    hipError_t e = hipSuccess;
+#define GSIM_ALLOC(ptr, count)                                                  \
+    do {                                                                        \
+        if (e == hipSuccess) e = hipMalloc((void **)&(ptr), sizeof(*(ptr)) * (count)); \
+    } while (0)
+    for (int k = 0; k < 3; ++k) {
+        GSIM_ALLOC(b->keys[k], na);
+        GSIM_ALLOC(b->idx[k], na);
+    }
+    GSIM_ALLOC(b->hist, 256 * b->sort_blocks);
+    GSIM_ALLOC(b->block_cells, b->prep_blocks);
+    GSIM_ALLOC(b->block_leaves, b->prep_blocks);
+    GSIM_ALLOC(b->sdep, na + 1);
+    GSIM_ALLOC(b->cpos, na);
+    GSIM_ALLOC(b->prod, na);
+    GSIM_ALLOC(b->leaf_first, na + 1);
+    GSIM_ALLOC(b->leaf_key, na);
+    GSIM_ALLOC(b->leaf_sum, na);
+    GSIM_ALLOC(b->tile_range, 2 * kTiles);
+    GSIM_ALLOC(b->top, kTiles);
+    GSIM_ALLOC(b->node4, b->cap);
+    GSIM_ALLOC(b->ncount, b->cap);
+    GSIM_ALLOC(b->sorted_src, na);
+    GSIM_ALLOC(b->ctr, 1);
+#undef GSIM_ALLOC
+    if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
+    if (e == hipSuccess) e = hipMemset(b->ncount, 0, sizeof(uint32_t) * b->cap);
+    if (e == hipSuccess) e = hipMemset(b->node4, 0, sizeof(float4) * b->cap);
+    return e;
+}
+
+void bh_free(BhBuffers *b) {
+    for (int k = 0; k < 3; ++k) {
+        (void)hipFree(b->keys[k]);
+        (void)hipFree(b->idx[k]);
+    }
+    void *ptrs[] = {b->hist, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
+                    b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
+                    b->node4, b->ncount, b->sorted_src, b->ctr};
+    for (void *p : ptrs) (void)hipFree(p);
+    *b = BhBuffers{};
+}
+
+hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_out,
+                           const float2 *vel, float2 *vel_out) {
+    BhBuffers &b = s->bh;
+    const uint32_t na = (uint32_t)s->na;
+    const uint32_t g256 = (na + 255) / 256;
+    const uint32_t gkeys = g256 > (2 * kTiles + 255) / 256 ? g256 : (2 * kTiles + 255) / 256;
+    bh_keys_kernel<<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
+                                                 b.keys[0], b.idx[0], b.tile_range, b.ctr);
+    // keys[0] -> keys[1] -> keys[2] -> keys[1]; keys[0] keeps the per-particle cells
+    const int in_of[3] = {0, 1, 2}, out_of[3] = {1, 2, 1};
+    const uint32_t nblk = (uint32_t)b.sort_blocks;
+    for (int pass = 0; pass < 3; ++pass) {
+        const uint32_t shift = 8u * pass;
+        const int i = in_of[pass], o = out_of[pass];
+        radix_upsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(b.keys[i], na, shift, b.hist, nblk);
+        scan_exclusive_kernel<<<1, 1024, 0, s->stream>>>(b.hist, nullptr, 256 * nblk);
+        radix_downsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
+            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, shift, b.hist, nblk);
+    }
+    b.sorted_slot = 1;
+    const uint32_t *sk = b.keys[1], *si = b.idx[1];
+    const uint32_t pb = (uint32_t)b.prep_blocks;
+    bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, b.block_cells, b.block_leaves);
+    scan_exclusive_kernel<<<2, 1024, 0, s->stream>>>(b.block_cells, b.block_leaves, pb);
+    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, na, b.block_cells,
+                                              b.block_leaves, b.sdep, b.cpos, b.prod,
+                                              b.leaf_first, b.leaf_key, b.node4, b.ncount,
+                                              b.sorted_src, b.ctr);
+    bh_leafsum_kernel<<<pb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, b.ctr,
+                                                 b.leaf_sum, b.tile_range);
+    bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
+                                                  b.leaf_sum, b.sdep, b.cpos, b.node4,
+                                                  b.ncount, b.top);
+    bh_top_kernel<<<1, 1024, 0, s->stream>>>(b.top, b.sdep, b.cpos, b.node4, b.ncount);
+    return hipGetLastError();
+}
+
+hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
+                              float2 *pos_out, float2 *vel_out) {
+    BhBuffers &b = s->bh;
+    const uint32_t na = (uint32_t)s->na;
+    const uint32_t grid = (na + 255) / 256;
+    const bool dbg = (s->debug_flags & GSIM_DEBUG_ACCEL) != 0;
+    float2 *acc = dbg ? s->accel : nullptr;
+    if (s->W == 64) {
+        if (dbg)
+            bh_traverse64_kernel<true><<<grid, 256, 0, s->stream>>>(
+                b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+        else
+            bh_traverse64_kernel<false><<<grid, 256, 0, s->stream>>>(
+                b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+    } else {
+        if (dbg)
+            bh_traverse32_kernel<true><<<grid, 256, 0, s->stream>>>(
+                b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+        else
+            bh_traverse32_kernel<false><<<grid, 256, 0, s->stream>>>(
+                b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+    }
+    return hipGetLastError();
+}
+
+hipError_t bh_launch_morton30(gsim_sim *s, const float2 *pos, uint32_t *out) {
+    const uint32_t na = (uint32_t)s->na;
+    bh_morton30_kernel<<<(na + 255) / 256, 256, 0, s->stream>>>(pos, na, out);
+    return hipGetLastError();
+}
+
+}  // namespace gsim

@@ -1,0 +1,123 @@
+// Internal state of one simulation handle and the kernel launchers behind the
+// C ABI.  Device layout (HBM), per handle:
+//   particle state  pos float2[Na] x2 (ping-pong), vel float2[Na] x2, mass f32[Na]
+//                   (mass is only written by the Barnes-Hut kill, so one copy)
+//   direct sum      partial float2[S*Na] per-slice partial accelerations
+//   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
+//                   cpos u32[Na], prod float4[Na], leaf records, tile results,
+//                   node4 float4[cap] {x, y, mass, radius}, ncount u32[cap],
+//                   sorted_src u32[Na], device counters
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.hpp"
+#include "gsim.h"
+
+namespace gsim {
+
+// Fixed domain of the reference: [-500, 500)^2, a 1024^2 leaf grid, depth 10
+// (BarnesHutSimulation.cpp:579,581).
+constexpr float kSimSize = 500.0f;
+constexpr uint32_t kTreeSize = 1024;
+constexpr uint32_t kTreeDepth = 10;
+constexpr uint32_t kKilledKey = 1u << 20;  // sorts after every 20-bit cell
+constexpr uint32_t kTileDepth = 5;          // tile kernel: one depth-5 cell per workgroup
+constexpr uint32_t kTiles = 1u << (2 * kTileDepth);
+
+struct BhCounters {
+    uint32_t n_in;      // particles inside the domain (padding included)
+    uint32_t n_cells;   // cell nodes (cells holding >= 2 particles)
+    uint32_t n_nodes;   // n_in + n_cells = counts[0]
+    uint32_t n_leaves;  // occupied depth-10 cells
+    unsigned long long visits;   // real-lane node visits of the traversal
+    unsigned long long accepts;  // real-lane acceptances
+};
+
+struct TileCell {  // one cell of the dense tile/top reduction
+    uint32_t n, y, f, pad;
+    float sx, sy, sm, pad2;
+};
+
+struct BhBuffers {
+    size_t na = 0, cap = 0, sort_blocks = 0, prep_blocks = 0;
+    uint32_t *keys[3] = {}, *idx[3] = {};
+    uint32_t *hist = nullptr;        // [256 * sort_blocks]
+    uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
+    uint32_t *block_leaves = nullptr;
+    int8_t *sdep = nullptr;          // [Na] shared depth with the previous sorted key
+    uint32_t *cpos = nullptr;        // [Na] cell nodes starting before sorted position k
+    float4 *prod = nullptr;          // [Na] (m*x, m*y, m, -) in sorted order
+    uint32_t *leaf_first = nullptr;  // [Na+1]
+    uint32_t *leaf_key = nullptr;    // [Na]
+    float4 *leaf_sum = nullptr;      // [Na] (sx, sy, sm, n as bits)
+    uint32_t *tile_range = nullptr;  // [2*kTiles] first/end leaf of each depth-5 tile
+    TileCell *top = nullptr;         // [kTiles]
+    float4 *node4 = nullptr;         // [cap]
+    uint32_t *ncount = nullptr;      // [cap]
+    uint32_t *sorted_src = nullptr;  // [Na]
+    BhCounters *ctr = nullptr;
+    uint32_t sorted_slot = 0;        // which keys/idx buffer holds the sorted result
+};
+
+// Events recorded around one step: e[0] start, e[1] between the two phases,
+// e[2] end.  Barnes-Hut: (e0,e1) build, (e1,e2) traversal; direct sum:
+// (e0,e1) force, (e1,e2) combine + integrate.
+struct PhaseEvents {
+    hipEvent_t e[3];
+    bool bh;
+};
+
+}  // namespace gsim
+
+struct gsim_sim {
+    gsim_params params{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t n = 0, na = 0;
+    // FP32 constants squared/multiplied on the host as the reference does
+    // (DirectSimulation.cpp:179, BarnesHutSimulation.cpp:575-578).
+    float eps2 = 0, theta2 = 0, dt = 0, kv = 0, kp = 0;
+    uint32_t W = 64;
+    // state
+    float2 *pos[2] = {}, *vel[2] = {};
+    float *mass = nullptr;
+    int cur = 0;  // index of the input set
+    // direct sum
+    float2 *partial = nullptr;
+    uint32_t ds_R = 1, ds_S = 1, ds_S_cap = 0, ds_S_used = 1;
+    // Barnes-Hut
+    gsim::BhBuffers bh;
+    // debug / timing
+    float2 *accel = nullptr;
+    uint32_t debug_flags = 0;
+    bool timing = false;
+    std::vector<gsim::PhaseEvents> pending;
+    std::vector<hipEvent_t> event_pool;
+    gsim_timing acc{};
+    uint64_t steps = 0;
+    bool has_tree = false;
+};
+
+namespace gsim {
+
+// Direct sum (ds_kernels.hip)
+void ds_pick_shape(size_t na, uint32_t *R, uint32_t *S);
+hipError_t ds_launch_partial(gsim_sim *s, const float2 *pos, const float *mass);
+hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
+                             float2 *pos_out, float2 *vel_out);
+
+// Barnes-Hut (bh_kernels.hip)
+size_t bh_capacity(size_t na);
+hipError_t bh_alloc(BhBuffers *b, size_t na);
+void bh_free(BhBuffers *b);
+hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_out,
+                           const float2 *vel, float2 *vel_out);
+hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
+                              float2 *pos_out, float2 *vel_out);
+hipError_t bh_launch_morton30(gsim_sim *s, const float2 *pos, uint32_t *out);
+
+}  // namespace gsim

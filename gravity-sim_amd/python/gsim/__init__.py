@@ -1,0 +1,387 @@
+"""Python binding of libgsim (the MI355X force-and-integrate library) over its
+C ABI (include/gsim.h), with the reference's object surface:
+
+    ParticleSystem     particle set + parameters (src/Particles/ParticleSystem.hpp:58,72)
+    DirectSimulation   RunSimulations / GetRequiredParticleAlignment
+                       (src/Simulation/Direct/DirectSimulation.hpp:15-24)
+    BarnesHutSimulation  the same for --simulation-algorithm=barnes-hut
+                       (src/Simulation/BarnesHut/BarnesHutSimulation.hpp:15-24)
+
+Errors raise GsimError carrying the library's message, as the reference throws
+gsim::Exception (src/Debug/Exception.hpp:72).  There is no CPU fallback: if
+libgsim.so is missing or no gfx950 device is visible, the call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_PKG_ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+LIB_PATH = os.path.join(_PKG_ROOT, "lib", "libgsim.so")
+
+DIRECT_SUM = 0
+BARNES_HUT = 1
+
+GENERATE_RANDOM = 0
+GENERATE_GALAXY = 1
+GENERATE_GALAXY_COLLISION = 2
+GENERATE_SYMMETRICAL_GALAXY_COLLISION = 3
+GENERATE_EXPONENTIAL_DISK = 16
+GENERATE_TWO_EXPONENTIAL_DISKS = 17
+
+GENERATE_TYPES = {
+    "random": GENERATE_RANDOM,
+    "galaxy": GENERATE_GALAXY,
+    "galaxy-collision": GENERATE_GALAXY_COLLISION,
+    "symmetrical-galaxy-collision": GENERATE_SYMMETRICAL_GALAXY_COLLISION,
+    "exponential-disk": GENERATE_EXPONENTIAL_DISK,
+    "two-exponential-disks": GENERATE_TWO_EXPONENTIAL_DISKS,
+}
+ALGORITHMS = {"direct-sum": DIRECT_SUM, "barnes-hut": BARNES_HUT}
+
+DUMP_CELLS, DUMP_MORTON30, DUMP_SORTED_SRC = 1, 2, 3
+DUMP_COUNTS, DUMP_RADII, DUMP_NODE_POS, DUMP_NODE_MASS = 4, 5, 6, 7
+DUMP_NODE_TOTAL, DUMP_ACCEL, DUMP_COUNTERS = 8, 9, 10
+DEBUG_ACCEL = 1
+
+STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
+          4: "NO_DEVICE", 5: "IO", 6: "CAPACITY"}
+
+
+class GsimError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS.get(status, status)}: {message}")
+        self.status = status
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("gravitational_const", C.c_float),
+        ("simulation_time", C.c_float),
+        ("simulation_speed", C.c_float),
+        ("softening_len", C.c_float),
+        ("accuracy_parameter", C.c_float),
+        ("algorithm", C.c_int32),
+        ("group_width", C.c_uint32),
+        ("device", C.c_int32),
+    ]
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("force_ms", C.c_double),
+        ("build_ms", C.c_double),
+        ("integrate_ms", C.c_double),
+        ("step_ms", C.c_double),
+        ("steps", C.c_uint64),
+        ("force_launches", C.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Loads libgsim.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise GsimError(-1, f"{path} not found: build it with python gravity-sim_amd/build.py")
+    lib = C.CDLL(path)
+    fp = C.POINTER(C.c_float)
+    vp = C.c_void_p
+    st = C.c_int
+    sig = {
+        "gsim_default_params": (None, [C.POINTER(Params)]),
+        "gsim_last_error": (C.c_char_p, []),
+        "gsim_version": (C.c_char_p, []),
+        "gsim_required_alignment": (C.c_size_t, [C.c_int32]),
+        "gsim_create": (st, [C.POINTER(Params), fp, fp, fp, C.c_size_t, C.POINTER(vp)]),
+        "gsim_step": (st, [vp, C.c_uint32]),
+        "gsim_synchronize": (st, [vp]),
+        "gsim_read": (st, [vp, fp, fp, fp]),
+        "gsim_destroy": (st, [vp]),
+        "gsim_particle_count": (C.c_size_t, [vp]),
+        "gsim_aligned_particle_count": (C.c_size_t, [vp]),
+        "gsim_steps_done": (C.c_uint64, [vp]),
+        "gsim_debug_dump": (st, [vp, C.c_int32, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "gsim_set_debug": (st, [vp, C.c_uint32]),
+        "gsim_set_timing": (st, [vp, C.c_int32]),
+        "gsim_get_timing": (st, [vp, C.POINTER(Timing), C.c_int32]),
+        "gsim_set_ds_shape": (st, [vp, C.c_uint32, C.c_uint32]),
+        "gsim_generate": (st, [C.c_int32, C.c_size_t, C.c_float, C.c_float, C.c_float,
+                               C.c_float, C.c_uint64, fp, fp, fp]),
+        "gsim_count_particles": (st, [C.c_char_p, C.POINTER(C.c_size_t)]),
+        "gsim_load_particles": (st, [C.c_char_p, fp, fp, fp, C.c_size_t,
+                                     C.POINTER(C.c_size_t)]),
+        "gsim_save_particles": (st, [C.c_char_p, fp, fp, fp, C.c_size_t]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(status: int) -> None:
+    if status != 0:
+        msg = _lib.gsim_last_error().decode(errors="replace") if _lib else ""
+        raise GsimError(status, msg)
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def default_params() -> Params:
+    p = Params()
+    load_library().gsim_default_params(C.byref(p))
+    return p
+
+
+def required_alignment(algorithm: int) -> int:
+    return int(load_library().gsim_required_alignment(algorithm))
+
+
+@dataclass
+class Particles:
+    """Host particle set: pos (n,2), vel (n,2), mass (n,), all float32."""
+    pos: np.ndarray
+    vel: np.ndarray
+    mass: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.mass.shape[0])
+
+    def copy(self) -> "Particles":
+        return Particles(self.pos.copy(), self.vel.copy(), self.mass.copy())
+
+    def padded(self, alignment: int = 64) -> "Particles":
+        na = (self.n + alignment - 1) // alignment * alignment
+        p = np.zeros((na, 2), np.float32)
+        v = np.zeros((na, 2), np.float32)
+        m = np.zeros(na, np.float32)
+        p[: self.n] = self.pos
+        v[: self.n] = self.vel
+        m[: self.n] = self.mass
+        return Particles(p, v, m)
+
+
+def generate(kind, n: int, generate_size: float, min_mass: float, max_mass: float,
+             gravitational_const: float = 1.0, seed: int = 0) -> Particles:
+    """One of the reference presets (or a synthetic disk), seeded."""
+    lib = load_library()
+    t = GENERATE_TYPES[kind] if isinstance(kind, str) else int(kind)
+    pos = np.empty((n, 2), np.float32)
+    vel = np.empty((n, 2), np.float32)
+    mass = np.empty(n, np.float32)
+    _check(lib.gsim_generate(t, n, generate_size, min_mass, max_mass, gravitational_const,
+                             seed, _fptr(pos), _fptr(vel), _fptr(mass)))
+    return Particles(pos, vel, mass)
+
+
+def load_particles(path: str) -> Particles:
+    lib = load_library()
+    n = C.c_size_t(0)
+    _check(lib.gsim_count_particles(path.encode(), C.byref(n)))
+    cnt = n.value
+    pos = np.empty((cnt, 2), np.float32)
+    vel = np.empty((cnt, 2), np.float32)
+    mass = np.empty(cnt, np.float32)
+    _check(lib.gsim_load_particles(path.encode(), _fptr(pos), _fptr(vel), _fptr(mass), cnt,
+                                   C.byref(n)))
+    return Particles(pos, vel, mass)
+
+
+def save_particles(path: str, p: Particles) -> None:
+    lib = load_library()
+    pos = np.ascontiguousarray(p.pos, np.float32)
+    vel = np.ascontiguousarray(p.vel, np.float32)
+    mass = np.ascontiguousarray(p.mass, np.float32)
+    _check(lib.gsim_save_particles(path.encode(), _fptr(pos), _fptr(vel), _fptr(mass), p.n))
+
+
+class Simulation:
+    """One device-resident simulation (a gsim_sim handle)."""
+
+    def __init__(self, particles: Particles, algorithm: int | str = DIRECT_SUM, *,
+                 gravitational_const: float = 1.0, simulation_time: float = 1e-3,
+                 simulation_speed: float = 1.0, softening_len: float = 0.2,
+                 accuracy_parameter: float = 1.0, group_width: int = 64, device: int = 0):
+        lib = load_library()
+        p = Params()
+        lib.gsim_default_params(C.byref(p))
+        p.gravitational_const = gravitational_const
+        p.simulation_time = simulation_time
+        p.simulation_speed = simulation_speed
+        p.softening_len = softening_len
+        p.accuracy_parameter = accuracy_parameter
+        p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        p.group_width = group_width
+        p.device = device
+        self.params = p
+        pos = np.ascontiguousarray(particles.pos, np.float32)
+        vel = np.ascontiguousarray(particles.vel, np.float32)
+        mass = np.ascontiguousarray(particles.mass, np.float32)
+        h = C.c_void_p()
+        _check(lib.gsim_create(C.byref(p), _fptr(pos), _fptr(vel), _fptr(mass),
+                               particles.n, C.byref(h)))
+        self._h = h
+        self.n = particles.n
+        self.na = int(lib.gsim_aligned_particle_count(h))
+
+    # RunSimulations(n): asynchronous, stream ordered
+    def step(self, count: int = 1) -> None:
+        _check(_lib.gsim_step(self._h, count))
+
+    run_simulations = step
+
+    def synchronize(self) -> None:
+        _check(_lib.gsim_synchronize(self._h))
+
+    def read(self) -> Particles:
+        pos = np.empty((self.n, 2), np.float32)
+        vel = np.empty((self.n, 2), np.float32)
+        mass = np.empty(self.n, np.float32)
+        _check(_lib.gsim_read(self._h, _fptr(pos), _fptr(vel), _fptr(mass)))
+        return Particles(pos, vel, mass)
+
+    def set_debug(self, flags: int) -> None:
+        _check(_lib.gsim_set_debug(self._h, flags))
+
+    def set_timing(self, enable: bool) -> None:
+        _check(_lib.gsim_set_timing(self._h, 1 if enable else 0))
+
+    def timing(self, reset: bool = True) -> dict:
+        t = Timing()
+        _check(_lib.gsim_get_timing(self._h, C.byref(t), 1 if reset else 0))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def set_ds_shape(self, targets_per_lane: int, slices: int) -> None:
+        _check(_lib.gsim_set_ds_shape(self._h, targets_per_lane, slices))
+
+    def dump(self, what: int) -> np.ndarray:
+        written = C.c_size_t(0)
+        status = _lib.gsim_debug_dump(self._h, what, None, 0, C.byref(written))
+        if status not in (0, 6):
+            _check(status)
+        nbytes = written.value
+        dtype = {DUMP_RADII: np.float32, DUMP_NODE_POS: np.float32,
+                 DUMP_NODE_MASS: np.float32, DUMP_ACCEL: np.float32,
+                 DUMP_COUNTERS: np.uint64}.get(what, np.uint32)
+        buf = np.zeros(max(nbytes, 1) // np.dtype(dtype).itemsize, dtype)
+        _check(_lib.gsim_debug_dump(self._h, what, buf.ctypes.data_as(C.c_void_p),
+                                    buf.nbytes, C.byref(written)))
+        buf = buf[: written.value // np.dtype(dtype).itemsize]
+        if what in (DUMP_NODE_POS, DUMP_ACCEL):
+            buf = buf.reshape(-1, 2)
+        return buf
+
+    def tree(self) -> dict:
+        """Node arrays of the last Barnes-Hut build in the reference's layout."""
+        return {
+            "counts": self.dump(DUMP_COUNTS),
+            "radii": self.dump(DUMP_RADII),
+            "node_pos": self.dump(DUMP_NODE_POS),
+            "node_mass": self.dump(DUMP_NODE_MASS),
+            "sorted_src": self.dump(DUMP_SORTED_SRC),
+            "total": int(self.dump(DUMP_NODE_TOTAL)[0]),
+        }
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.gsim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class ParticleSystem:
+    """Particle set plus parameters, the reference's ParticleSystem surface
+    (src/Particles/ParticleSystem.hpp:58-123): built either from a particle file
+    or from a generation preset, padded to the algorithm's alignment."""
+
+    def __init__(self, particles: Particles, gravitational_const: float = 1.0,
+                 simulation_time: float = 1e-3, simulation_speed: float = 1.0,
+                 softening_len: float = 0.2, accuracy_parameter: float = 1.0,
+                 simulation_algorithm: int | str = DIRECT_SUM):
+        self.particles = particles
+        self.gravitational_const = gravitational_const
+        self.simulation_time = simulation_time
+        self.simulation_speed = simulation_speed
+        self.softening_len = softening_len
+        self.accuracy_parameter = accuracy_parameter
+        self.simulation_algorithm = (ALGORITHMS[simulation_algorithm]
+                                     if isinstance(simulation_algorithm, str)
+                                     else int(simulation_algorithm))
+        align = required_alignment(self.simulation_algorithm)
+        self.particle_count = particles.n
+        self.aligned_particle_count = (particles.n + align - 1) // align * align
+
+    @classmethod
+    def from_file(cls, path: str, **kw) -> "ParticleSystem":
+        return cls(load_particles(path), **kw)
+
+    @classmethod
+    def generated(cls, count: int, generate_type, generate_size: float, min_mass: float,
+                  max_mass: float, seed: int = 0, **kw) -> "ParticleSystem":
+        t = GENERATE_TYPES[generate_type] if isinstance(generate_type, str) else generate_type
+        if t == GENERATE_SYMMETRICAL_GALAXY_COLLISION:
+            count &= ~1  # ParticleSystem.cpp:522-525
+        g = kw.get("gravitational_const", 1.0)
+        return cls(generate(t, count, generate_size, min_mass, max_mass, g, seed), **kw)
+
+
+class _SimulationBase:
+    algorithm = DIRECT_SUM
+
+    def __init__(self, particle_system: ParticleSystem, group_width: int = 64, device: int = 0):
+        ps = particle_system
+        self.particle_system = ps
+        self.sim = Simulation(ps.particles, self.algorithm,
+                              gravitational_const=ps.gravitational_const,
+                              simulation_time=ps.simulation_time,
+                              simulation_speed=ps.simulation_speed,
+                              softening_len=ps.softening_len,
+                              accuracy_parameter=ps.accuracy_parameter,
+                              group_width=group_width, device=device)
+
+    @staticmethod
+    def get_required_particle_alignment() -> int:
+        return 64
+
+    def run_simulations(self, simulation_count: int) -> None:
+        self.sim.step(simulation_count)
+
+    def get_particles(self) -> Particles:
+        return self.sim.read()
+
+    def save_particles(self, path: str) -> None:
+        save_particles(path, self.sim.read())
+
+    def close(self) -> None:
+        self.sim.close()
+
+
+class DirectSimulation(_SimulationBase):
+    algorithm = DIRECT_SUM
+
+
+class BarnesHutSimulation(_SimulationBase):
+    algorithm = BARNES_HUT

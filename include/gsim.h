@@ -1,0 +1,183 @@
+/*
+ * gsim.h — C ABI of the MI355X force-and-integrate library (libgsim.so).
+ *
+ * Plain pointers and sizes only.  One gsim_sim owns its device state and one
+ * HIP stream; calls on one handle must be serialised by the caller (the
+ * reference is single-threaded too).  Every function returns a gsim_status;
+ * on failure gsim_last_error() holds a thread-local message, the analogue of
+ * the reference's gsim::Exception text (src/Debug/Exception.hpp:10-72).
+ *
+ * Reference surface replaced (file:line in victordobra/Gravity-Sim):
+ *   gsim_required_alignment  DirectSimulation::GetRequiredParticleAlignment
+ *                            (DirectSimulation.cpp:26-28) and the Barnes-Hut one
+ *                            (BarnesHutSimulation.cpp:834-836)
+ *   gsim_create              ParticleSystem ctor upload (ParticleSystem.cpp:438-559,
+ *                            150-407) + DirectSimulation / BarnesHutSimulation ctor
+ *                            (DirectSimulation.hpp:24, BarnesHutSimulation.hpp:24)
+ *   gsim_step                RunSimulations(uint32_t) (DirectSimulation.cpp:271-353,
+ *                            BarnesHutSimulation.cpp:849-952): asynchronous,
+ *                            stream-ordered
+ *   gsim_synchronize         vkDeviceWaitIdle before readback (Main.cpp:286)
+ *   gsim_read                ParticleSystem::GetParticles (ParticleSystem.cpp:561-726)
+ *   gsim_destroy             ~DirectSimulation / ~BarnesHutSimulation / ~ParticleSystem
+ *   gsim_generate            the four generation presets (ParticleSystem.cpp:15-149)
+ *   gsim_load_particles      file constructor parsing (ParticleSystem.cpp:451-484)
+ *   gsim_save_particles      SaveParticles (ParticleSystem.cpp:727-750)
+ */
+#ifndef GSIM_H
+#define GSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSIM_ABI_VERSION 1
+
+typedef enum gsim_status {
+    GSIM_OK = 0,
+    GSIM_ERROR_INVALID_ARGUMENT = 1,
+    GSIM_ERROR_OUT_OF_MEMORY = 2,
+    GSIM_ERROR_DEVICE = 3,       /* a HIP call failed */
+    GSIM_ERROR_NO_DEVICE = 4,    /* no gfx950 device visible */
+    GSIM_ERROR_IO = 5,           /* particle file could not be read/written */
+    GSIM_ERROR_CAPACITY = 6      /* a fixed-size buffer would overflow */
+} gsim_status;
+
+/* --simulation-algorithm (Main.cpp:197-202) */
+typedef enum gsim_algorithm {
+    GSIM_DIRECT_SUM = 0,
+    GSIM_BARNES_HUT = 1
+} gsim_algorithm;
+
+/* --generate-type (Main.cpp:176-185, ParticleSystem.hpp:12-18) */
+typedef enum gsim_generate_type {
+    GSIM_GENERATE_RANDOM = 0,
+    GSIM_GENERATE_GALAXY = 1,
+    GSIM_GENERATE_GALAXY_COLLISION = 2,
+    GSIM_GENERATE_SYMMETRICAL_GALAXY_COLLISION = 3,
+    /* Synthetic benchmark inputs (not reference presets): */
+    GSIM_GENERATE_EXPONENTIAL_DISK = 16,      /* one disk, radius = size, scale = size/5 */
+    GSIM_GENERATE_TWO_EXPONENTIAL_DISKS = 17  /* two approaching disks at +-40 */
+} gsim_generate_type;
+
+/* Simulation parameters; the reference's ProgramInfo defaults (Main.cpp:59-63)
+ * come from gsim_default_params. */
+typedef struct gsim_params {
+    float gravitational_const; /* --gravitational-const, default 1            */
+    float simulation_time;     /* --simulation-time (dt), default 1e-3        */
+    float simulation_speed;    /* --simulation-speed, default 1; Barnes-Hut
+                                  only, as in the reference (SimShader.comp:68) */
+    float softening_len;       /* --softening-len, default 0.2                */
+    float accuracy_parameter;  /* --accuracy-parameter (theta), default 1     */
+    int32_t algorithm;         /* gsim_algorithm                              */
+    uint32_t group_width;      /* Barnes-Hut vote width W: 64 (one wave64) or
+                                  32 (the subgroup of the published runs);
+                                  0 selects 64                                */
+    int32_t device;            /* HIP device ordinal, default 0               */
+} gsim_params;
+
+typedef struct gsim_sim gsim_sim;
+
+/* Debug dump selectors for gsim_debug_dump (state of the most recent step). */
+typedef enum gsim_dump {
+    GSIM_DUMP_CELLS = 1,       /* u32[Na]  20-bit cell per particle, 0xffffffff = killed */
+    GSIM_DUMP_MORTON30 = 2,    /* u32[Na]  30-bit Morton code, 0xffffffff = killed     */
+    GSIM_DUMP_SORTED_SRC = 3,  /* u32[Na]  particle index per sorted slot (Na = empty)  */
+    GSIM_DUMP_COUNTS = 4,      /* u32[nodes] subtree sizes                              */
+    GSIM_DUMP_RADII = 5,       /* f32[nodes]                                            */
+    GSIM_DUMP_NODE_POS = 6,    /* f32[2*nodes]                                          */
+    GSIM_DUMP_NODE_MASS = 7,   /* f32[nodes]                                            */
+    GSIM_DUMP_NODE_TOTAL = 8,  /* u32[1]   node count of the last tree (counts[0])      */
+    GSIM_DUMP_ACCEL = 9,       /* f32[2*Na] accelerations of the last step, by particle
+                                  index (needs gsim_set_debug(sim, GSIM_DEBUG_ACCEL))  */
+    GSIM_DUMP_COUNTERS = 10    /* u64[2]   real-lane node visits and acceptances of
+                                  the last Barnes-Hut step                              */
+} gsim_dump;
+
+#define GSIM_DEBUG_ACCEL 1u
+
+/* Timing phases returned by gsim_get_timing (milliseconds, summed). */
+typedef struct gsim_timing {
+    double force_ms;      /* direct-sum force kernel / Barnes-Hut traversal   */
+    double build_ms;      /* Barnes-Hut: keys, sort and tree build            */
+    double integrate_ms;  /* direct-sum partial-sum combine and integrate     */
+    double step_ms;       /* whole steps                                       */
+    uint64_t steps;       /* steps covered                                     */
+    uint64_t force_launches;
+} gsim_timing;
+
+void gsim_default_params(gsim_params *params);
+const char *gsim_last_error(void);
+const char *gsim_version(void);
+
+/* 64 for both algorithms: N is padded to a multiple with {0,0,0,0,0}
+ * particles (ParticleSystem.cpp:487-491). */
+size_t gsim_required_alignment(int32_t algorithm);
+
+/* Copies n particles in (host arrays: pos_xy[2n], vel_xy[2n], mass[n]) and pads
+ * them to the alignment.  The library owns the device copies. */
+gsim_status gsim_create(const gsim_params *params, const float *pos_xy,
+                        const float *vel_xy, const float *mass, size_t n,
+                        gsim_sim **out);
+
+/* Enqueues count steps on the handle's stream and returns (no host wait). */
+gsim_status gsim_step(gsim_sim *sim, uint32_t count);
+
+/* Waits for every enqueued step. */
+gsim_status gsim_synchronize(gsim_sim *sim);
+
+/* Synchronises, then copies the n unpadded particles out in original index
+ * order.  Any pointer may be NULL to skip that field. */
+gsim_status gsim_read(gsim_sim *sim, float *pos_xy, float *vel_xy, float *mass);
+
+gsim_status gsim_destroy(gsim_sim *sim);
+
+size_t gsim_particle_count(const gsim_sim *sim);
+size_t gsim_aligned_particle_count(const gsim_sim *sim);
+uint64_t gsim_steps_done(const gsim_sim *sim);
+
+/* Debug intermediates for parity tests; synchronises.  *written receives the
+ * number of bytes copied (or required, when bytes is too small). */
+gsim_status gsim_debug_dump(gsim_sim *sim, int32_t what, void *buf,
+                            size_t bytes, size_t *written);
+gsim_status gsim_set_debug(gsim_sim *sim, uint32_t flags);
+
+/* HIP-event timing of each kernel phase on the handle's stream. */
+gsim_status gsim_set_timing(gsim_sim *sim, int32_t enable);
+gsim_status gsim_get_timing(gsim_sim *sim, gsim_timing *out, int32_t reset);
+
+/* Direct-sum launch shape: targets per lane (1, 2 or 4) and source slices
+ * (0 = pick by size).  Results stay within the documented tolerance for every
+ * shape; the partial sums are combined in slice order, so a shape is
+ * deterministic. */
+gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
+                              uint32_t slices);
+
+/* ---- particle sets: presets and the text format --------------------------- */
+
+/* Fills n particles (pos_xy[2n], vel_xy[2n], mass[n]) from a preset with an
+ * explicit seed.  For GSIM_GENERATE_SYMMETRICAL_GALAXY_COLLISION n must be even
+ * (the reference rounds it down, ParticleSystem.cpp:522-525). */
+gsim_status gsim_generate(int32_t type, size_t n, float generate_size,
+                          float min_mass, float max_mass,
+                          float gravitational_const, uint64_t seed,
+                          float *pos_xy, float *vel_xy, float *mass);
+
+/* Counts the particles of a text file ("%f%f%f%f%f" records until the first
+ * failed match), then loads up to capacity of them. */
+gsim_status gsim_count_particles(const char *path, size_t *n);
+gsim_status gsim_load_particles(const char *path, float *pos_xy, float *vel_xy,
+                                float *mass, size_t capacity, size_t *n);
+
+/* Writes "%.7f %.7f %.7f %.7f %.7f\n" per particle. */
+gsim_status gsim_save_particles(const char *path, const float *pos_xy,
+                                const float *vel_xy, const float *mass,
+                                size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

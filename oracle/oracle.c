@@ -1,0 +1,509 @@
+/*
+ * CPU oracle for the force-and-integrate step — TEST INFRASTRUCTURE ONLY
+ * (the checker for tests/, the smoke check and bench.py's CPU baseline; the
+ * product never loads it).  Compiled with -ffp-contract=off so every FP32
+ * operation rounds exactly as written; no -ffast-math.
+ *
+ * Parity status of each function (how it is pinned is in tests/):
+ *   oracle_cell20          pinned: exhaustive 2^20 bit-loop Morton check,
+ *                          boundary known-answer cases, 30->20 bit prefix.
+ *   oracle_ds_accel_f32    pinned: two-body and ring closed forms, Newton III,
+ *                          FP64 sum, bit-exact vs BH traversal at theta = 0.
+ *   oracle_ds_accel_f64    pinned: closed forms (two-body, ring, lattice).
+ *   oracle_ds_step         pinned: free particle x + v*dt, two-body step.
+ *   oracle_ds_step_f64     pinned: two-body orbit energy and momentum drift.
+ *   oracle_bh_build        pinned: recursive pointer-quadtree brute force on
+ *                          small N, y = n + #cells invariants, dyadic COM.
+ *   oracle_bh_traverse     pinned: recursive group-vote brute force for
+ *                          W in {1,32,64}, theta = 0 == sorted direct sum,
+ *                          theta -> inf == pull of the global COM.
+ *   oracle_bh_integrate    pinned: free particle, kill semantics.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- tiny parallel-for ------------------------------------------------------ */
+
+typedef void (*range_fn)(size_t lo, size_t hi, void *ctx);
+typedef struct {
+    range_fn fn;
+    void *ctx;
+    size_t lo, hi;
+} range_job;
+
+static void *range_thread(void *p) {
+    range_job *j = (range_job *)p;
+    j->fn(j->lo, j->hi, j->ctx);
+    return NULL;
+}
+
+/* Splits [0, n) into nthreads contiguous chunks.  Results never depend on the
+ * thread count: every output element is computed by exactly one thread with
+ * the same operation order. */
+static void parallel_for(size_t n, int nthreads, range_fn fn, void *ctx) {
+    if (nthreads <= 1 || n < 2) {
+        fn(0, n, ctx);
+        return;
+    }
+    if ((size_t)nthreads > n) nthreads = (int)n;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    range_job *jobs = (range_job *)malloc(sizeof(range_job) * (size_t)nthreads);
+    size_t chunk = (n + (size_t)nthreads - 1) / (size_t)nthreads;
+    int started = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        size_t lo = (size_t)t * chunk, hi = lo + chunk;
+        if (lo >= n) break;
+        if (hi > n) hi = n;
+        jobs[t].fn = fn; jobs[t].ctx = ctx; jobs[t].lo = lo; jobs[t].hi = hi;
+        if (pthread_create(&th[t], NULL, range_thread, &jobs[t]) != 0) {
+            fn(lo, hi, ctx); /* fall back to the calling thread */
+            jobs[t].fn = NULL;
+        }
+        started = t + 1;
+    }
+    for (int t = 0; t < started; ++t)
+        if (jobs[t].fn) pthread_join(th[t], NULL);
+    free(th);
+    free(jobs);
+}
+
+/* ---- direct sum ------------------------------------------------------------ */
+
+typedef struct {
+    size_t na;
+    const float *pos, *mass;
+    float eps2;
+    float *acc;
+} ds32_ctx;
+
+/* SimShader.comp:39-65: j ascending over [0, na); the 64-wide LDS tiling of
+ * the reference does not change the order. */
+static void ds32_range(size_t lo, size_t hi, void *p) {
+    ds32_ctx *c = (ds32_ctx *)p;
+    for (size_t i = lo; i < hi; ++i) {
+        const float px = c->pos[2 * i], py = c->pos[2 * i + 1];
+        float ax = 0.0f, ay = 0.0f;
+        for (size_t j = 0; j < c->na; ++j) {
+            const float dx = c->pos[2 * j] - px;             /* :55 */
+            const float dy = c->pos[2 * j + 1] - py;
+            const float r2 = (dx * dx + dy * dy) + c->eps2;  /* :56 dot + eps2 */
+            const float r = 1.0f / sqrtf(r2);                /* inversesqrt */
+            const float s = ((c->mass[j] * r) * r) * r;      /* :59 */
+            ax = ax + dx * s;
+            ay = ay + dy * s;
+        }
+        c->acc[2 * i] = ax;
+        c->acc[2 * i + 1] = ay;
+    }
+}
+
+void oracle_ds_accel_f32(size_t na, const float *pos, const float *mass,
+                         float eps2, float *acc, int nthreads) {
+    ds32_ctx c = {na, pos, mass, eps2, acc};
+    parallel_for(na, nthreads, ds32_range, &c);
+}
+
+typedef struct {
+    size_t na;
+    const float *pos, *mass;
+    double eps2;
+    double *acc;
+} ds64_ctx;
+
+static void ds64_range(size_t lo, size_t hi, void *p) {
+    ds64_ctx *c = (ds64_ctx *)p;
+    for (size_t i = lo; i < hi; ++i) {
+        const double px = c->pos[2 * i], py = c->pos[2 * i + 1];
+        double ax = 0.0, ay = 0.0;
+        for (size_t j = 0; j < c->na; ++j) {
+            const double dx = (double)c->pos[2 * j] - px;
+            const double dy = (double)c->pos[2 * j + 1] - py;
+            const double r2 = dx * dx + dy * dy + c->eps2;
+            const double r = 1.0 / sqrt(r2);
+            const double s = (double)c->mass[j] * r * r * r;
+            ax += dx * s;
+            ay += dy * s;
+        }
+        c->acc[2 * i] = ax;
+        c->acc[2 * i + 1] = ay;
+    }
+}
+
+void oracle_ds_accel_f64(size_t na, const float *pos, const float *mass,
+                         double eps2, double *acc, int nthreads) {
+    ds64_ctx c = {na, pos, mass, eps2, acc};
+    parallel_for(na, nthreads, ds64_range, &c);
+}
+
+void oracle_ds_step(size_t na, const float *pos_in, const float *vel_in,
+                    const float *mass_in, float *pos_out, float *vel_out,
+                    float G, float dt, float eps2, int nthreads) {
+    float *acc = (float *)malloc(sizeof(float) * 2 * (na ? na : 1));
+    oracle_ds_accel_f32(na, pos_in, mass_in, eps2, acc, nthreads);
+    const float kv = dt * G; /* SIMULATION_TIME * GRAVITATIONAL_CONST, :68 */
+    for (size_t i = 0; i < na; ++i) {
+        for (int k = 0; k < 2; ++k) {
+            const float v = vel_in[2 * i + k];
+            const float nv = v + acc[2 * i + k] * kv;               /* :68 */
+            const float np = pos_in[2 * i + k] + ((v + nv) * 0.5f) * dt; /* :69 */
+            vel_out[2 * i + k] = nv;
+            pos_out[2 * i + k] = np;
+        }
+    }
+    free(acc);
+}
+
+typedef struct {
+    size_t na;
+    const double *pos, *mass;
+    double eps2;
+    double *acc;
+} ds64s_ctx;
+
+static void ds64s_range(size_t lo, size_t hi, void *p) {
+    ds64s_ctx *c = (ds64s_ctx *)p;
+    for (size_t i = lo; i < hi; ++i) {
+        double ax = 0.0, ay = 0.0;
+        for (size_t j = 0; j < c->na; ++j) {
+            const double dx = c->pos[2 * j] - c->pos[2 * i];
+            const double dy = c->pos[2 * j + 1] - c->pos[2 * i + 1];
+            const double r2 = dx * dx + dy * dy + c->eps2;
+            const double r = 1.0 / sqrt(r2);
+            const double s = c->mass[j] * r * r * r;
+            ax += dx * s;
+            ay += dy * s;
+        }
+        c->acc[2 * i] = ax;
+        c->acc[2 * i + 1] = ay;
+    }
+}
+
+void oracle_ds_step_f64(size_t na, double *pos, double *vel, const double *mass,
+                        double G, double dt, double eps2, int nthreads) {
+    double *acc = (double *)malloc(sizeof(double) * 2 * (na ? na : 1));
+    ds64s_ctx c = {na, pos, mass, eps2, acc};
+    parallel_for(na, nthreads, ds64s_range, &c);
+    for (size_t i = 0; i < 2 * na; ++i) {
+        const double v = vel[i];
+        const double nv = v + acc[i] * (dt * G);
+        pos[i] = pos[i] + (v + nv) * 0.5 * dt;
+        vel[i] = nv;
+    }
+    free(acc);
+}
+
+/* ---- Barnes-Hut: cells ------------------------------------------------------ */
+
+/* InitShader.comp:84-98: spread the 10 low bits of v to the even bits. */
+static uint32_t part1by1(uint32_t v) {
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+uint32_t oracle_cell20(float x, float y) {
+    /* InitShader.comp:80-83: (pos / S + 1) * 0.5, then *= TREE_SIZE */
+    float qx = ((x / ORACLE_SIM_SIZE) + 1.0f) * 0.5f;
+    float qy = ((y / ORACLE_SIM_SIZE) + 1.0f) * 0.5f;
+    qx = qx * (float)ORACLE_TREE_SIZE;
+    qy = qy * (float)ORACLE_TREE_SIZE;
+    if (qx >= 0.0f && qx < (float)ORACLE_TREE_SIZE && qy >= 0.0f &&
+        qy < (float)ORACLE_TREE_SIZE) {
+        const uint32_t ix = (uint32_t)qx, iy = (uint32_t)qy;
+        return part1by1(ix) | (part1by1(iy) << 1);
+    }
+    return 0xffffffffu;
+}
+
+size_t oracle_bh_capacity(size_t na) { return na + ORACLE_PYRAMID_CELLS; }
+
+/* Offsets of level d inside one dense pyramid array: (4^d - 1)/3. */
+static size_t level_offset(unsigned d) { return ((size_t)1 << (2 * d)) / 3; }
+
+long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
+                     float *mass_out, oracle_tree *tree, size_t *n_inside) {
+    const size_t cells = ORACLE_PYRAMID_CELLS;
+    /* dense per-level arrays (BarnesHutSimulation.cpp:143-234) */
+    uint32_t *cn = (uint32_t *)calloc(cells, sizeof(uint32_t));
+    uint32_t *cy = (uint32_t *)calloc(cells, sizeof(uint32_t));
+    uint32_t *stx = (uint32_t *)calloc(cells, sizeof(uint32_t));
+    uint32_t *sty = (uint32_t *)calloc(cells, sizeof(uint32_t));
+    float *sx = (float *)calloc(cells, sizeof(float));
+    float *sy = (float *)calloc(cells, sizeof(float));
+    float *sm = (float *)calloc(cells, sizeof(float));
+    if (!cn || !cy || !stx || !sty || !sx || !sy || !sm) {
+        free(cn); free(cy); free(stx); free(sty); free(sx); free(sy); free(sm);
+        return -1;
+    }
+    const size_t L10 = level_offset(10);
+    size_t inside = 0;
+
+    /* ClearShader.comp:81-82: every sorted slot starts as the sentinel. */
+    for (size_t i = 0; i < na; ++i) tree->sorted_src[i] = (uint32_t)na;
+
+    /* InitShader.comp:73-119, particles in ascending index (canonical order of
+     * the float atomics). */
+    for (size_t i = 0; i < na; ++i) {
+        const float x = pos[2 * i], y = pos[2 * i + 1], m = mass[i];
+        const uint32_t c = oracle_cell20(x, y);
+        if (c != 0xffffffffu) {
+            const size_t k = L10 + c;
+            cn[k] += 1u;
+            if (cy[k] == 1u) cy[k] = 2u; /* race-free form of :105-106 */
+            cy[k] += 1u;
+            sx[k] = sx[k] + x * m; /* pos *= mass, :102 */
+            sy[k] = sy[k] + y * m;
+            sm[k] = sm[k] + m;
+            ++inside;
+        } else {
+            const float kill = -ORACLE_SIM_SIZE * 2.0f; /* :113 */
+            pos[2 * i] = kill; pos[2 * i + 1] = kill; mass[i] = 0.0f;
+            if (pos_out) { pos_out[2 * i] = kill; pos_out[2 * i + 1] = kill; }
+            if (mass_out) mass_out[i] = 0.0f;
+        }
+    }
+    if (n_inside) *n_inside = inside;
+
+    /* TreeInitShader.comp:58-90, d = 9..0 */
+    for (int d = 9; d >= 0; --d) {
+        const size_t Ld = level_offset((unsigned)d), Lc = level_offset((unsigned)d + 1);
+        const size_t nc = (size_t)1 << (2 * d);
+        for (size_t c = 0; c < nc; ++c) {
+            const size_t ch = Lc + 4 * c;
+            uint32_t n = cn[ch] + cn[ch + 1] + cn[ch + 2] + cn[ch + 3];
+            uint32_t y = cy[ch] + cy[ch + 1] + cy[ch + 2] + cy[ch + 3];
+            y += (y > 1u) ? 1u : 0u;                               /* :83 */
+            cn[Ld + c] = n;
+            cy[Ld + c] = y;
+            sx[Ld + c] = ((sx[ch] + sx[ch + 1]) + sx[ch + 2]) + sx[ch + 3]; /* :79 */
+            sy[Ld + c] = ((sy[ch] + sy[ch + 1]) + sy[ch + 2]) + sy[ch + 3];
+            sm[Ld + c] = ((sm[ch] + sm[ch + 1]) + sm[ch + 2]) + sm[ch + 3]; /* :80 */
+        }
+    }
+
+    /* TreeSortShader.comp:57-94, d = 0..9; the root start is (0, 0). */
+    stx[0] = 0u; sty[0] = 0u;
+    for (unsigned d = 0; d <= 9; ++d) {
+        const size_t Ld = level_offset(d), Lc = level_offset(d + 1);
+        const size_t nc = (size_t)1 << (2 * d);
+        for (size_t c = 0; c < nc; ++c) {
+            uint32_t ox = stx[Ld + c], oy = sty[Ld + c];
+            if (cy[Ld + c] > 1u) oy += 1u;                         /* :76-77 */
+            for (size_t k = 0; k < 4; ++k) {
+                const size_t ch = Lc + 4 * c + k;
+                stx[ch] = ox; sty[ch] = oy;
+                ox += cn[ch]; oy += cy[ch];                        /* :80-85 */
+            }
+        }
+    }
+
+    /* TreeMoveShader.comp:53-84, d = 0..10 */
+    long status = 0;
+    for (unsigned d = 0; d <= 10; ++d) {
+        const size_t Ld = level_offset(d);
+        const size_t nc = (size_t)1 << (2 * d);
+        const float radius = (ORACLE_SIM_SIZE * ORACLE_SIM_SIZE * 2.0f) /
+                             (float)(1u << (2 * d));               /* :70-71 */
+        for (size_t c = 0; c < nc; ++c) {
+            const size_t k = Ld + c;
+            if (cy[k] > 1u) {
+                float px = sx[k], py = sy[k];
+                const float m = sm[k];
+                if (m != 0.0f) { px = px / m; py = py / m; }      /* :66-67 */
+                const uint32_t slot = sty[k];
+                if ((size_t)slot >= tree->cap) { status = -1; continue; }
+                tree->counts[slot] = cy[k];
+                tree->radii[slot] = radius;
+                tree->node_pos[2 * (size_t)slot] = px;
+                tree->node_pos[2 * (size_t)slot + 1] = py;
+                tree->node_mass[slot] = m;
+                if (d == 10) sty[k] += 1u;                         /* :80-81 */
+            }
+        }
+    }
+
+    /* ParticleSortShader.comp:71-110, ascending index (canonical). */
+    for (size_t i = 0; i < na && status == 0; ++i) {
+        const float x = pos[2 * i], y = pos[2 * i + 1];
+        const uint32_t c = oracle_cell20(x, y);
+        if (c == 0xffffffffu) continue;
+        const size_t k = L10 + c;
+        const uint32_t dsrc = stx[k]++;
+        tree->sorted_src[dsrc] = (uint32_t)i;
+        const uint32_t slot = sty[k]++;
+        if ((size_t)slot >= tree->cap) { status = -1; break; }
+        tree->counts[slot] = 1u;
+        tree->radii[slot] = 0.0f;
+        tree->node_pos[2 * (size_t)slot] = x;
+        tree->node_pos[2 * (size_t)slot + 1] = y;
+        tree->node_mass[slot] = mass[i];
+    }
+
+    free(cn); free(cy); free(stx); free(sty); free(sx); free(sy); free(sm);
+    if (status != 0) return -1;
+    return inside ? (long)tree->counts[0] : 0;
+}
+
+/* ---- Barnes-Hut: traversal ------------------------------------------------- */
+
+typedef struct {
+    size_t na;
+    uint32_t W;
+    const float *pos, *mass;
+    const oracle_tree *tree;
+    size_t n_nodes;
+    float eps2, theta2;
+    float *acc;
+    uint64_t *visits, *accepts; /* per group */
+    uint8_t *ambiguous;         /* per group, may be NULL */
+} trav_ctx;
+
+static void trav_range(size_t glo, size_t ghi, void *p) {
+    trav_ctx *c = (trav_ctx *)p;
+    const uint32_t W = c->W;
+    float *lx = (float *)malloc(sizeof(float) * W * 6);
+    float *ly = lx + W, *ax = lx + 2 * W, *ay = lx + 3 * W, *ddx = lx + 4 * W,
+          *ddy = lx + 5 * W;
+    uint8_t *real = (uint8_t *)malloc(W);
+    uint32_t *src = (uint32_t *)malloc(sizeof(uint32_t) * W);
+    const oracle_tree *t = c->tree;
+    for (size_t g = glo; g < ghi; ++g) {
+        uint32_t nreal = 0;
+        for (uint32_t l = 0; l < W; ++l) {
+            const size_t slot = g * W + l;
+            src[l] = t->sorted_src[slot];                /* ForceShader :80 */
+            real[l] = src[l] != (uint32_t)c->na;
+            lx[l] = real[l] ? c->pos[2 * (size_t)src[l]] : 0.0f;
+            ly[l] = real[l] ? c->pos[2 * (size_t)src[l] + 1] : 0.0f;
+            ax[l] = 0.0f; ay[l] = 0.0f;
+            nreal += real[l];
+        }
+        uint64_t vis = 0, acc_n = 0;
+        uint8_t amb = 0;
+        size_t ind = 0;
+        const size_t total = c->n_nodes; /* counts[0], :106 */
+        while (ind < total) {
+            const float nx = t->node_pos[2 * ind], ny = t->node_pos[2 * ind + 1];
+            const float rad = t->radii[ind];
+            int all = 1;
+            for (uint32_t l = 0; l < W; ++l) {
+                const float dx = nx - lx[l], dy = ny - ly[l];
+                const float r2 = (dx * dx + dy * dy) + c->eps2;
+                ddx[l] = dx; ddy[l] = dy;
+                const float lim = r2 * c->theta2;
+                const int ok = (rad <= lim) || !real[l];
+                all &= ok;
+                if (real[l] && rad > 0.0f) {
+                    const float gap = fabsf(rad - lim);
+                    if (gap <= 4.0f * (nextafterf(rad, INFINITY) - rad)) amb = 1;
+                }
+            }
+            vis += nreal;
+            if (all) {
+                const float m = t->node_mass[ind];
+                for (uint32_t l = 0; l < W; ++l) {
+                    const float dx = ddx[l], dy = ddy[l];
+                    const float r2 = (dx * dx + dy * dy) + c->eps2;
+                    const float r = 1.0f / sqrtf(r2);
+                    const float s = ((m * r) * r) * r;     /* :134 */
+                    ax[l] = ax[l] + dx * s;
+                    ay[l] = ay[l] + dy * s;
+                }
+                acc_n += nreal;
+                ind += t->counts[ind];                     /* :136 */
+            } else {
+                ++ind;                                     /* :139 */
+            }
+        }
+        for (uint32_t l = 0; l < W; ++l) {
+            if (!real[l]) continue;
+            c->acc[2 * (size_t)src[l]] = ax[l];
+            c->acc[2 * (size_t)src[l] + 1] = ay[l];
+        }
+        c->visits[g] = vis;
+        c->accepts[g] = acc_n;
+        if (c->ambiguous) c->ambiguous[g] = amb;
+    }
+    free(lx); free(real); free(src);
+}
+
+void oracle_bh_traverse(size_t na, uint32_t W, const float *pos,
+                        const float *mass, const oracle_tree *tree,
+                        size_t n_nodes, float eps2, float theta2, float *acc,
+                        uint64_t *visits, uint64_t *accepts,
+                        uint64_t *ambiguous, uint8_t *group_ambiguous,
+                        int nthreads) {
+    if (W == 0 || na % W != 0) return;
+    const size_t groups = na / W;
+    uint64_t *gv = (uint64_t *)calloc(groups ? groups : 1, sizeof(uint64_t));
+    uint64_t *ga = (uint64_t *)calloc(groups ? groups : 1, sizeof(uint64_t));
+    uint8_t *gamb = (uint8_t *)calloc(groups ? groups : 1, 1);
+    trav_ctx c = {na, W, pos, mass, tree, n_nodes, eps2, theta2, acc, gv, ga, gamb};
+    if (n_nodes > 0) parallel_for(groups, nthreads, trav_range, &c);
+    uint64_t V = 0, A = 0, AM = 0;
+    for (size_t g = 0; g < groups; ++g) { V += gv[g]; A += ga[g]; AM += gamb[g]; }
+    if (group_ambiguous) memcpy(group_ambiguous, gamb, groups);
+    if (visits) *visits = V;
+    if (accepts) *accepts = A;
+    if (ambiguous) *ambiguous = AM;
+    free(gv); free(ga); free(gamb);
+}
+
+void oracle_bh_integrate(size_t na, const float *pos, const float *vel,
+                         const float *mass, const float *acc, float *pos_out,
+                         float *vel_out, float *mass_out, float G, float dt) {
+    const float kv = G * dt;    /* GRAVITATIONAL_CONST * SIMULATION_TIME, :145 */
+    const float kp = 0.5f * dt; /* 0.5 * SIMULATION_TIME, :146 */
+    for (size_t i = 0; i < na; ++i) {
+        if (oracle_cell20(pos[2 * i], pos[2 * i + 1]) == 0xffffffffu) {
+            vel_out[2 * i] = vel[2 * i];
+            vel_out[2 * i + 1] = vel[2 * i + 1];
+            continue; /* killed: position and mass were written by the build */
+        }
+        for (int k = 0; k < 2; ++k) {
+            const float v = vel[2 * i + k];
+            const float nv = v + acc[2 * i + k] * kv;
+            vel_out[2 * i + k] = nv;
+            pos_out[2 * i + k] = pos[2 * i + k] + (v + nv) * kp;
+        }
+        if (mass_out) mass_out[i] = mass[i];
+    }
+}
+
+int oracle_bh_step(size_t na, float *pos_in, float *vel_in, float *mass_in,
+                   float *pos_out, float *vel_out, float *mass_out, float G,
+                   float dt, float eps2, float theta2, uint32_t W,
+                   uint64_t *visits, uint64_t *accepts, int nthreads) {
+    const size_t cap = oracle_bh_capacity(na);
+    oracle_tree t;
+    t.cap = cap;
+    t.counts = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+    t.radii = (float *)malloc(sizeof(float) * cap);
+    t.node_pos = (float *)malloc(sizeof(float) * 2 * cap);
+    t.node_mass = (float *)malloc(sizeof(float) * cap);
+    t.sorted_src = (uint32_t *)malloc(sizeof(uint32_t) * (na ? na : 1));
+    float *acc = (float *)calloc(2 * (na ? na : 1), sizeof(float));
+    int rc = 0;
+    if (!t.counts || !t.radii || !t.node_pos || !t.node_mass || !t.sorted_src || !acc) {
+        rc = -2;
+    } else {
+        const long nn = oracle_bh_build(na, pos_in, mass_in, pos_out, mass_out, &t, NULL);
+        if (nn < 0) {
+            rc = -1;
+        } else {
+            oracle_bh_traverse(na, W, pos_in, mass_in, &t, (size_t)nn, eps2, theta2,
+                               acc, visits, accepts, NULL, NULL, nthreads);
+            oracle_bh_integrate(na, pos_in, vel_in, mass_in, acc, pos_out, vel_out,
+                                mass_out, G, dt);
+        }
+    }
+    free(t.counts); free(t.radii); free(t.node_pos); free(t.node_mass);
+    free(t.sorted_src); free(acc);
+    return rc;
+}

@@ -1,0 +1,276 @@
+"""GPU parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Tolerances (FP32 forces): the GPU's v_rsq_f32 and FMA differ from the
+oracle's IEEE 1/sqrt and unfused sums by a few ulp per interaction, so forces
+are compared with
+    system-relative  ||a_gpu - a_ref|| / ||a_ref||            <= 1e-5
+    per particle     |a_gpu,i - a_ref,i| / max(|a_ref,i|, rms) <= 1e-5
+Integer work (cells, permutation, tree topology, visit/accept counts) and the
+centre-of-mass arithmetic are compared bit for bit.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import gsim
+import helpers as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FORCE_TOL = 1e-5
+EPS2 = H.sq32(0.2)
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _sim(p, algo, **kw):
+    s = gsim.Simulation(p, algo, **kw)
+    s.set_debug(gsim.DEBUG_ACCEL)
+    return s
+
+
+def _force_ok(a, ref, tol=FORCE_TOL):
+    sysr = H.system_rel_err(a, ref)
+    per = H.per_particle_rel(a, ref)
+    assert sysr <= tol, f"system rel err {sysr:.3e}"
+    assert per.max() <= tol, f"per-particle max {per.max():.3e}, p99.9 {np.quantile(per, .999):.3e}"
+    return sysr, per.max()
+
+
+# ---- direct sum --------------------------------------------------------------------
+
+def test_ds_config1_4096_100_steps():
+    """4096 particles, uniform disk r=100, v=0, m in [1,10], G=1, eps=0.2,
+    dt=1e-3, 100 steps: one-step forces vs the FP32 oracle, and the 100-step
+    trajectory vs FP64 no worse than the FP32 oracle's own deviation x 4."""
+    _say("ds config1")
+    p = gsim.generate("random", 4096, 100.0, 1.0, 10.0, seed=101)
+    s = _sim(p, "direct-sum")
+    s.step(1)
+    a = s.dump(gsim.DUMP_ACCEL)[: p.n]
+    ref = O.ds_accel_f32(p.pos, p.mass, EPS2)
+    _force_ok(a, ref)
+    a64 = O.ds_accel_f64(p.pos, p.mass, EPS2)
+    assert H.system_rel_err(a, a64) < 2e-6
+    s.step(99)
+    g = s.read()
+    s.close()
+    po, vo = p.pos.copy(), p.vel.copy()
+    for _ in range(100):
+        po, vo = O.ds_step(po, vo, p.mass, 1.0, 1e-3, EPS2)
+    p64, v64 = O.ds_steps_f64(p.pos, p.vel, p.mass, 1.0, 1e-3, EPS2, 100)
+    dev32 = np.linalg.norm(po - p64)
+    devg = np.linalg.norm(g.pos - p64)
+    _say(f"  trajectory deviation from FP64: oracle {dev32:.3e}, gpu {devg:.3e}")
+    assert devg <= 4.0 * dev32 + 1e-6
+    assert np.linalg.norm(g.vel - v64) <= 4.0 * np.linalg.norm(vo - v64) + 1e-6
+    assert np.array_equal(g.mass, p.mass)
+
+
+@pytest.mark.parametrize("R,S", [(1, 0), (1, 1), (2, 8), (4, 3)])
+def test_ds_50k_one_step_all_shapes(R, S):
+    """50,000-particle exponential disk (the direct-sum benchmark workload):
+    forces of every launch shape within the tolerance of the FP32 oracle."""
+    _say(f"ds 50k R={R} S={S}")
+    p = gsim.generate("exponential-disk", 50000, 100.0, 1.0, 10.0, seed=102)
+    s = _sim(p, "direct-sum")
+    s.set_ds_shape(R, S)
+    s.step(1)
+    a = s.dump(gsim.DUMP_ACCEL)[: p.n]
+    g = s.read()
+    s.close()
+    pp = p.padded()
+    ref = O.ds_accel_f32(pp.pos, pp.mass, EPS2)[: p.n]
+    _force_ok(a, ref)
+    # integrate: v' = v + a*(dt*G), p' = p + ((v+v')*0.5)*dt on the GPU's forces
+    kv = np.float32(1e-3) * np.float32(1.0)
+    nv = (p.vel + a * kv).astype(np.float32)
+    assert np.array_equal(g.vel, nv)
+    np_ = (p.pos + ((p.vel + nv) * np.float32(0.5)) * np.float32(1e-3)).astype(np.float32)
+    assert np.array_equal(g.pos, np_)
+
+
+def test_ds_ragged_and_tiny_sizes():
+    for n in (1, 2, 63, 64, 65, 1000):
+        _say(f"ds n={n}")
+        p = gsim.generate("galaxy", n, 50.0, 1.0, 10.0, seed=n)
+        s = _sim(p, "direct-sum")
+        assert s.na == (n + 63) // 64 * 64
+        s.step(3)
+        g = s.read()
+        s.close()
+        pp = p.padded()
+        po, vo = pp.pos, pp.vel
+        for _ in range(3):
+            po, vo = O.ds_step(po, vo, pp.mass, 1.0, 1e-3, EPS2)
+        assert np.allclose(g.pos, po[:n], rtol=1e-6, atol=1e-6)
+        assert np.allclose(g.vel, vo[:n], rtol=1e-4, atol=1e-4)
+
+
+def test_ds_ignores_simulation_speed_and_zero_softening_is_nan():
+    p = gsim.generate("random", 128, 10.0, 1.0, 2.0, seed=3)
+    a = _sim(p, "direct-sum", simulation_speed=5.0)
+    b = _sim(p, "direct-sum", simulation_speed=1.0)
+    a.step(2); b.step(2)
+    assert np.array_equal(a.read().pos, b.read().pos)  # SimShader.comp:68-69 use dt only
+    a.close(); b.close()
+    z = _sim(p, "direct-sum", softening_len=0.0)
+    z.step(1)
+    assert np.isnan(z.read().vel).all()
+    z.close()
+
+
+# ---- Barnes-Hut: build -----------------------------------------------------------------
+
+def _bh_inputs():
+    yield "dyadic-700", *H.dyadic_particles(700, seed=201)[::2]
+    rng = np.random.default_rng(202)
+    pos = rng.normal(0, 40, size=(3000, 2)).astype(np.float32)
+    pos[:20] = rng.uniform(-700, 700, size=(20, 2))  # some leave the domain
+    pos[20] = [np.nan, 0]
+    yield "normal-3000-kills", pos, rng.uniform(1, 10, 3000).astype(np.float32)
+    p = gsim.generate("galaxy-collision", 10000, 300.0, 1.0, 10.0, seed=203)
+    yield "collision-10000", p.pos, p.mass
+    p = gsim.generate("random", 5000, 0.5, 1.0, 10.0, seed=204)  # crowded leaves
+    yield "crowded-5000", p.pos, p.mass
+    p = gsim.generate("two-exponential-disks", 500000, 33.0, 1.0, 10.0, seed=205)
+    yield "two-disks-500000", p.pos, p.mass
+
+
+@pytest.mark.parametrize("case", list(range(5)))
+def test_bh_tree_bit_exact(case):
+    name, pos, mass = list(_bh_inputs())[case]
+    _say(f"bh tree {name}")
+    n = len(mass)
+    p = gsim.Particles(pos, np.zeros_like(pos), mass)
+    s = _sim(p, "barnes-hut")
+    s.step(1)
+    t = s.tree()
+    cells = s.dump(gsim.DUMP_CELLS)
+    m30 = s.dump(gsim.DUMP_MORTON30)
+    s.close()
+    pp = p.padded()
+    ot, nn, nin, pk, mk = O.bh_build(pp.pos, pp.mass)
+    assert t["total"] == nn
+    assert np.array_equal(cells, H.cells_numpy(pp.pos))
+    ok = cells != 0xFFFFFFFF
+    assert np.array_equal(m30[ok] >> np.uint32(10), cells[ok])
+    assert np.array_equal(t["sorted_src"], ot["sorted_src"])
+    assert np.array_equal(t["counts"], ot["counts"])
+    assert np.array_equal(t["radii"], ot["radii"])
+    assert np.array_equal(t["node_mass"], ot["node_mass"])
+    assert np.array_equal(t["node_pos"], ot["node_pos"])
+
+
+def test_bh_degenerate_trees():
+    cases = {
+        "single": (np.array([[1.0, 2.0]], np.float32), np.array([3.0], np.float32)),
+        "all-killed": (np.full((70, 2), 900.0, np.float32), np.ones(70, np.float32)),
+        "one-leaf": (np.full((64, 2), 0.1, np.float32), np.ones(64, np.float32)),
+        "edge": (np.array([[-500, -500], [499.99994, 499.99994], [499.99997, 0]],
+                          np.float32), np.ones(3, np.float32)),
+    }
+    for name, (pos, mass) in cases.items():
+        _say(f"bh degenerate {name}")
+        p = gsim.Particles(pos, np.ones_like(pos), mass)
+        s = _sim(p, "barnes-hut")
+        s.step(1)
+        t = s.tree()
+        g = s.read()
+        s.close()
+        pp = p.padded()
+        po, vo, mo, V, A = O.bh_step(pp.pos, pp.vel, pp.mass, 1.0, 1e-3, EPS2, 1.0, 64)
+        ot, nn, *_ = O.bh_build(pp.pos, pp.mass)
+        assert t["total"] == nn
+        assert np.array_equal(t["counts"], ot["counts"])
+        assert np.array_equal(t["sorted_src"], ot["sorted_src"])
+        assert np.array_equal(g.mass, mo[: p.n])
+        assert np.allclose(g.pos, po[: p.n], rtol=1e-6, atol=1e-6)
+        assert np.allclose(g.vel, vo[: p.n], rtol=1e-5, atol=1e-5)
+
+
+# ---- Barnes-Hut: traversal -----------------------------------------------------------------
+
+@pytest.mark.parametrize("W", [64, 32])
+@pytest.mark.parametrize("case", [1, 2, 3, 4])
+def test_bh_traversal_counts_exact_forces_within_tol(W, case):
+    """On the GPU's own (bit-exact) tree, the GPU walk visits and accepts the
+    same nodes as the oracle's walk, so V/A match exactly; forces differ only
+    by rsqrt rounding."""
+    name, pos, mass = list(_bh_inputs())[case]
+    _say(f"bh traverse W={W} {name}")
+    vel = np.zeros_like(pos)
+    p = gsim.Particles(pos, vel, mass)
+    s = _sim(p, "barnes-hut", group_width=W)
+    s.step(1)
+    a = s.dump(gsim.DUMP_ACCEL)
+    V, A = (int(x) for x in s.dump(gsim.DUMP_COUNTERS))
+    t = s.tree()
+    g = s.read()
+    s.close()
+    pp = p.padded()
+    _, _, _, pk, mk = O.bh_build(pp.pos, pp.mass)
+    oa, oV, oA, amb, _ = O.bh_traverse(t, pk, mk, EPS2, 1.0, W=W)
+    _say(f"  V={V} A={A} oracle V={oV} A={oA}")
+    assert (V, A) == (oV, oA)
+    inside = t["sorted_src"][t["sorted_src"] < len(mk)]
+    _force_ok(a[inside], oa[inside])
+    # integrate of the in-domain particles
+    kv = np.float32(1.0) * np.float32(1e-3)
+    kp = np.float32(0.5) * np.float32(1e-3)
+    real = inside[inside < p.n]
+    nv = (pp.vel[real] + a[real] * kv).astype(np.float32)
+    assert np.array_equal(g.vel[real], nv)
+    assert np.array_equal(g.pos[real], (pk[real] + (pp.vel[real] + nv) * kp).astype(np.float32))
+
+
+def test_bh_theta_zero_equals_sorted_direct_sum():
+    p = gsim.generate("random", 2000, 60.0, 1.0, 10.0, seed=206)
+    s = _sim(p, "barnes-hut", accuracy_parameter=0.0)
+    s.step(1)
+    a = s.dump(gsim.DUMP_ACCEL)
+    V, A = (int(x) for x in s.dump(gsim.DUMP_COUNTERS))
+    s.close()
+    pp = p.padded()
+    ref = O.ds_accel_f32(pp.pos, pp.mass, EPS2)
+    assert A == len(pp.mass) ** 2
+    _force_ok(a[: p.n], ref[: p.n])
+
+
+@pytest.mark.parametrize("W", [64, 32])
+def test_bh_multistep_trajectory(W):
+    """20 steps of a galaxy collision: the GPU run and the oracle run stay
+    within FP32 rounding of each other (accelerations differ by rsqrt ulps,
+    which can flip a vote near its threshold, so the bound is on the system
+    norm of the trajectory)."""
+    _say(f"bh multistep W={W}")
+    p = gsim.generate("galaxy-collision", 6000, 150.0, 1.0, 10.0, seed=207)
+    s = _sim(p, "barnes-hut", group_width=W, simulation_speed=2.0)
+    s.step(20)
+    g = s.read()
+    s.close()
+    pp = p.padded()
+    po, vo, mo = pp.pos, pp.vel, pp.mass
+    dt = float(np.float32(1e-3) * np.float32(2.0))
+    for _ in range(20):
+        po, vo, mo, _, _ = O.bh_step(po, vo, mo, 1.0, dt, EPS2, 1.0, W)
+    disp = np.linalg.norm(po[: p.n] - p.pos)
+    assert np.linalg.norm(g.pos - po[: p.n]) <= 1e-4 * disp
+    assert H.system_rel_err(g.vel, vo[: p.n]) <= 1e-4
+    assert np.array_equal(g.mass, mo[: p.n])
+
+
+def test_timing_and_counters_api():
+    p = gsim.generate("two-exponential-disks", 20000, 33.0, 1.0, 10.0, seed=208)
+    s = gsim.Simulation(p, "barnes-hut")
+    s.set_timing(True)
+    s.step(3)
+    t = s.timing()
+    assert t["steps"] == 3 and t["force_ms"] > 0 and t["build_ms"] > 0
+    assert abs(t["step_ms"] - t["force_ms"] - t["build_ms"]) < 1e-2 + 0.05 * t["step_ms"]
+    s.close()

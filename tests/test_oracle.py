@@ -1,0 +1,309 @@
+"""Pins the CPU oracle (the checker) against closed forms, brute force and
+invariants — none of these retype the oracle's own formula.  CPU only."""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle as O
+
+
+# ---- Morton cells (InitShader.comp:80-98) --------------------------------------
+
+def test_morton_exhaustive_against_bit_loop():
+    """All 2^20 (ix, iy): the oracle's cell of the centre of leaf (ix, iy)
+    equals the bit-loop interleave; 2^20 C calls are too slow via ctypes, so the
+    mask form is checked in numpy against the bit loop exhaustively and the
+    oracle on a dense sample of the same centres."""
+    ix, iy = np.meshgrid(np.arange(1024, dtype=np.uint32), np.arange(1024, dtype=np.uint32))
+    ix = ix.ravel(); iy = iy.ravel()
+    ref = H.morton_bitloop(ix, iy)
+
+    def part1by1(v):
+        v = v.astype(np.uint32)
+        v = (v | (v << 8)) & np.uint32(0x00FF00FF)
+        v = (v | (v << 4)) & np.uint32(0x0F0F0F0F)
+        v = (v | (v << 2)) & np.uint32(0x33333333)
+        v = (v | (v << 1)) & np.uint32(0x55555555)
+        return v
+
+    assert np.array_equal(part1by1(ix) | (part1by1(iy) << np.uint32(1)), ref)
+    # positions at leaf centres: x = (ix + 0.5) * (1000/1024) - 500
+    rng = np.random.default_rng(0)
+    sel = rng.choice(ix.size, 4000, replace=False)
+    xs = ((ix[sel] + 0.5) * (1000.0 / 1024.0) - 500.0).astype(np.float32)
+    ys = ((iy[sel] + 0.5) * (1000.0 / 1024.0) - 500.0).astype(np.float32)
+    got = np.array([O.cell20(float(x), float(y)) for x, y in zip(xs, ys)], np.uint32)
+    assert np.array_equal(got, ref[sel])
+
+
+@pytest.mark.parametrize("x,y,expect_inside", [
+    (-500.0, -500.0, True),       # lower edge is inside
+    (500.0, 0.0, False),          # q = 1024 is outside
+    (0.0, 500.0, False),
+    # one ulp below 500: x/500 + 1 ties to 2.0, so q = 1024 and the particle
+    # is killed by the reference formula; two ulps below stays inside
+    (np.nextafter(np.float32(500), np.float32(0)), 0.0, False),
+    (np.float32(500) - np.float32(2.0 ** -14), 0.0, True),
+    (-500.0001, 0.0, False),
+    (float("nan"), 0.0, False),   # NaN fails every comparison -> killed
+    (0.0, float("inf"), False),
+    (0.0, 0.0, True),
+])
+def test_cell_boundaries(x, y, expect_inside):
+    c = O.cell20(float(np.float32(x)), float(np.float32(y)))
+    assert (c != 0xFFFFFFFF) == expect_inside
+
+
+def test_cell_known_answers():
+    # origin: q = 512 on both axes -> part1by1(512) | part1by1(512) << 1
+    assert O.cell20(0.0, 0.0) == (1 << 18) | (1 << 19)
+    assert O.cell20(-500.0, -500.0) == 0
+    last = np.float32(500) - np.float32(2.0 ** -14)
+    assert O.cell20(float(last), float(last)) == (1 << 20) - 1
+    # the numpy FP32 restatement agrees on random positions, inside and out
+    rng = np.random.default_rng(1)
+    pos = rng.uniform(-520, 520, size=(5000, 2)).astype(np.float32)
+    got = np.array([O.cell20(float(a), float(b)) for a, b in pos], np.uint32)
+    assert np.array_equal(got, H.cells_numpy(pos))
+
+
+def test_morton30_prefix_is_cell20():
+    """30-bit code of the same normalised coordinate, shifted right by 10, is
+    the 20-bit cell (scaling by 32 is exact)."""
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(-499.9, 499.9, size=(20000, 2)).astype(np.float32)
+    q = ((pos / np.float32(500)) + np.float32(1)) * np.float32(0.5)
+    q30 = (q * np.float32(32768)).astype(np.uint32)
+    m30 = H.morton_bitloop(q30[:, 0], q30[:, 1], bits=15)
+    assert np.array_equal(m30 >> np.uint32(10), H.cells_numpy(pos))
+
+
+# ---- direct sum (SimShader.comp) -----------------------------------------------
+
+def test_ds_two_body_closed_form():
+    pos = np.array([[0.0, 0.0], [3.0, 4.0]], np.float32)
+    mass = np.array([2.0, 5.0], np.float32)
+    eps2 = H.sq32(0.2)
+    a = O.ds_accel_f64(pos, mass, eps2)
+    r2 = 25.0 + eps2
+    k = 1.0 / (r2 * np.sqrt(r2))
+    assert np.allclose(a[0], np.array([3.0, 4.0]) * 5.0 * k, rtol=1e-12)
+    assert np.allclose(a[1], -np.array([3.0, 4.0]) * 2.0 * k, rtol=1e-12)
+    a32 = O.ds_accel_f32(pos, mass, eps2)
+    assert np.allclose(a32, a, rtol=1e-6)
+
+
+def test_ds_ring_centre_has_zero_force():
+    n = 64
+    ang = 2 * np.pi * np.arange(n) / n
+    pos = np.zeros((n + 1, 2), np.float32)
+    pos[1:, 0] = 10 * np.cos(ang)
+    pos[1:, 1] = 10 * np.sin(ang)
+    mass = np.ones(n + 1, np.float32)
+    a64 = O.ds_accel_f64(pos, mass, H.sq32(0.2))
+    ring = np.linalg.norm(a64[1:], axis=1).mean()
+    assert np.linalg.norm(a64[0]) < 1e-12 * ring * n
+
+
+def test_ds_newton_third_law_and_f64_agreement():
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(-50, 50, size=(512, 2)).astype(np.float32)
+    mass = rng.uniform(1, 10, size=512).astype(np.float32)
+    eps2 = H.sq32(0.2)
+    a64 = O.ds_accel_f64(pos, mass, eps2)
+    mom = (mass[:, None].astype(np.float64) * a64).sum(axis=0)
+    scale = np.abs(mass[:, None] * a64).sum()
+    assert np.linalg.norm(mom) < 1e-12 * scale
+    a32 = O.ds_accel_f32(pos, mass, eps2)
+    assert H.system_rel_err(a32, a64) < 2e-6
+
+
+def test_ds_step_free_particle_and_thread_invariance():
+    pos = np.array([[1.0, 2.0]], np.float32)
+    vel = np.array([[3.0, -4.0]], np.float32)
+    mass = np.array([7.0], np.float32)
+    po, vo = O.ds_step(pos, vel, mass, 1.0, 1e-3, H.sq32(0.2))
+    # self term: d = 0 -> no force; p' = p + ((v + v)*0.5)*dt
+    assert np.array_equal(vo, vel)
+    dt = np.float32(1e-3)
+    assert np.array_equal(po, pos + ((vel + vel) * np.float32(0.5)) * dt)
+    rng = np.random.default_rng(4)
+    p = rng.uniform(-20, 20, size=(300, 2)).astype(np.float32)
+    m = rng.uniform(1, 3, size=300).astype(np.float32)
+    a1 = O.ds_accel_f32(p, m, 0.04, nthreads=1)
+    a7 = O.ds_accel_f32(p, m, 0.04, nthreads=7)
+    assert np.array_equal(a1, a7)
+
+
+def test_ds_zero_softening_gives_nan_like_reference():
+    pos = np.array([[0.0, 0.0], [1.0, 0.0]], np.float32)
+    a = O.ds_accel_f32(pos, np.ones(2, np.float32), 0.0)
+    assert np.isnan(a).all()  # self term 0 * inf
+
+
+# ---- Barnes-Hut build ------------------------------------------------------------
+
+def _compare_tree(t, b):
+    assert t["total"] == b["total"]
+    assert np.array_equal(t["counts"], b["counts"])
+    assert np.array_equal(t["radii"], b["radii"])
+    assert np.array_equal(t["node_pos"], b["node_pos"])
+    assert np.array_equal(t["node_mass"], b["node_mass"])
+    assert np.array_equal(t["sorted_src"], b["sorted_src"])
+
+
+@pytest.mark.parametrize("seed,n,spread", [(0, 37, 400.0), (1, 64, 5.0), (2, 150, 1.0),
+                                           (3, 200, 0.01)])
+def test_bh_build_matches_recursive_quadtree(seed, n, spread):
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-spread, spread, size=(n, 2)).astype(np.float32)
+    mass = rng.uniform(1, 10, size=n).astype(np.float32)
+    pad = (-n) % 64
+    pos = np.vstack([pos, np.zeros((pad, 2), np.float32)])
+    mass = np.concatenate([mass, np.zeros(pad, np.float32)])
+    tree, nn, nin, _, _ = O.bh_build(pos, mass)
+    _compare_tree(tree, H.brute_tree(pos, mass))
+
+
+def test_bh_build_counts_invariants():
+    rng = np.random.default_rng(5)
+    pos = np.vstack([rng.normal(0, 3, size=(900, 2)), rng.uniform(-480, 480, size=(124, 2))])
+    pos = pos.astype(np.float32)
+    mass = rng.uniform(1, 10, size=1024).astype(np.float32)
+    tree, nn, nin, _, _ = O.bh_build(pos, mass)
+    cells = H.cells_numpy(pos)
+    inside = cells[cells != 0xFFFFFFFF]
+    # counts[0] = N_in + #cells with n >= 2 over all depths
+    ncell = 0
+    for d in range(11):
+        _, cnt = np.unique(inside >> np.uint32(2 * (10 - d)), return_counts=True)
+        ncell += int((cnt >= 2).sum())
+    assert nin == inside.size
+    assert tree["counts"][0] == nin + ncell == nn
+    # every subtree ends inside its parent's subtree (pre-order consistency)
+    c = tree["counts"]
+    for u in range(nn):
+        end = u + c[u]
+        assert end <= nn
+        v = u + 1
+        while v < end:
+            v += c[v]
+        assert v == end
+    # particle nodes are exactly the radius-0 nodes, one per inside particle
+    assert int((tree["radii"] == 0).sum()) == nin
+    assert np.all(tree["counts"][tree["radii"] == 0] == 1)
+
+
+def test_bh_build_dyadic_com_exact_and_kills():
+    pos, vel, mass = H.dyadic_particles(700, seed=6)
+    pos[:5] = [[600, 0], [0, -700], [float("nan"), 1], [500, 500], [-500, -500]]
+    pos = np.vstack([pos, np.zeros((68, 2), np.float32)])
+    mass = np.concatenate([mass, np.zeros(68, np.float32)])
+    tree, nn, nin, pos_k, mass_k = O.bh_build(pos, mass)
+    assert nin == 768 - 4  # four killed, (-500,-500) stays inside
+    assert np.all(pos_k[:4] == -1000) and np.all(mass_k[:4] == 0)
+    assert np.array_equal(pos_k[4], [-500, -500])
+    _compare_tree(tree, H.brute_tree(pos_k, mass_k))
+    # dyadic: the root COM equals the exact (float64) centre of mass
+    m = mass_k.astype(np.float64)
+    com = (pos_k.astype(np.float64) * m[:, None]).sum(0) / m.sum()
+    assert np.allclose(tree["node_pos"][0], com, rtol=0, atol=1e-6)
+    assert tree["node_mass"][0] == np.float32(m.sum())
+
+
+def test_bh_build_degenerate_inputs():
+    # one particle: the root is the particle node itself
+    pos = np.zeros((64, 2), np.float32); pos[1:] = 900  # 63 killed
+    mass = np.zeros(64, np.float32); mass[0] = 3
+    tree, nn, nin, _, _ = O.bh_build(pos, mass)
+    assert (nn, nin) == (1, 1) and tree["counts"][0] == 1
+    # nothing inside: no nodes
+    tree, nn, nin, _, _ = O.bh_build(np.full((64, 2), 900, np.float32), np.ones(64, np.float32))
+    assert (nn, nin) == (0, 0)
+    assert np.all(tree["sorted_src"] == 64)
+    # all in one leaf cell: root chain of 10 single-child cells, then the leaf
+    pos = np.full((64, 2), 0.1, np.float32)
+    tree, nn, nin, _, _ = O.bh_build(pos, np.ones(64, np.float32))
+    assert nn == 64 + 11
+    assert list(tree["counts"][:11]) == [75 - d for d in range(11)]
+
+
+# ---- Barnes-Hut traversal ------------------------------------------------------------
+
+def _input(n=640, seed=7, spread=30.0):
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(0, spread, size=(n, 2)).astype(np.float32)
+    mass = rng.uniform(1, 10, size=n).astype(np.float32)
+    return pos, mass
+
+
+@pytest.mark.parametrize("W", [1, 32, 64])
+@pytest.mark.parametrize("theta", [0.5, 1.0])
+def test_bh_traverse_matches_recursive_group_vote(W, theta):
+    pos, mass = _input(n=320 if W == 1 else 640)
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass)
+    eps2, t2 = H.sq32(0.2), H.sq32(theta)
+    acc, V, A, _, _ = O.bh_traverse(tree, pk, mk, eps2, t2, W=W)
+    bacc, bV, bA = H.brute_group_forces(tree, pk, eps2, t2, W)
+    assert (V, A) == (bV, bA)
+    assert np.array_equal(acc, bacc)
+
+
+def test_bh_theta_zero_is_direct_sum_in_sorted_order():
+    pos, mass = _input(n=512, seed=8)
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass)
+    eps2 = H.sq32(0.2)
+    acc, V, A, _, _ = O.bh_traverse(tree, pk, mk, eps2, 0.0, W=64)
+    order = tree["sorted_src"][tree["sorted_src"] < len(mk)]
+    # direct sum over sources in sorted order == particle nodes in pre-order;
+    # ds[k] is the acceleration of particle order[k]
+    ds = O.ds_accel_f32(pk[order], mk[order], eps2)
+    assert np.array_equal(acc[order], ds)
+    assert A == nin * nin  # every real lane accepts every particle node
+
+
+def test_bh_theta_infinite_is_pull_of_global_com():
+    pos, mass = _input(n=256, seed=9)
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass)
+    eps2 = H.sq32(0.2)
+    acc, V, A, _, _ = O.bh_traverse(tree, pk, mk, eps2, 1e30, W=64)
+    m = mk.astype(np.float64)
+    com = (pk.astype(np.float64) * m[:, None]).sum(0) / m.sum()
+    d = com[None, :] - pk.astype(np.float64)
+    r2 = (d ** 2).sum(1) + eps2
+    ref = d * (m.sum() / r2 ** 1.5)[:, None]
+    assert H.system_rel_err(acc, ref) < 1e-5
+    assert V == A == nin  # one visit per lane: the root
+
+
+def test_bh_error_against_f64_shrinks_with_theta():
+    pos, mass = _input(n=1024, seed=10, spread=20.0)
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass)
+    eps2 = H.sq32(0.2)
+    a64 = O.ds_accel_f64(pk, mk, eps2)
+    errs = []
+    for th in (1.0, 0.5, 0.25):
+        acc, *_ = O.bh_traverse(tree, pk, mk, eps2, H.sq32(th), W=64)
+        errs.append(H.system_rel_err(acc, a64))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 5e-3
+
+
+def test_bh_step_integrates_and_keeps_killed_velocity():
+    pos, mass = _input(n=200, seed=11)
+    pos[0] = [800, 0]
+    pos = np.vstack([pos, np.zeros((56, 2), np.float32)])
+    mass = np.concatenate([mass, np.zeros(56, np.float32)])
+    vel = np.ones_like(pos)
+    po, vo, mo, V, A = O.bh_step(pos, vel, mass, 1.0, 1e-3, H.sq32(0.2), 1.0, 64)
+    assert np.array_equal(po[0], [-1000, -1000]) and mo[0] == 0
+    assert np.array_equal(vo[0], vel[0])
+    # integrate: v' = v + a*(G*dt); p' = p + (v+v')*(0.5*dt), per ForceShader.comp:145-146
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass)
+    acc, *_ = O.bh_traverse(tree, pk, mk, H.sq32(0.2), 1.0, W=64)
+    kv = np.float32(1.0) * np.float32(1e-3)
+    kp = np.float32(0.5) * np.float32(1e-3)
+    nv = (vel[1:] + acc[1:] * kv).astype(np.float32)
+    np_ = (pk[1:] + ((vel[1:] + nv).astype(np.float32) * kp)).astype(np.float32)
+    assert np.array_equal(vo[1:], nv)
+    assert np.array_equal(po[1:], np_)
