@@ -160,6 +160,18 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
                       f"({threads} threads), integrate excluded"}
 
 
+def copy_calibration(mib: int = 256, reps: int = 3) -> None:
+    """Device-to-device copies of a known size, the reference point for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of the kernels above."""
+    import torch
+    a = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB x {reps}")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,7 +180,12 @@ def main() -> int:
     ap.add_argument("--bh-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run known-size device copies (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the simulation step has no multi-GPU path "
               "in this build (see DESIGN.md for how it would shard); run with --gpus 1.",

@@ -340,6 +340,27 @@ gsim_status gsim_set_ds_shape(gsim_sim *s, uint32_t R, uint32_t S) {
     return GSIM_OK;
 }
 
+gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
+    gsim::clear_error();
+    if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_tuning: NULL handle");
+    switch (knob) {
+        case GSIM_TUNE_DS_RCP:
+            s->ds_rcp = value ? 1u : 0u;
+            return GSIM_OK;
+        case GSIM_TUNE_BH_WAVES_PER_SIMD:
+            if (value < 1 || value > 8)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "waves per SIMD must be in [1, 8]");
+            s->trav_waves_per_simd = value;
+            return GSIM_OK;
+        case GSIM_TUNE_BH_TRAVERSAL:
+            if (value > 3) return fail(GSIM_ERROR_INVALID_ARGUMENT, "traversal mode must be 0..3");
+            s->trav_mode = value;
+            return GSIM_OK;
+        default:
+            return fail(GSIM_ERROR_INVALID_ARGUMENT, "unknown tuning knob %d", knob);
+    }
+}
+
 gsim_status gsim_set_timing(gsim_sim *s, int32_t enable) {
     gsim::clear_error();
     if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_timing: NULL handle");

@@ -74,12 +74,14 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
-    BhCounters *__restrict__ ctr) {
+    uint32_t *__restrict__ digit_total, BhCounters *__restrict__ ctr) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < 2 * kTiles) tile_range[i] = 0;
+    if (i < 3 * 256) digit_total[i] = 0;
     if (i == 0) {
         ctr->visits = 0;
         ctr->accepts = 0;
+        ctr->next_group = 0;
     }
     if (i >= na) return;
     const float2 p = pos[i];
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
 
 __global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
-    uint32_t *__restrict__ hist, uint32_t nblk) {
+    uint32_t *__restrict__ hist, uint32_t nblk, uint32_t *__restrict__ digit_total) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -113,7 +115,49 @@ __global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+    const uint32_t c = h[threadIdx.x];
+    hist[threadIdx.x * nblk + blockIdx.x] = c;
+    if (c) atomicAdd(&digit_total[threadIdx.x], c);
+}
+
+// Inclusive scan of one value across a 256-thread workgroup; returns the
+// inclusive prefix and writes the workgroup total.
+__device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t *total,
+                                                  uint32_t *wsum /* [4] LDS */) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t u = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += u;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t w = 0; w < wave; ++w) before += wsum[w];
+    *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return x + before;
+}
+
+// One workgroup per digit: exclusive offsets of that digit for every sort
+// block, starting at the digit's global base (sum of smaller digits' totals).
+__global__ __launch_bounds__(256) void radix_scan_kernel(uint32_t *__restrict__ hist,
+                                                         uint32_t nblk,
+                                                         const uint32_t *__restrict__ digit_total) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t d = blockIdx.x, t = threadIdx.x;
+    uint32_t total = 0;
+    (void)block_scan256(t < d ? digit_total[t] : 0u, &total, wsum);
+    uint32_t carry = total;
+    uint32_t *row = hist + (size_t)d * nblk;
+    for (uint32_t c0 = 0; c0 < nblk; c0 += 256) {
+        const uint32_t i = c0 + t;
+        const uint32_t x = i < nblk ? row[i] : 0u;
+        uint32_t tot = 0;
+        const uint32_t incl = block_scan256(x, &tot, wsum);
+        if (i < nblk) row[i] = carry + incl - x;
+        carry += tot;
+    }
 }
 
 // Exclusive scan of one array per workgroup (blockIdx.x selects a or b), in
@@ -262,7 +306,8 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
     uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
     float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
-    uint32_t *__restrict__ sorted_src, BhCounters *__restrict__ ctr) {
+    uint32_t *__restrict__ sorted_src, uint32_t *__restrict__ breaker,
+    BhCounters *__restrict__ ctr) {
     __shared__ uint32_t wa[4], wb[4];
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     const PosInfo q = pos_info(keys, na, k);
@@ -308,6 +353,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         leaf_first[L] = k;
         leaf_key[L] = q.key;
     }
+    for (uint32_t j = 0; j < q.ns; ++j) breaker[C + j] = k;
     if (q.sk1 < 0) {  // last in-domain position
         ctr->n_in = k + 1;
         ctr->n_cells = C + q.ns;
@@ -317,30 +363,74 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     }
 }
 
-// ---- 5. leaf sums -------------------------------------------------------------
+// ---- 5. leaf sums and particle runs ------------------------------------------
 
-__global__ __launch_bounds__(256) void bh_leafsum_kernel(
+constexpr uint32_t kLeafBlocks = 1024;
+
+// Phase A, one wave per leaf: the sum runs over the leaf's particles in
+// sorted (= ascending index) order, the canonical order of the reference's
+// float atomics; the wave stages 64 products at a time in LDS with one
+// coalesced load and adds them in order.  Phase B, one thread per sorted position: the
+// number of consecutive particle nodes from that particle's slot, i.e. up to
+// the next position where a cell node starts (breaker[C(k+1)]).
+__global__ __launch_bounds__(256) void bh_leaf_kernel(
     const uint32_t *__restrict__ leaf_first, const uint32_t *__restrict__ leaf_key,
-    const float4 *__restrict__ prod, const BhCounters *__restrict__ ctr,
-    float4 *__restrict__ leaf_sum, uint32_t *__restrict__ tile_range) {
-    const uint32_t L = blockIdx.x * 256 + threadIdx.x;
-    const uint32_t nl = ctr->n_leaves;
-    if (L >= nl) return;
-    const uint32_t n_in = ctr->n_in;
-    const uint32_t e = min(leaf_first[L + 1], n_in);
-    const uint32_t f = min(leaf_first[L], e);
-    float sx = 0.0f, sy = 0.0f, sm = 0.0f;
-#pragma unroll 8
-    for (uint32_t j = f; j < e; ++j) {
-        const float4 v = prod[j];
-        sx = sx + v.x;
-        sy = sy + v.y;
-        sm = sm + v.z;
+    const float4 *__restrict__ prod, const uint32_t *__restrict__ keys, uint32_t na,
+    const uint32_t *__restrict__ cpos, const uint32_t *__restrict__ breaker,
+    const BhCounters *__restrict__ ctr, float4 *__restrict__ leaf_sum,
+    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ prun) {
+    __shared__ float4 stage[4][64];
+    const uint32_t n_in = ctr->n_in, nl = ctr->n_leaves, ncells = ctr->n_cells;
+    const uint32_t waves = gridDim.x * 4;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t L = wave_id; L < nl; L += waves) {
+        const uint32_t e = __builtin_amdgcn_readfirstlane(min(leaf_first[L + 1], n_in));
+        const uint32_t f = __builtin_amdgcn_readfirstlane(min(leaf_first[L], e));
+        float sx = 0.0f, sy = 0.0f, sm = 0.0f;
+        // 64 products per coalesced load, the next chunk in flight while the
+        // current one is summed from LDS in order
+        float4 next = f + lane < e ? prod[f + lane] : zero4;
+        for (uint32_t base = f; base < e; base += 64) {
+            const uint32_t cnt = min(64u, e - base);
+            stage[w][lane] = next;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t nb = base + 64 + lane;
+            next = nb < e ? prod[nb] : zero4;
+            uint32_t k = 0;
+            for (; k + 8 <= cnt; k += 8) {
+                float4 q[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) q[u] = stage[w][k + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    sx = sx + q[u].x;
+                    sy = sy + q[u].y;
+                    sm = sm + q[u].z;
+                }
+            }
+            for (; k < cnt; ++k) {
+                const float4 q = stage[w][k];
+                sx = sx + q.x;
+                sy = sy + q.y;
+                sm = sm + q.z;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            leaf_sum[L] = make_float4(sx, sy, sm, __uint_as_float(e - f));
+            const uint32_t T = leaf_key[L] >> 10;
+            if (L == 0 || (leaf_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
+            if (L + 1 == nl || (leaf_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
+        }
     }
-    leaf_sum[L] = make_float4(sx, sy, sm, __uint_as_float(e - f));
-    const uint32_t T = leaf_key[L] >> 10;
-    if (L == 0 || (leaf_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
-    if (L + 1 == nl || (leaf_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n_in; k += gridDim.x * 256) {
+        const PosInfo q = pos_info(keys, na, k);
+        const uint32_t c1 = cpos[k] + q.ns;  // C(k+1)
+        const uint32_t run_end = c1 < ncells ? breaker[c1] : n_in;
+        prun[k + c1] = run_end > k ? run_end - k : 1u;
+    }
 }
 
 // ---- 6./7. dense reduction per tile, then the top ------------------------------
@@ -463,17 +553,34 @@ __global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict
 // ---- 8. traversal + integrate ----------------------------------------------------
 
 // W = 64: one wave is one group, so the node index is wave-uniform and node
-// records arrive through scalar loads.
-template <bool kDebugAccel>
-__global__ __launch_bounds__(256) void bh_traverse64_kernel(
-    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
-    const float2 *__restrict__ vel, const float4 *__restrict__ node4,
-    const uint32_t *__restrict__ ncount, BhCounters *__restrict__ ctr,
-    float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
-    uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp) {
-    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-    if ((slot & ~63u) >= na) return;  // whole wave beyond the last group
-    const uint32_t src = sorted_src[slot];
+// records arrive through scalar loads.  A particle node (radius 0) is accepted
+// by every lane, so a run of consecutive particle nodes (a leaf bucket) is
+// streamed as a plain direct sum without votes; cell nodes are voted on with
+// r2 = (dx*dx + dy*dy) + eps2 rounded exactly as the reference's shader, so
+// the visited and accepted node sets are the reference's.
+__device__ __forceinline__ void accumulate(float nx, float ny, float m, float px, float py,
+                                           float eps2, float &ax, float &ay) {
+    const float dx = nx - px, dy = ny - py;
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
+    const float inv = __builtin_amdgcn_rsqf(r2);
+    const float s = (m * inv) * (inv * inv);
+    ax = __builtin_fmaf(dx, s, ax);
+    ay = __builtin_fmaf(dy, s, ay);
+}
+
+// One group's walk and integrate.  kStaged: leaf-bucket runs come through a
+// per-wave LDS stage (one coalesced 1 KB load per 64 nodes, broadcast reads)
+// instead of scalar loads.
+template <bool kDebugAccel, bool kStaged>
+__device__ __forceinline__ void walk_group(
+    uint32_t g, uint32_t total, float4 *stage, const uint32_t *__restrict__ sorted_src,
+    const float2 *__restrict__ pos, const float2 *__restrict__ vel,
+    const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
+    const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
+    float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
+    float theta2, float kv, float kp, unsigned long long &vis, unsigned long long &acc) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t src = sorted_src[g * 64 + lane];
     const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
@@ -481,26 +588,67 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
         px = p.x;
         py = p.y;
     }
-    // node count is bounded by the node arrays; every iteration advances ind by
-    // at least one, so the walk always ends
-    const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
     const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
     float ax = 0.0f, ay = 0.0f;
     uint32_t ind = 0;
-    unsigned long long vis = 0, acc = 0;
     while (ind < total) {
         const float4 nd = node4[ind];
+        if (nd.w == 0.0f) {
+            uint32_t n = prun[ind];
+            n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), total - ind));
+            const uint32_t end = ind + n;
+            if (kStaged) {
+                for (uint32_t base = ind; base < end; base += 64) {
+                    const uint32_t cnt = min(64u, end - base);
+                    stage[lane] = lane < cnt ? node4[base + lane]
+                                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    __builtin_amdgcn_wave_barrier();
+                    uint32_t k = 0;
+                    for (; k + 8 <= cnt; k += 8) {
+                        float4 q[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) q[u] = stage[k + u];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
+                    }
+                    for (; k < cnt; ++k) {
+                        const float4 q = stage[k];
+                        accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+            } else {
+                uint32_t j = ind;
+                for (; j + 4 <= end; j += 4) {
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = node4[j + u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
+                }
+                for (; j < end; ++j) {
+                    const float4 q = node4[j];
+                    accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
+                }
+            }
+            vis += (unsigned long long)n * nreal;
+            acc += (unsigned long long)n * nreal;
+            ind = end;
+            continue;
+        }
         const uint32_t cnt = ncount[ind];
         const float dx = nd.x - px;
         const float dy = nd.y - py;
-        const float r2 = (dx * dx + dy * dy) + eps2;
+        const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
         const bool ok = (nd.w <= r2 * theta2) || !real;
         vis += nreal;
         if (__ballot(ok) == ~0ull) {
             const float inv = __builtin_amdgcn_rsqf(r2);
-            const float s = ((nd.z * inv) * inv) * inv;
-            ax = ax + dx * s;
-            ay = ay + dy * s;
+            const float s = (nd.z * inv) * (inv * inv);
+            ax = __builtin_fmaf(dx, s, ax);
+            ay = __builtin_fmaf(dy, s, ay);
             ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
             acc += nreal;
         } else {
@@ -514,7 +662,45 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
         vel_out[src] = make_float2(nvx, nvy);
         if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
     }
-    if ((threadIdx.x & 63) == 0 && nreal) {
+}
+
+// kPersistent: waves take groups from a device counter until all na/64 are
+// done (every wave leaves once the counter passes the last group); otherwise
+// wave i walks group i.
+template <bool kDebugAccel, bool kStaged, bool kPersistent>
+__global__ __launch_bounds__(256) void bh_traverse64_kernel(
+    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun,
+    BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
+    float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
+    float kv, float kp) {
+    __shared__ float4 stage[kStaged ? 4 : 1][64];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t groups = na / 64;
+    // node count is bounded by the node arrays; every iteration advances ind by
+    // at least one, so each walk ends
+    const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
+    unsigned long long vis = 0, acc = 0;
+    float4 *st = stage[kStaged ? w : 0];
+    if (kPersistent) {
+        while (true) {
+            uint32_t g = 0;
+            if (lane == 0) g = atomicAdd(&ctr->next_group, 1u);
+            g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+            if (g >= groups) break;
+            walk_group<kDebugAccel, kStaged>(g, total, st, sorted_src, pos, vel, node4, ncount,
+                                             prun, pos_out, vel_out, acc_dbg, na, eps2, theta2,
+                                             kv, kp, vis, acc);
+        }
+    } else {
+        const uint32_t g = blockIdx.x * 4 + w;
+        if (g >= groups) return;  // whole wave beyond the last group
+        walk_group<kDebugAccel, kStaged>(g, total, st, sorted_src, pos, vel, node4, ncount, prun,
+                                         pos_out, vel_out, acc_dbg, na, eps2, theta2, kv, kp,
+                                         vis, acc);
+    }
+    if (lane == 0 && vis) {
         atomicAdd(&ctr->visits, vis);
         atomicAdd(&ctr->accepts, acc);
     }
@@ -632,6 +818,9 @@ hipError_t bh_alloc(BhBuffers *b, size_t na) {
         GSIM_ALLOC(b->idx[k], na);
     }
     GSIM_ALLOC(b->hist, 256 * b->sort_blocks);
+    GSIM_ALLOC(b->digit_total, 3 * 256);
+    GSIM_ALLOC(b->breaker, b->cap);
+    GSIM_ALLOC(b->prun, b->cap);
     GSIM_ALLOC(b->block_cells, b->prep_blocks);
     GSIM_ALLOC(b->block_leaves, b->prep_blocks);
     GSIM_ALLOC(b->sdep, na + 1);
@@ -658,7 +847,7 @@ void bh_free(BhBuffers *b) {
         (void)hipFree(b->keys[k]);
         (void)hipFree(b->idx[k]);
     }
-    void *ptrs[] = {b->hist, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
+    void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->prun, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->sorted_src, b->ctr};
     for (void *p : ptrs) (void)hipFree(p);
@@ -672,15 +861,18 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     const uint32_t g256 = (na + 255) / 256;
     const uint32_t gkeys = g256 > (2 * kTiles + 255) / 256 ? g256 : (2 * kTiles + 255) / 256;
     bh_keys_kernel<<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
-                                                 b.keys[0], b.idx[0], b.tile_range, b.ctr);
+                                                 b.keys[0], b.idx[0], b.tile_range,
+                                                 b.digit_total, b.ctr);
     // keys[0] -> keys[1] -> keys[2] -> keys[1]; keys[0] keeps the per-particle cells
     const int in_of[3] = {0, 1, 2}, out_of[3] = {1, 2, 1};
     const uint32_t nblk = (uint32_t)b.sort_blocks;
     for (int pass = 0; pass < 3; ++pass) {
         const uint32_t shift = 8u * pass;
         const int i = in_of[pass], o = out_of[pass];
-        radix_upsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(b.keys[i], na, shift, b.hist, nblk);
-        scan_exclusive_kernel<<<1, 1024, 0, s->stream>>>(b.hist, nullptr, 256 * nblk);
+        uint32_t *totals = b.digit_total + 256 * pass;
+        radix_upsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(b.keys[i], na, shift, b.hist,
+                                                                   nblk, totals);
+        radix_scan_kernel<<<256, 256, 0, s->stream>>>(b.hist, nblk, totals);
         radix_downsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
             b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, shift, b.hist, nblk);
     }
@@ -692,9 +884,10 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, na, b.block_cells,
                                               b.block_leaves, b.sdep, b.cpos, b.prod,
                                               b.leaf_first, b.leaf_key, b.node4, b.ncount,
-                                              b.sorted_src, b.ctr);
-    bh_leafsum_kernel<<<pb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, b.ctr,
-                                                 b.leaf_sum, b.tile_range);
+                                              b.sorted_src, b.breaker, b.ctr);
+    const uint32_t lb = pb < kLeafBlocks ? pb : kLeafBlocks;
+    bh_leaf_kernel<<<lb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, sk, na, b.cpos,
+                                              b.breaker, b.ctr, b.leaf_sum, b.tile_range, b.prun);
     bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
                                                   b.leaf_sum, b.sdep, b.cpos, b.node4,
                                                   b.ncount, b.top);
@@ -710,14 +903,27 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     const bool dbg = (s->debug_flags & GSIM_DEBUG_ACCEL) != 0;
     float2 *acc = dbg ? s->accel : nullptr;
     if (s->W == 64) {
-        if (dbg)
-            bh_traverse64_kernel<true><<<grid, 256, 0, s->stream>>>(
-                b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
-        else
-            bh_traverse64_kernel<false><<<grid, 256, 0, s->stream>>>(
-                b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+        const uint32_t groups = na / 64;
+        const bool persistent = (s->trav_mode & 1u) != 0, staged = (s->trav_mode & 2u) != 0;
+        uint32_t g = grid;
+        if (persistent) {
+            // waves_per_simd waves on each of the 1024 SIMDs, at most one per group
+            uint32_t waves = 1024u * s->trav_waves_per_simd;
+            if (waves > groups) waves = groups;
+            g = (waves + 3) / 4;
+        }
+#define GSIM_TRAV(D, S, P)                                                                 \
+    bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
+        b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc, na, \
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp)
+        if (dbg) {
+            if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
+            else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }
+        } else {
+            if (persistent) { if (staged) GSIM_TRAV(false, true, true); else GSIM_TRAV(false, false, true); }
+            else { if (staged) GSIM_TRAV(false, true, false); else GSIM_TRAV(false, false, false); }
+        }
+#undef GSIM_TRAV
     } else {
         if (dbg)
             bh_traverse32_kernel<true><<<grid, 256, 0, s->stream>>>(

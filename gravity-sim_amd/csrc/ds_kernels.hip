@@ -17,7 +17,10 @@ namespace {
 
 constexpr int kDsThreads = 256;
 
-template <int R>
+// kRcp: 1/r2 comes from v_rcp_f32 (the transcendental unit, which issues
+// beside the VALU on gfx950) instead of inv*inv, one VALU multiply less per
+// interaction.
+template <int R, bool kRcp>
 __global__ __launch_bounds__(kDsThreads) void ds_partial_kernel(
     const float2 *__restrict__ pos, const float *__restrict__ mass,
     float2 *__restrict__ partial, uint32_t na, uint32_t jlen, float eps2) {
@@ -45,7 +48,7 @@ __global__ __launch_bounds__(kDsThreads) void ds_partial_kernel(
                 const float dy = s.y - py[r];
                 const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
                 const float inv = __builtin_amdgcn_rsqf(r2);
-                const float inv2 = inv * inv;
+                const float inv2 = kRcp ? __builtin_amdgcn_rcpf(r2) : inv * inv;
                 const float t = (m * inv) * inv2;
                 ax[r] = __builtin_fmaf(dx, t, ax[r]);
                 ay[r] = __builtin_fmaf(dy, t, ay[r]);
@@ -82,13 +85,14 @@ __global__ __launch_bounds__(256) void ds_combine_kernel(
 
 }  // namespace
 
-// Launch shape by size: enough workgroups to fill 256 CUs several times over
-// while each slice stays a multiple of 64 sources.
+// Launch shape by size: two targets per lane (each source load then feeds two
+// interactions), and enough source slices for ~6k workgroups so 256 CUs stay
+// full to the end; each slice keeps >= 512 sources.  Measured at 50,048
+// particles: 0.546 ms/step for 2 x 64 against 0.721 ms for 1 x 16.
 void ds_pick_shape(size_t na, uint32_t *R, uint32_t *S) {
-    const uint32_t r = 1;
+    const uint32_t r = 2;
     const size_t tblocks = (na + kDsThreads * r - 1) / (kDsThreads * r);
-    // aim for >= ~4096 workgroups, but keep slices >= 512 sources
-    size_t s = (4096 + tblocks - 1) / tblocks;
+    size_t s = (6144 + tblocks - 1) / tblocks;
     const size_t max_s = na / 512 ? na / 512 : 1;
     if (s > max_s) s = max_s;
     if (s < 1) s = 1;
@@ -104,12 +108,16 @@ hipError_t ds_launch_partial(gsim_sim *s, const float2 *pos, const float *mass) 
     uint32_t jlen = (uint32_t)(((na + S - 1) / S + 63) / 64 * 64);
     const uint32_t slices = (na + jlen - 1) / jlen;
     dim3 grid((na + kDsThreads * R - 1) / (kDsThreads * R), slices);
+#define GSIM_DS(RR, RCP) \
+    ds_partial_kernel<RR, RCP><<<grid, kDsThreads, 0, s->stream>>>(pos, mass, s->partial, na, jlen, s->eps2)
+    const bool rcp = s->ds_rcp != 0;
     switch (R) {
-        case 1: ds_partial_kernel<1><<<grid, kDsThreads, 0, s->stream>>>(pos, mass, s->partial, na, jlen, s->eps2); break;
-        case 2: ds_partial_kernel<2><<<grid, kDsThreads, 0, s->stream>>>(pos, mass, s->partial, na, jlen, s->eps2); break;
-        case 4: ds_partial_kernel<4><<<grid, kDsThreads, 0, s->stream>>>(pos, mass, s->partial, na, jlen, s->eps2); break;
+        case 1: if (rcp) GSIM_DS(1, true); else GSIM_DS(1, false); break;
+        case 2: if (rcp) GSIM_DS(2, true); else GSIM_DS(2, false); break;
+        case 4: if (rcp) GSIM_DS(4, true); else GSIM_DS(4, false); break;
         default: return hipErrorInvalidValue;
     }
+#undef GSIM_DS
     s->ds_S_used = slices;
     return hipGetLastError();
 }

@@ -5,6 +5,7 @@
 //   direct sum      partial float2[S*Na] per-slice partial accelerations
 //   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
 //                   cpos u32[Na], prod float4[Na], leaf records, tile results,
+//                   breaker u32[cap], prun u32[cap] (particle-run lengths),
 //                   node4 float4[cap] {x, y, mass, radius}, ncount u32[cap],
 //                   sorted_src u32[Na], device counters
 #pragma once
@@ -35,6 +36,8 @@ struct BhCounters {
     uint32_t n_leaves;  // occupied depth-10 cells
     unsigned long long visits;   // real-lane node visits of the traversal
     unsigned long long accepts;  // real-lane acceptances
+    uint32_t next_group;         // work counter of the persistent traversal
+    uint32_t pad[3];
 };
 
 struct TileCell {  // one cell of the dense tile/top reduction
@@ -45,7 +48,10 @@ struct TileCell {  // one cell of the dense tile/top reduction
 struct BhBuffers {
     size_t na = 0, cap = 0, sort_blocks = 0, prep_blocks = 0;
     uint32_t *keys[3] = {}, *idx[3] = {};
-    uint32_t *hist = nullptr;        // [256 * sort_blocks]
+    uint32_t *hist = nullptr;        // [256 * sort_blocks] per-block digit counts
+    uint32_t *digit_total = nullptr; // [3 * 256] per-pass digit totals
+    uint32_t *breaker = nullptr;     // [cap] sorted position where cell node #o starts
+    uint32_t *prun = nullptr;        // [cap] particle nodes left in the run at a slot
     uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
     uint32_t *block_leaves = nullptr;
     int8_t *sdep = nullptr;          // [Na] shared depth with the previous sorted key
@@ -82,6 +88,9 @@ struct gsim_sim {
     // (DirectSimulation.cpp:179, BarnesHutSimulation.cpp:575-578).
     float eps2 = 0, theta2 = 0, dt = 0, kv = 0, kp = 0;
     uint32_t W = 64;
+    uint32_t trav_waves_per_simd = 8;  // persistent traversal width
+    uint32_t trav_mode = 2;            // bit 0 persistent, bit 1 LDS-staged runs
+    uint32_t ds_rcp = 0;               // direct sum: 1/r2 from the transcendental unit
     // state
     float2 *pos[2] = {}, *vel[2] = {};
     float *mass = nullptr;

@@ -46,6 +46,7 @@ DUMP_CELLS, DUMP_MORTON30, DUMP_SORTED_SRC = 1, 2, 3
 DUMP_COUNTS, DUMP_RADII, DUMP_NODE_POS, DUMP_NODE_MASS = 4, 5, 6, 7
 DUMP_NODE_TOTAL, DUMP_ACCEL, DUMP_COUNTERS = 8, 9, 10
 DEBUG_ACCEL = 1
+TUNE_DS_RCP, TUNE_BH_WAVES_PER_SIMD, TUNE_BH_TRAVERSAL = 1, 2, 3
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
           4: "NO_DEVICE", 5: "IO", 6: "CAPACITY"}
@@ -113,6 +114,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_set_timing": (st, [vp, C.c_int32]),
         "gsim_get_timing": (st, [vp, C.POINTER(Timing), C.c_int32]),
         "gsim_set_ds_shape": (st, [vp, C.c_uint32, C.c_uint32]),
+        "gsim_set_tuning": (st, [vp, C.c_int32, C.c_uint32]),
         "gsim_generate": (st, [C.c_int32, C.c_size_t, C.c_float, C.c_float, C.c_float,
                                C.c_float, C.c_uint64, fp, fp, fp]),
         "gsim_count_particles": (st, [C.c_char_p, C.POINTER(C.c_size_t)]),
@@ -265,6 +267,9 @@ class Simulation:
 
     def set_ds_shape(self, targets_per_lane: int, slices: int) -> None:
         _check(_lib.gsim_set_ds_shape(self._h, targets_per_lane, slices))
+
+    def set_tuning(self, knob: int, value: int) -> None:
+        _check(_lib.gsim_set_tuning(self._h, knob, value))
 
     def dump(self, what: int) -> np.ndarray:
         written = C.c_size_t(0)

@@ -156,6 +156,15 @@ gsim_status gsim_get_timing(gsim_sim *sim, gsim_timing *out, int32_t reset);
 gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
                               uint32_t slices);
 
+/* Launch tuning that never changes results beyond the documented tolerance
+ * (visit/accept counts and trees are unaffected by every knob). */
+typedef enum gsim_tuning {
+    GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
+    GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
+    GSIM_TUNE_BH_TRAVERSAL = 3        /* bit 0 persistent groups, bit 1 LDS-staged runs */
+} gsim_tuning;
+gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
+
 /* ---- particle sets: presets and the text format --------------------------- */
 
 /* Fills n particles (pos_xy[2n], vel_xy[2n], mass[n]) from a preset with an
