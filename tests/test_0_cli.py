@@ -1,0 +1,85 @@
+"""The gravity-sim command line (the reference's headless Main.cpp surface).
+
+This file is named to run first: it starts the CLI as a child process, which
+must happen before this pytest process makes any HIP call.
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import gsim
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "gravity-sim_amd", "bin", "gravity-sim")
+
+
+def run(*args, timeout=120):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_help_prints_and_exits():
+    r = run("--help")
+    assert r.returncode == 0
+    assert "--simulation-algorithm" in r.stdout and "--particles-out" in r.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    ([], "If a particle input file is not provided, valid generation args must be provided!"),
+    (["--particle-count=10", "--generate-type=random", "--generate-size=5", "--min-mass=1",
+      "--max-mass=2"], "A valid simulation algorithm must be given!"),
+    (["--particle-count=10", "--generate-type=random", "--generate-size=5", "--min-mass=1",
+      "--max-mass=2", "--simulation-algorithm=direct-sum", "--no-graphics"],
+     "If --no-graphics was specified, a valid simulation count must be given!"),
+    (["--particle-count=10", "--generate-type=nope", "--generate-size=5", "--min-mass=1",
+      "--max-mass=2", "--simulation-algorithm=direct-sum"],
+     "If a particle input file is not provided"),
+])
+def test_invalid_arguments_are_fatal(args, msg):
+    r = run(*args)
+    assert r.returncode != 0  # FATAL aborts, Logger.cpp:46-47
+    assert "[FATAL ERROR] " + msg in r.stdout
+
+
+def test_log_file_and_generation_warning(tmp_path):
+    pfile = tmp_path / "p.txt"
+    pfile.write_text("1 2 0 0 1\n")
+    log = tmp_path / "log.txt"
+    r = run(f"--particles-in={pfile}", "--particle-count=5", f"--log-file={log}",
+            "--simulation-algorithm=barnes-hut")
+    assert "[WARNING]     A particle input file was provided" in r.stdout
+    assert "[WARNING]     A particle input file was provided" in log.read_text()
+
+
+def _final_state(algo, pin, steps, extra=()):
+    p = gsim.load_particles(str(pin))
+    with gsim.Simulation(p, algo, **dict(extra)) as s:
+        s.step(steps)
+        return s.read()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("algo", ["direct-sum", "barnes-hut"])
+def test_cli_run_matches_library(tmp_path, algo):
+    """The CLI (file in, 250 steps in batches of 100, file out) produces the
+    same final state as the library driven directly from the same file."""
+    pin = tmp_path / "in.txt"
+    pout = tmp_path / "out.txt"
+    gsim.save_particles(str(pin), gsim.generate("galaxy-collision", 3000, 150.0, 1.0, 10.0,
+                                                seed=7))
+    r = run(f"--particles-in={pin}", f"--particles-out={pout}", f"--simulation-algorithm={algo}",
+            "--simulation-count=250", "--no-graphics", "--benchmark", "--simulation-speed=2")
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert re.search(r"\[INFO\]        Total simulation runtime: [\d.]+(ms|s)", r.stdout)
+    assert re.search(r"\[INFO\]        Average runtime/simulation: [\d.]+(us|ms|s)", r.stdout)
+    got = gsim.load_particles(str(pout))
+    # the library run is started only now, after the child process has exited
+    ref = _final_state(algo, pin, 250, {"simulation_speed": 2.0})
+    assert got.n == 3000
+    # the file holds %.7f, so compare at that resolution
+    assert np.allclose(got.pos, ref.pos, atol=1e-6, rtol=0)
+    assert np.allclose(got.vel, ref.vel, atol=1e-6, rtol=0)
+    assert np.allclose(got.mass, ref.mass, atol=1e-6, rtol=0)
