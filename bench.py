@@ -133,6 +133,68 @@ def bench_barnes_hut(steps: int, warmup: int) -> dict:
     }
 
 
+def batched_systems(count: int = 256, seed: int = 4):
+    """The batched workload: `count` systems, N log-uniform in [1k, 64k],
+    random-disk or exponential-disk (radius 100) chosen per system, masses
+    U[1,10]; N < 8192 -> direct sum, else Barnes-Hut with theta = 0.7."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(64000))))
+        kind = "random" if rng.uniform() < 0.5 else "exponential-disk"
+        p = gsim.generate(kind, n, 100.0, 1.0, 10.0, 1.0, seed=1000 + k)
+        algo = "direct-sum" if n < 8192 else "barnes-hut"
+        out.append((p, algo, {"accuracy_parameter": 0.7} if algo == "barnes-hut" else {}))
+    return out
+
+
+def bench_batched(steps: int, serial_steps: int = 5) -> dict:
+    systems = batched_systems()
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    batch = gsim.Batch(cap, max_systems=len(systems))
+    for p, algo, kw in systems:
+        batch.add(p, algo, **kw)
+    waves = batch.autotune(steps_per_trial=2)
+    batch.step(2)
+    batch.synchronize()
+    t0 = time.perf_counter()
+    batch.step(steps)
+    batch.synchronize()
+    t_batch = time.perf_counter() - t0
+    batch.close()
+    # the same systems one by one through the plain single-system path
+    sims = [gsim.Simulation(p, algo, **kw) for p, algo, kw in systems]
+    for s in sims:
+        s.step(1)
+    for s in sims:
+        s.synchronize()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(serial_steps)
+        s.synchronize()
+    t_serial = time.perf_counter() - t0
+    for s in sims:
+        s.close()
+    n_ds = sum(1 for _, a, _ in systems if a == "direct-sum")
+    batched = len(systems) * steps / t_batch
+    serial = len(systems) * serial_steps / t_serial
+    return {
+        "metric": "batched system-steps/s (256 mixed systems)",
+        "value": batched,
+        "unit": "system-steps/s",
+        "higher_is_better": True,
+        "ms_per_step": t_batch * 1e3 / steps,
+        "one_by_one_value": serial,
+        "speedup_vs_one_by_one": batched / serial,
+        "particles": int(sum(p.n for p, _, _ in systems)),
+        "config": {"workload": f"{len(systems)} systems, N log-uniform [1k, 64k], random or "
+                               f"exponential disk (radius 100), N < 8192 direct sum ({n_ds}), "
+                               f"else Barnes-Hut theta=0.7 ({len(systems) - n_ds}); one hipGraph "
+                               f"replay per step", "steps": steps,
+                   "one_by_one_steps": serial_steps, "waves_per_simd": waves},
+    }
+
+
 def cpu_baseline(budget_s: float = 12.0) -> dict:
     """The oracle's FP32 direct sum (scalar C, one thread per core slice) on
     the same 50k workload, timed on this host: a bounded sample of whole
@@ -180,6 +242,8 @@ def main() -> int:
     ap.add_argument("--bh-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run known-size device copies (counter calibration)")
     args = ap.parse_args()
@@ -200,6 +264,12 @@ def main() -> int:
         bh = bench_barnes_hut(args.bh_steps, max(args.warmup, 1))
         log(f"  {bh['value']:.4f} ms/step (build {bh['build_ms']:.4f}, traverse "
             f"{bh['traverse_ms']:.4f})")
+    bt = None
+    if not args.no_batch:
+        log(f"batched 256 systems: steps {args.batch_steps}")
+        bt = bench_batched(args.batch_steps)
+        log(f"  {bt['value']:.0f} system-steps/s ({bt['ms_per_step']:.3f} ms/step), "
+            f"{bt['speedup_vs_one_by_one']:.2f}x one by one")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
@@ -225,7 +295,7 @@ def main() -> int:
                    "particles": DS_N, "global_batch": 1, "parallelism": "single-gpu"},
         "roofline": ds["roofline"],
         "cpu_baseline": cpu,
-        "secondary": [bh] if bh else [],
+        "secondary": [x for x in (bh, bt) if x],
     }
     print(json.dumps(line), flush=True)
     return 0

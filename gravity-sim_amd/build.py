@@ -37,6 +37,8 @@ DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
 SOURCES = [
     (os.path.join(CSRC, "ds_kernels.hip"), "hip"),
     (os.path.join(CSRC, "bh_kernels.hip"), "hip"),
+    (os.path.join(CSRC, "batch_kernels.hip"), "hip"),
+    (os.path.join(CSRC, "batch.cpp"), "host"),
     (os.path.join(CSRC, "api.cpp"), "host"),
     (os.path.join(HOST, "particles.cpp"), "host"),
 ]
@@ -57,7 +59,7 @@ def _compile(src: str, kind: str) -> str:
     if kind == "hip":
         cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, "-c", src, "-o", obj]
     else:
-        cmd = [HIPCC, *COMMON, "-c", src, "-o", obj]
+        cmd = [HIPCC, f"--offload-arch={ARCH}", "--cuda-host-only", *COMMON, "-c", src, "-o", obj]
     _run(cmd)
     return obj
 

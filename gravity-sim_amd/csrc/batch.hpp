@@ -1,0 +1,112 @@
+// Batched multi-system stepping: many independent systems of uneven size in
+// one device arena, all stepped by the same kernels (one launch per stage for
+// every system), replayed from a hipGraph.  Sizes and membership live in
+// device memory, so the captured graph stays valid while systems come and go
+// within the arena's capacity.
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace gsim {
+
+// One system's slot in the arena and its constants (host mirror + device copy).
+struct SysDesc {
+    uint32_t off, na, n, algo;       // arena offset (multiple of 64), padded/real count
+    float eps2, theta2, kv, kp;      // softening^2, theta^2, velocity and position factors
+    float dt, pad0;
+    uint32_t bh_rank, sorted_base;   // Barnes-Hut: rank in the global sort, first sorted position
+    uint32_t ds_part_off, ds_jlen;   // direct sum: partial-sum offset (float2) and slice length
+    uint32_t ds_slices, pad1;
+};
+
+// Per Barnes-Hut rank, written by the emit kernel every step.
+struct RankStats {
+    uint32_t root;      // first node slot of the system's tree
+    uint32_t end_slot;  // one past its last node slot
+    uint32_t n_in;      // particles inside the domain
+    uint32_t pad;
+};
+
+struct BatchCounts {
+    uint32_t n_bh;            // Barnes-Hut systems (ranks)
+    uint32_t n_bh_particles;  // sum of their padded counts = sorted positions
+    uint32_t n_ds;            // direct-sum systems
+    uint32_t n_ds_particles;  // sum of their padded counts
+    uint32_t n_items;         // work items of the force launch
+    uint32_t n_cells;         // cell nodes of all trees (emit)
+    uint32_t n_leaves;        // occupied leaves of all trees (emit)
+    uint32_t pad;
+};
+
+constexpr uint32_t kBatchRankShift = 21;                   // key = rank << 21 | cell (bit 20 = killed)
+constexpr uint32_t kBatchMaxSystems = 1u << (32 - kBatchRankShift);  // 2048
+constexpr uint32_t kBatchSentinel = 0xffffffffu;
+
+struct BatchDevice {
+    // arena
+    float2 *pos[2] = {}, *vel[2] = {};
+    float *mass = nullptr;
+    float2 *partial = nullptr;
+    // membership
+    SysDesc *desc = nullptr;
+    uint32_t *bh_sys = nullptr, *bh_prefix = nullptr;  // [max_systems (+1)]
+    uint32_t *ds_sys = nullptr, *ds_prefix = nullptr;
+    uint2 *items = nullptr;
+    BatchCounts *counts = nullptr;
+    RankStats *stats = nullptr;
+    // Barnes-Hut scratch over all sorted positions
+    uint32_t *keys[2] = {}, *idx[2] = {};
+    uint32_t *hist = nullptr, *digit_total = nullptr;
+    uint32_t *block_cells = nullptr, *block_leaves = nullptr;
+    int8_t *sdep = nullptr;
+    uint32_t *cpos = nullptr, *sorted_src = nullptr;
+    float4 *prod = nullptr, *leaf_sum = nullptr;
+    uint32_t *leaf_first = nullptr, *leaf_key = nullptr;
+    uint32_t *tile_range = nullptr;
+    TileCell *top = nullptr;
+    // nodes of all trees
+    float4 *node4 = nullptr;
+    uint32_t *ncount = nullptr, *prun = nullptr, *breaker = nullptr;
+};
+
+struct BatchSystem {
+    gsim_params params;
+    uint32_t off = 0, na = 0, n = 0;
+    size_t node_reserve = 0;
+    uint64_t steps = 0;
+};
+
+}  // namespace gsim
+
+struct gsim_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t capacity = 0;        // particles (padded) in the arena
+    size_t node_capacity = 0;   // node slots
+    size_t partial_capacity = 0;
+    size_t item_capacity = 0;
+    uint32_t max_systems = 0;
+    size_t tiles_cap = 0, blocks_cap = 0;
+    gsim::BatchDevice d;
+    std::map<uint32_t, gsim::BatchSystem> systems;  // id -> system
+    std::map<uint32_t, uint32_t> free_ranges;       // arena offset -> length (units of particles)
+    uint32_t next_id = 1;
+    size_t nodes_reserved = 0, partial_used = 0;
+    bool dirty = true;
+    int cur = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    uint32_t waves_per_simd = 8;
+    uint64_t steps = 0;
+    // host mirror of the membership arrays
+    std::vector<gsim::SysDesc> h_desc;
+    gsim::BatchCounts h_counts{};
+};
+
+namespace gsim {
+
+hipError_t batch_launch_step(gsim_batch *b, int in, int out);
+
+}  // namespace gsim

@@ -1,0 +1,692 @@
+// Batched stepping kernels: every kernel covers all systems of a batch and
+// reads its sizes from device memory (BatchCounts), so a step captured once
+// in a hipGraph serves any membership that fits the arena.
+//
+// Barnes-Hut systems share one stable radix sort on (rank << 21 | cell):
+// sorted positions group by system, and because node slots are
+// position + scanned cell-node count, each system's pre-order tree lands
+// contiguously in one global node array ([root, end_slot) per rank, recorded
+// by the emit kernel).  The per-system semantics are those of the
+// single-system kernels in bh_kernels.hip (same cells, same canonical order,
+// same reduction association, same vote), so a system's result in a batch is
+// bit-identical to its result alone.  Direct-sum systems use the same slice
+// split as the single-system path for their size, so they match bit for bit
+// as well.
+#include "batch.hpp"
+#include "bh_device.hpp"
+
+#include <algorithm>
+
+namespace gsim {
+namespace {
+
+constexpr int kSortTile = 2048;
+constexpr uint32_t kCellMask = (1u << 20) - 1u;
+constexpr uint32_t kKilledBit = 1u << 20;
+
+// Largest r in [0, n) with prefix[r] <= f (prefix ascending, prefix[0] = 0).
+__device__ __forceinline__ uint32_t rank_of(const uint32_t *__restrict__ prefix, uint32_t n,
+                                            uint32_t f) {
+    uint32_t lo = 0, hi = n;  // answer in [lo, hi)
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (prefix[mid] <= f) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// ---- keys ------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void bb_keys_kernel(
+    const SysDesc *__restrict__ desc, const uint32_t *__restrict__ bh_sys,
+    const uint32_t *__restrict__ bh_prefix, BatchCounts *__restrict__ counts,
+    float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
+    const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t *__restrict__ keys,
+    uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
+    RankStats *__restrict__ stats, uint32_t *__restrict__ digit_total) {
+    const uint32_t nb = counts->n_bh, n = counts->n_bh_particles;
+    const uint32_t tid = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+    for (uint32_t t = tid; t < nb * 2 * kTiles; t += stride) tile_range[t] = 0;
+    for (uint32_t t = tid; t < nb; t += stride) stats[t] = RankStats{0u, 0u, 0u, 0u};
+    for (uint32_t t = tid; t < 4 * 256; t += stride) digit_total[t] = 0;
+    if (tid == 0) {
+        counts->n_cells = 0;
+        counts->n_leaves = 0;
+    }
+    for (uint32_t f = tid; f < n; f += stride) {
+        const uint32_t r = rank_of(bh_prefix, nb, f);
+        const SysDesc &d = desc[bh_sys[r]];
+        const uint32_t i = d.off + (f - bh_prefix[r]);
+        const float2 p = pos[i];
+        uint32_t c = cell20(p.x, p.y);
+        if (c == 0xffffffffu) {  // InitShader.comp:110-117
+            const float k = -kSimSize * 2.0f;
+            pos[i] = make_float2(k, k);
+            pos_out[i] = make_float2(k, k);
+            mass[i] = 0.0f;
+            vel_out[i] = vel[i];
+            c = kKilledBit;
+        }
+        keys[f] = (r << kBatchRankShift) | c;
+        idx[f] = i;
+    }
+}
+
+// ---- stable LSD radix sort over all sorted positions (4 x 8 bits) -----------
+
+__global__ __launch_bounds__(256) void bb_upsweep_kernel(const uint32_t *__restrict__ keys,
+                                                         const BatchCounts *__restrict__ counts,
+                                                         uint32_t shift, uint32_t *__restrict__ hist,
+                                                         uint32_t *__restrict__ digit_total) {
+    __shared__ uint32_t h[256];
+    const uint32_t n = counts->n_bh_particles;
+    const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        h[threadIdx.x] = 0;
+        __syncthreads();
+        for (int r = 0; r < kSortTile / 256; ++r) {
+            const uint32_t i = tile * kSortTile + r * 256 + threadIdx.x;
+            if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        const uint32_t c = h[threadIdx.x];
+        hist[threadIdx.x * ntiles + tile] = c;
+        if (c) atomicAdd(&digit_total[threadIdx.x], c);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void bb_digit_scan_kernel(uint32_t *__restrict__ hist,
+                                                            const BatchCounts *__restrict__ counts,
+                                                            const uint32_t *__restrict__ digit_total) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t ntiles = (counts->n_bh_particles + kSortTile - 1) / kSortTile;
+    const uint32_t d = blockIdx.x, t = threadIdx.x;
+    uint32_t total = 0;
+    (void)block_scan256(t < d ? digit_total[t] : 0u, &total, wsum);
+    uint32_t carry = total;
+    uint32_t *row = hist + (size_t)d * ntiles;
+    for (uint32_t c0 = 0; c0 < ntiles; c0 += 256) {
+        const uint32_t i = c0 + t;
+        const uint32_t x = i < ntiles ? row[i] : 0u;
+        uint32_t tot = 0;
+        const uint32_t incl = block_scan256(x, &tot, wsum);
+        if (i < ntiles) row[i] = carry + incl - x;
+        carry += tot;
+    }
+}
+
+__global__ __launch_bounds__(256) void bb_downsweep_kernel(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
+    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out,
+    const BatchCounts *__restrict__ counts, uint32_t shift, const uint32_t *__restrict__ hist) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[4][256];
+    __shared__ uint32_t base_off[256];
+    const uint32_t n = counts->n_bh_particles;
+    const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        run[t] = 0;
+        for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
+        base_off[t] = hist[t * ntiles + tile];
+        __syncthreads();
+        for (int r = 0; r < kSortTile / 256; ++r) {
+            const uint32_t i = tile * kSortTile + r * 256 + t;
+            const bool valid = i < n;
+            const uint32_t key = valid ? keys_in[i] : 0u;
+            const uint32_t val = valid ? idx_in[i] : 0u;
+            const uint32_t d = (key >> shift) & 255u;
+            uint64_t m = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; ++bit) {
+                const bool set = (d >> bit) & 1u;
+                const uint64_t bb = __ballot(set);
+                m &= set ? bb : ~bb;
+            }
+            const uint32_t rank = (uint32_t)__popcll(m & below);
+            if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
+            __syncthreads();
+            if (valid) {
+                uint32_t off = base_off[d] + run[d] + rank;
+                for (uint32_t w = 0; w < wave; ++w) off += wcnt[w][d];
+                keys_out[off] = key;
+                idx_out[off] = val;
+            }
+            __syncthreads();
+            uint32_t tot = 0;
+            for (int w = 0; w < 4; ++w) {
+                tot += wcnt[w][t];
+                wcnt[w][t] = 0;
+            }
+            run[t] += tot;
+            __syncthreads();
+        }
+    }
+}
+
+// ---- slot scan and emission ----------------------------------------------------
+
+struct BPos {
+    int sk, sk1;
+    uint32_t ns, lf, rank, key;
+    bool inside;
+};
+
+// s(k) and s(k+1) never cross a system boundary: the first position of a
+// system behaves like position 0 of a single system, its last in-domain
+// position like the last one.
+__device__ __forceinline__ BPos bpos_info(const uint32_t *__restrict__ keys, uint32_t n,
+                                          uint32_t k) {
+    BPos q{-1, -1, 0u, 0u, 0u, 0u, false};
+    if (k >= n) return q;
+    q.key = keys[k];
+    q.rank = q.key >> kBatchRankShift;
+    if (q.key & kKilledBit) return q;
+    q.inside = true;
+    if (k > 0) {
+        const uint32_t pk = keys[k - 1];
+        if ((pk >> kBatchRankShift) == q.rank) q.sk = shared_depth(pk & kCellMask, q.key & kCellMask);
+    }
+    if (k + 1 < n) {
+        const uint32_t nk = keys[k + 1];
+        if ((nk >> kBatchRankShift) == q.rank && !(nk & kKilledBit))
+            q.sk1 = shared_depth(q.key & kCellMask, nk & kCellMask);
+    }
+    q.ns = q.sk1 > q.sk ? (uint32_t)(q.sk1 - q.sk) : 0u;
+    q.lf = q.sk < 10 ? 1u : 0u;
+    return q;
+}
+
+__global__ __launch_bounds__(256) void bb_prep_kernel(const uint32_t *__restrict__ keys,
+                                                      const BatchCounts *__restrict__ counts,
+                                                      uint32_t *__restrict__ block_cells,
+                                                      uint32_t *__restrict__ block_leaves) {
+    __shared__ uint32_t wa[4], wb[4];
+    const uint32_t n = counts->n_bh_particles;
+    const uint32_t nblocks = (n + 255) / 256;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const BPos q = bpos_info(keys, n, blk * 256 + threadIdx.x);
+        uint32_t a = q.ns, b = q.lf;
+        wave_scan2(a, b);
+        if (lane == 63) {
+            wa[wave] = a;
+            wb[wave] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            block_cells[blk] = wa[0] + wa[1] + wa[2] + wa[3];
+            block_leaves[blk] = wb[0] + wb[1] + wb[2] + wb[3];
+        }
+        __syncthreads();
+    }
+}
+
+// Exclusive scan of the per-block sums (blockIdx.x 0: cells, 1: leaves).
+__global__ __launch_bounds__(1024) void bb_block_scan_kernel(uint32_t *__restrict__ a,
+                                                             uint32_t *__restrict__ b,
+                                                             const BatchCounts *__restrict__ counts) {
+    __shared__ uint32_t part[1024];
+    uint32_t *data = blockIdx.x == 0 ? a : b;
+    const uint32_t count = (counts->n_bh_particles + 255) / 256;
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (count + 1023) / 1024;
+    const uint32_t lo = min(count, t * per), hi = min(count, lo + per);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += data[i];
+    part[t] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(256) void bb_emit_kernel(
+    const uint32_t *__restrict__ keys, const uint32_t *__restrict__ sidx,
+    const float2 *__restrict__ pos, const float *__restrict__ mass,
+    BatchCounts *__restrict__ counts, const uint32_t *__restrict__ bh_prefix,
+    const uint32_t *__restrict__ block_cells, const uint32_t *__restrict__ block_leaves,
+    int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
+    uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
+    float4 *__restrict__ node4, uint32_t *__restrict__ ncount, uint32_t *__restrict__ sorted_src,
+    uint32_t *__restrict__ breaker, RankStats *__restrict__ stats) {
+    __shared__ uint32_t wa[4], wb[4];
+    const uint32_t n = counts->n_bh_particles;
+    const uint32_t nblocks = (n + 255) / 256;
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+        const uint32_t k = blk * 256 + threadIdx.x;
+        const BPos q = bpos_info(keys, n, k);
+        uint32_t a = q.ns, b = q.lf;
+        wave_scan2(a, b);
+        if (lane == 63) {
+            wa[wave] = a;
+            wb[wave] = b;
+        }
+        __syncthreads();
+        uint32_t C = a - q.ns + block_cells[blk];
+        uint32_t L = b - q.lf + block_leaves[blk];
+        for (uint32_t w = 0; w < wave; ++w) {
+            C += wa[w];
+            L += wb[w];
+        }
+        __syncthreads();
+        if (k >= n) continue;
+        if (k + 1 == n) {  // totals over all systems, and the end of the last leaf
+            counts->n_cells = C + q.ns;
+            counts->n_leaves = L + q.lf;
+            leaf_first[L + q.lf] = q.inside ? k + 1 : n;
+        }
+        if (!q.inside) {
+            sorted_src[k] = kBatchSentinel;
+            continue;
+        }
+        const uint32_t i = sidx[k];
+        const float2 p = pos[i];
+        const float m = mass[i];
+        const uint32_t slot = k + C + q.ns;
+        node4[slot] = make_float4(p.x, p.y, m, 0.0f);
+        ncount[slot] = 1u;
+        sorted_src[k] = i;
+        sdep[k] = (int8_t)q.sk;
+        cpos[k] = C;
+        prod[k] = make_float4(p.x * m, p.y * m, m, 0.0f);
+        if (q.lf) {
+            leaf_first[L] = k;
+            leaf_key[L] = q.key;
+        }
+        for (uint32_t j = 0; j < q.ns; ++j) breaker[C + j] = k;
+        if (q.sk < 0) stats[q.rank].root = k + C;
+        if (q.sk1 < 0) {
+            stats[q.rank].end_slot = k + 1 + C + q.ns;
+            stats[q.rank].n_in = k + 1 - bh_prefix[q.rank];
+        }
+    }
+}
+
+// ---- leaf sums and particle runs ------------------------------------------------
+
+__device__ __forceinline__ uint32_t tile_of(uint32_t key) {
+    return (key >> kBatchRankShift) * kTiles + ((key >> 10) & (kTiles - 1));
+}
+
+__global__ __launch_bounds__(256) void bb_leaf_kernel(
+    const uint32_t *__restrict__ leaf_first, const uint32_t *__restrict__ leaf_key,
+    const float4 *__restrict__ prod, const uint32_t *__restrict__ keys,
+    const uint32_t *__restrict__ cpos, const uint32_t *__restrict__ breaker,
+    const BatchCounts *__restrict__ counts, const uint32_t *__restrict__ bh_prefix,
+    const RankStats *__restrict__ stats, float4 *__restrict__ leaf_sum,
+    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ prun) {
+    __shared__ float4 stage[4][64];
+    const uint32_t n = counts->n_bh_particles, nl = counts->n_leaves, ncells = counts->n_cells;
+    const uint32_t waves = gridDim.x * 4;
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (uint32_t L = wave_id; L < nl; L += waves) {
+        const uint32_t key = leaf_key[L];
+        const uint32_t r = key >> kBatchRankShift;
+        const uint32_t sys_end = bh_prefix[r] + stats[r].n_in;
+        const uint32_t e = __builtin_amdgcn_readfirstlane(min(leaf_first[L + 1], sys_end));
+        const uint32_t f = __builtin_amdgcn_readfirstlane(min(leaf_first[L], e));
+        float sx = 0.0f, sy = 0.0f, sm = 0.0f;
+        float4 next = f + lane < e ? prod[f + lane] : zero4;
+        for (uint32_t base = f; base < e; base += 64) {
+            const uint32_t cnt = min(64u, e - base);
+            stage[w][lane] = next;
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t nb = base + 64 + lane;
+            next = nb < e ? prod[nb] : zero4;
+            uint32_t k = 0;
+            for (; k + 8 <= cnt; k += 8) {
+                float4 q[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) q[u] = stage[w][k + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    sx = sx + q[u].x;
+                    sy = sy + q[u].y;
+                    sm = sm + q[u].z;
+                }
+            }
+            for (; k < cnt; ++k) {
+                const float4 q = stage[w][k];
+                sx = sx + q.x;
+                sy = sy + q.y;
+                sm = sm + q.z;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane == 0) {
+            leaf_sum[L] = make_float4(sx, sy, sm, __uint_as_float(e - f));
+            const uint32_t T = tile_of(key);
+            if (L == 0 || tile_of(leaf_key[L - 1]) != T) tile_range[2 * T] = L;
+            if (L + 1 == nl || tile_of(leaf_key[L + 1]) != T) tile_range[2 * T + 1] = L + 1;
+        }
+    }
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {
+        const BPos q = bpos_info(keys, n, k);
+        if (!q.inside) continue;
+        const uint32_t c1 = cpos[k] + q.ns;  // C(k+1)
+        uint32_t run_end = c1 < ncells ? breaker[c1] : n;
+        run_end = min(run_end, bh_prefix[q.rank] + stats[q.rank].n_in);
+        prun[k + c1] = run_end > k ? run_end - k : 1u;
+    }
+}
+
+// ---- dense reduction per (system, depth-5 tile), then per system --------------
+
+__device__ __forceinline__ int tile_level_offset(int d) {
+    return d == 9 ? 0 : d == 8 ? 256 : d == 7 ? 320 : d == 6 ? 336 : 340;
+}
+
+__global__ __launch_bounds__(256) void bb_tile_kernel(
+    const uint32_t *__restrict__ tile_range, const uint32_t *__restrict__ leaf_first,
+    const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
+    const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
+    const BatchCounts *__restrict__ counts, float4 *__restrict__ node4,
+    uint32_t *__restrict__ ncount, TileCell *__restrict__ top) {
+    __shared__ TileCell l10[1024];
+    __shared__ TileCell up[341];
+    const uint32_t ntiles = counts->n_bh * kTiles;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t T = blockIdx.x; T < ntiles; T += gridDim.x) {
+        const uint32_t lb = tile_range[2 * T], le = tile_range[2 * T + 1];
+        if (lb == le) continue;  // uniform across the workgroup
+        for (uint32_t c = t; c < 1024; c += 256)
+            l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
+        __syncthreads();
+        for (uint32_t L = lb + t; L < le; L += 256) {
+            const uint32_t c = leaf_key[L] & 1023u;
+            const float4 s = leaf_sum[L];
+            const uint32_t n = __float_as_uint(s.w);
+            TileCell cell{n, n + (n >= 2u ? 1u : 0u), leaf_first[L], 0u, s.x, s.y, s.z, 0.0f};
+            l10[c] = cell;
+            emit_cell(cell, 10, sdep, cpos, node4, ncount);
+        }
+        __syncthreads();
+        up[t] = reduce4(l10[4 * t], l10[4 * t + 1], l10[4 * t + 2], l10[4 * t + 3]);
+        __syncthreads();
+        for (int d = 8; d >= 5; --d) {
+            const uint32_t cnt = 1u << (2 * (d - 5));
+            const int o = tile_level_offset(d), oc = tile_level_offset(d + 1);
+            if (t < cnt)
+                up[o + t] = reduce4(up[oc + 4 * t], up[oc + 4 * t + 1], up[oc + 4 * t + 2],
+                                    up[oc + 4 * t + 3]);
+            __syncthreads();
+        }
+        for (uint32_t c = t; c < 341; c += 256) {
+            const int d = c < 256 ? 9 : c < 320 ? 8 : c < 336 ? 7 : c < 340 ? 6 : 5;
+            emit_cell(up[c], d, sdep, cpos, node4, ncount);
+        }
+        if (t == 0) top[T] = up[340];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int top_level_offset(int d) {
+    return d == 5 ? 0 : d == 4 ? 1024 : d == 3 ? 1280 : d == 2 ? 1344 : d == 1 ? 1360 : 1364;
+}
+
+__global__ __launch_bounds__(1024) void bb_top_kernel(const TileCell *__restrict__ top,
+                                                      const uint32_t *__restrict__ tile_range,
+                                                      const int8_t *__restrict__ sdep,
+                                                      const uint32_t *__restrict__ cpos,
+                                                      const BatchCounts *__restrict__ counts,
+                                                      float4 *__restrict__ node4,
+                                                      uint32_t *__restrict__ ncount) {
+    __shared__ TileCell lv[1365];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t r = blockIdx.x; r < counts->n_bh; r += gridDim.x) {
+        const uint32_t T = r * kTiles + t;
+        lv[t] = tile_range[2 * T] != tile_range[2 * T + 1]
+                    ? top[T]
+                    : TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
+        __syncthreads();
+        for (int d = 4; d >= 0; --d) {
+            const uint32_t cnt = 1u << (2 * d);
+            const int o = top_level_offset(d), oc = top_level_offset(d + 1);
+            if (t < cnt)
+                lv[o + t] = reduce4(lv[oc + 4 * t], lv[oc + 4 * t + 1], lv[oc + 4 * t + 2],
+                                    lv[oc + 4 * t + 3]);
+            __syncthreads();
+        }
+        if (t < 341) {
+            const uint32_t c = 1024 + t;
+            const int d = c < 1280 ? 4 : c < 1344 ? 3 : c < 1360 ? 2 : c < 1364 ? 1 : 0;
+            emit_cell(lv[c], d, sdep, cpos, node4, ncount);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- one persistent force launch for every system ------------------------------
+
+// Barnes-Hut group g of one system: the walk of bh_kernels.hip over the
+// system's node range [root, end_slot) with its own constants.
+__device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st, float4 *stage,
+                              const uint32_t *__restrict__ sorted_src,
+                              const float2 *__restrict__ pos, const float2 *__restrict__ vel,
+                              const float4 *__restrict__ node4,
+                              const uint32_t *__restrict__ ncount,
+                              const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
+                              float2 *__restrict__ vel_out, uint32_t node_cap) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t src = sorted_src[d.sorted_base + g * 64 + lane];
+    const bool real = src != kBatchSentinel;
+    float px = 0.0f, py = 0.0f;
+    if (real) {
+        const float2 p = pos[src];
+        px = p.x;
+        py = p.y;
+    }
+    const float eps2 = d.eps2, theta2 = d.theta2;
+    const uint32_t end = __builtin_amdgcn_readfirstlane(min(st.end_slot, node_cap));
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t ind = __builtin_amdgcn_readfirstlane(st.root);
+    while (ind < end) {
+        const float4 nd = node4[ind];
+        if (nd.w == 0.0f) {
+            uint32_t n = prun[ind];
+            n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), end - ind));
+            const uint32_t rend = ind + n;
+            for (uint32_t base = ind; base < rend; base += 64) {
+                const uint32_t cnt = min(64u, rend - base);
+                stage[lane] = lane < cnt ? node4[base + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                __builtin_amdgcn_wave_barrier();
+                uint32_t k = 0;
+                for (; k + 8 <= cnt; k += 8) {
+                    float4 q[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) q[u] = stage[k + u];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
+                }
+                for (; k < cnt; ++k) {
+                    const float4 q = stage[k];
+                    accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+            ind = rend;
+            continue;
+        }
+        const uint32_t cnt = ncount[ind];
+        const float dx = nd.x - px;
+        const float dy = nd.y - py;
+        const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
+        const bool ok = (nd.w <= r2 * theta2) || !real;
+        if (__ballot(ok) == ~0ull) {
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float s = (nd.z * inv) * (inv * inv);
+            ax = __builtin_fmaf(dx, s, ax);
+            ay = __builtin_fmaf(dy, s, ay);
+            ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
+        } else {
+            ind = __builtin_amdgcn_readfirstlane(ind + 1u);
+        }
+    }
+    if (real) {
+        const float2 v = vel[src];
+        const float nvx = v.x + ax * d.kv, nvy = v.y + ay * d.kv;
+        pos_out[src] = make_float2(px + (v.x + nvx) * d.kp, py + (v.y + nvy) * d.kp);
+        vel_out[src] = make_float2(nvx, nvy);
+    }
+}
+
+// Direct-sum tile: 128 targets (two per lane) x one source slice; the same
+// arithmetic as ds_partial_kernel, so the per-slice partials are identical.
+__device__ void bb_ds_tile(uint32_t sub, const SysDesc &d, const float2 *__restrict__ pos,
+                           const float *__restrict__ mass, float2 *__restrict__ partial) {
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tb = sub >> 16, slice = sub & 0xffffu;
+    const uint32_t t0 = tb * 128 + lane, t1 = t0 + 64;
+    const float2 p0 = t0 < d.na ? pos[d.off + t0] : make_float2(0.0f, 0.0f);
+    const float2 p1 = t1 < d.na ? pos[d.off + t1] : make_float2(0.0f, 0.0f);
+    float ax0 = 0.0f, ay0 = 0.0f, ax1 = 0.0f, ay1 = 0.0f;
+    const uint32_t j0 = d.off + slice * d.ds_jlen;
+    const uint32_t j1 = min(d.off + d.na, j0 + d.ds_jlen);
+    const float eps2 = d.eps2;
+    for (uint32_t j = j0; j < j1; j += 8) {  // slices are whole multiples of 64
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const float2 s = pos[j + u];
+            const float m = mass[j + u];
+            {
+                const float dx = s.x - p0.x, dy = s.y - p0.y;
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float tt = (m * inv) * (inv * inv);
+                ax0 = __builtin_fmaf(dx, tt, ax0);
+                ay0 = __builtin_fmaf(dy, tt, ay0);
+            }
+            {
+                const float dx = s.x - p1.x, dy = s.y - p1.y;
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float tt = (m * inv) * (inv * inv);
+                ax1 = __builtin_fmaf(dx, tt, ax1);
+                ay1 = __builtin_fmaf(dy, tt, ay1);
+            }
+        }
+    }
+    float2 *dst = partial + d.ds_part_off + (size_t)slice * d.na;
+    if (t0 < d.na) dst[t0] = make_float2(ax0, ay0);
+    if (t1 < d.na) dst[t1] = make_float2(ax1, ay1);
+}
+
+__global__ __launch_bounds__(256) void bb_force_kernel(
+    const uint2 *__restrict__ items, const SysDesc *__restrict__ desc,
+    const RankStats *__restrict__ stats, const BatchCounts *__restrict__ counts,
+    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, const float *__restrict__ mass,
+    const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
+    const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
+    float2 *__restrict__ vel_out, float2 *__restrict__ partial, uint32_t node_cap) {
+    __shared__ float4 stage[4][64];
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t waves = gridDim.x * 4;
+    const uint32_t nitems = counts->n_items;
+    for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w); it < nitems;
+         it += waves) {
+        const uint2 item = items[it];
+        const SysDesc d = desc[item.x];
+        if (d.algo == GSIM_BARNES_HUT) {
+            const RankStats st = stats[d.bh_rank];
+            if (st.n_in == 0) continue;  // every particle of the system left the domain
+            bb_walk_group(item.y, d, st, stage[w], sorted_src, pos, vel, node4, ncount, prun,
+                          pos_out, vel_out, node_cap);
+        } else {
+            bb_ds_tile(item.y, d, pos, mass, partial);
+        }
+    }
+}
+
+// Direct-sum combine + integrate (SimShader.comp:68-69) for every DS particle.
+__global__ __launch_bounds__(256) void bb_ds_combine_kernel(
+    const SysDesc *__restrict__ desc, const uint32_t *__restrict__ ds_sys,
+    const uint32_t *__restrict__ ds_prefix, const BatchCounts *__restrict__ counts,
+    const float2 *__restrict__ partial, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out) {
+    const uint32_t nd = counts->n_ds, n = counts->n_ds_particles;
+    for (uint32_t f = blockIdx.x * 256 + threadIdx.x; f < n; f += gridDim.x * 256) {
+        const uint32_t r = rank_of(ds_prefix, nd, f);
+        const SysDesc d = desc[ds_sys[r]];
+        const uint32_t li = f - ds_prefix[r];
+        float ax = 0.0f, ay = 0.0f;
+        for (uint32_t s = 0; s < d.ds_slices; ++s) {
+            const float2 a = partial[d.ds_part_off + (size_t)s * d.na + li];
+            ax = ax + a.x;
+            ay = ay + a.y;
+        }
+        const uint32_t i = d.off + li;
+        const float2 p = pos[i], v = vel[i];
+        const float nvx = v.x + ax * d.kv, nvy = v.y + ay * d.kv;
+        pos_out[i] = make_float2(p.x + ((v.x + nvx) * 0.5f) * d.dt, p.y + ((v.y + nvy) * 0.5f) * d.dt);
+        vel_out[i] = make_float2(nvx, nvy);
+    }
+}
+
+}  // namespace
+
+// One batched step: Barnes-Hut build for all BH systems, one persistent force
+// launch for every system, the direct-sum combine.  Launched onto b->stream
+// (captured into a graph by the caller).
+hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
+    BatchDevice &d = b->d;
+    hipStream_t st = b->stream;
+    const uint32_t g_part = (uint32_t)std::min<size_t>((b->capacity + 255) / 256, 4096);
+    const uint32_t g_tiles = (uint32_t)std::min<size_t>(b->tiles_cap, 2048);
+    bb_keys_kernel<<<g_part, 256, 0, st>>>(d.desc, d.bh_sys, d.bh_prefix, d.counts, d.pos[in],
+                                           d.mass, d.pos[out], d.vel[in], d.vel[out], d.keys[0],
+                                           d.idx[0], d.tile_range, d.stats, d.digit_total);
+    // 4 passes: keys[0] -> keys[1] -> keys[0] -> keys[1] -> keys[0]
+    for (int pass = 0; pass < 4; ++pass) {
+        const int i = pass & 1, o = i ^ 1;
+        uint32_t *totals = d.digit_total + 256 * pass;
+        bb_upsweep_kernel<<<g_tiles, 256, 0, st>>>(d.keys[i], d.counts, 8u * pass, d.hist, totals);
+        bb_digit_scan_kernel<<<256, 256, 0, st>>>(d.hist, d.counts, totals);
+        bb_downsweep_kernel<<<g_tiles, 256, 0, st>>>(d.keys[i], d.idx[i], d.keys[o], d.idx[o],
+                                                     d.counts, 8u * pass, d.hist);
+    }
+    const uint32_t *sk = d.keys[0], *si = d.idx[0];
+    const uint32_t g_blk = (uint32_t)std::min<size_t>(b->blocks_cap, 4096);
+    bb_prep_kernel<<<g_blk, 256, 0, st>>>(sk, d.counts, d.block_cells, d.block_leaves);
+    bb_block_scan_kernel<<<2, 1024, 0, st>>>(d.block_cells, d.block_leaves, d.counts);
+    bb_emit_kernel<<<g_blk, 256, 0, st>>>(sk, si, d.pos[in], d.mass, d.counts, d.bh_prefix,
+                                          d.block_cells, d.block_leaves, d.sdep, d.cpos, d.prod,
+                                          d.leaf_first, d.leaf_key, d.node4, d.ncount,
+                                          d.sorted_src, d.breaker, d.stats);
+    bb_leaf_kernel<<<1024, 256, 0, st>>>(d.leaf_first, d.leaf_key, d.prod, sk, d.cpos, d.breaker,
+                                         d.counts, d.bh_prefix, d.stats, d.leaf_sum,
+                                         d.tile_range, d.prun);
+    const uint32_t g_tile = (uint32_t)std::min<size_t>((size_t)b->max_systems * kTiles, 4096);
+    bb_tile_kernel<<<g_tile, 256, 0, st>>>(d.tile_range, d.leaf_first, d.leaf_key, d.leaf_sum,
+                                           d.sdep, d.cpos, d.counts, d.node4, d.ncount, d.top);
+    bb_top_kernel<<<std::min<uint32_t>(b->max_systems, 256), 1024, 0, st>>>(
+        d.top, d.tile_range, d.sdep, d.cpos, d.counts, d.node4, d.ncount);
+    const uint32_t waves = 1024u * b->waves_per_simd;
+    bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.stats, d.counts, d.sorted_src,
+                                               d.pos[in], d.vel[in], d.mass, d.node4, d.ncount,
+                                               d.prun, d.pos[out], d.vel[out], d.partial,
+                                               (uint32_t)b->node_capacity);
+    bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.ds_sys, d.ds_prefix, d.counts,
+                                                 d.partial, d.pos[in], d.vel[in], d.pos[out],
+                                                 d.vel[out]);
+    return hipGetLastError();
+}
+
+}  // namespace gsim

@@ -82,6 +82,15 @@ class Timing(C.Structure):
     ]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("systems", C.c_uint32), ("max_systems", C.c_uint32),
+        ("capacity_particles", C.c_uint64), ("used_particles", C.c_uint64),
+        ("largest_free_range", C.c_uint64), ("node_capacity", C.c_uint64),
+        ("nodes_reserved", C.c_uint64), ("steps", C.c_uint64), ("waves_per_simd", C.c_uint32),
+    ]
+
+
 _lib = None
 
 
@@ -115,6 +124,18 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_get_timing": (st, [vp, C.POINTER(Timing), C.c_int32]),
         "gsim_set_ds_shape": (st, [vp, C.c_uint32, C.c_uint32]),
         "gsim_set_tuning": (st, [vp, C.c_int32, C.c_uint32]),
+        "gsim_batch_create": (st, [C.c_int32, C.c_size_t, C.c_uint32, C.POINTER(vp)]),
+        "gsim_batch_add": (st, [vp, C.POINTER(Params), fp, fp, fp, C.c_size_t,
+                                C.POINTER(C.c_uint32)]),
+        "gsim_batch_remove": (st, [vp, C.c_uint32]),
+        "gsim_batch_step": (st, [vp, C.c_uint32]),
+        "gsim_batch_synchronize": (st, [vp]),
+        "gsim_batch_read": (st, [vp, C.c_uint32, fp, fp, fp]),
+        "gsim_batch_query": (st, [vp, C.POINTER(BatchInfo)]),
+        "gsim_batch_bytes_per_particle": (C.c_size_t, []),
+        "gsim_batch_set_waves": (st, [vp, C.c_uint32]),
+        "gsim_batch_autotune": (st, [vp, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "gsim_batch_destroy": (st, [vp]),
         "gsim_generate": (st, [C.c_int32, C.c_size_t, C.c_float, C.c_float, C.c_float,
                                C.c_float, C.c_uint64, fp, fp, fp]),
         "gsim_count_particles": (st, [C.c_char_p, C.POINTER(C.c_size_t)]),
@@ -302,6 +323,97 @@ class Simulation:
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.gsim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def make_params(algorithm=DIRECT_SUM, *, gravitational_const: float = 1.0,
+                simulation_time: float = 1e-3, simulation_speed: float = 1.0,
+                softening_len: float = 0.2, accuracy_parameter: float = 1.0,
+                group_width: int = 64, device: int = 0) -> Params:
+    p = default_params()
+    p.gravitational_const = gravitational_const
+    p.simulation_time = simulation_time
+    p.simulation_speed = simulation_speed
+    p.softening_len = softening_len
+    p.accuracy_parameter = accuracy_parameter
+    p.algorithm = ALGORITHMS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+    p.group_width = group_width
+    p.device = device
+    return p
+
+
+class Batch:
+    """Many independent systems in one device arena, stepped together
+    (gsim_batch_*).  add() returns a system id; step() advances every system."""
+
+    def __init__(self, capacity_particles: int, max_systems: int = 256, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.gsim_batch_create(device, capacity_particles, max_systems, C.byref(h)))
+        self._h = h
+        self.n = {}
+
+    @staticmethod
+    def bytes_per_particle() -> int:
+        return int(load_library().gsim_batch_bytes_per_particle())
+
+    def add(self, particles: Particles, algorithm=DIRECT_SUM, **kw) -> int:
+        p = make_params(algorithm, **kw)
+        pos = np.ascontiguousarray(particles.pos, np.float32)
+        vel = np.ascontiguousarray(particles.vel, np.float32)
+        mass = np.ascontiguousarray(particles.mass, np.float32)
+        sid = C.c_uint32(0)
+        _check(_lib.gsim_batch_add(self._h, C.byref(p), _fptr(pos), _fptr(vel), _fptr(mass),
+                                   particles.n, C.byref(sid)))
+        self.n[sid.value] = particles.n
+        return sid.value
+
+    def remove(self, sid: int) -> None:
+        _check(_lib.gsim_batch_remove(self._h, sid))
+        self.n.pop(sid, None)
+
+    def step(self, count: int = 1) -> None:
+        _check(_lib.gsim_batch_step(self._h, count))
+
+    def synchronize(self) -> None:
+        _check(_lib.gsim_batch_synchronize(self._h))
+
+    def read(self, sid: int) -> Particles:
+        n = self.n[sid]
+        pos = np.empty((n, 2), np.float32)
+        vel = np.empty((n, 2), np.float32)
+        mass = np.empty(n, np.float32)
+        _check(_lib.gsim_batch_read(self._h, sid, _fptr(pos), _fptr(vel), _fptr(mass)))
+        return Particles(pos, vel, mass)
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        _check(_lib.gsim_batch_query(self._h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in BatchInfo._fields_}
+
+    def set_waves(self, waves_per_simd: int) -> None:
+        _check(_lib.gsim_batch_set_waves(self._h, waves_per_simd))
+
+    def autotune(self, steps_per_trial: int = 3) -> int:
+        w = C.c_uint32(0)
+        _check(_lib.gsim_batch_autotune(self._h, steps_per_trial, C.byref(w)))
+        return w.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.gsim_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -165,6 +165,51 @@ typedef enum gsim_tuning {
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 
+/* ---- batched form: many independent systems stepped together ---------------
+ * A batch is one device arena holding many systems of uneven size (direct sum
+ * or Barnes-Hut, each with its own parameters).  Every stage of a step runs
+ * as one launch over all systems (Barnes-Hut systems share one radix sort and
+ * one node array; one persistent launch computes every system's forces), and
+ * the whole step is replayed from a captured hipGraph.  Membership can change
+ * between steps without re-capturing.  A system's results are bit-identical
+ * to stepping it alone with gsim_step (group width 64, default launch shape).
+ * No counterpart in the reference, which runs one system per process. */
+typedef struct gsim_batch gsim_batch;
+
+typedef struct gsim_batch_info {
+    uint32_t systems, max_systems;
+    uint64_t capacity_particles;   /* padded particles the arena holds       */
+    uint64_t used_particles;
+    uint64_t largest_free_range;   /* largest system (padded) that fits now   */
+    uint64_t node_capacity, nodes_reserved;
+    uint64_t steps;
+    uint32_t waves_per_simd;
+} gsim_batch_info;
+
+gsim_status gsim_batch_create(int32_t device, size_t capacity_particles,
+                              uint32_t max_systems, gsim_batch **out);
+/* Uploads one system (host arrays as for gsim_create) into a free arena
+ * range; GSIM_ERROR_CAPACITY when it does not fit now. */
+gsim_status gsim_batch_add(gsim_batch *batch, const gsim_params *params,
+                           const float *pos_xy, const float *vel_xy,
+                           const float *mass, size_t n, uint32_t *system_id);
+gsim_status gsim_batch_remove(gsim_batch *batch, uint32_t system_id);
+/* Enqueues count steps of every system in the batch (asynchronous). */
+gsim_status gsim_batch_step(gsim_batch *batch, uint32_t count);
+gsim_status gsim_batch_synchronize(gsim_batch *batch);
+gsim_status gsim_batch_read(gsim_batch *batch, uint32_t system_id,
+                            float *pos_xy, float *vel_xy, float *mass);
+gsim_status gsim_batch_query(const gsim_batch *batch, gsim_batch_info *out);
+/* Device bytes the arena needs per padded particle (for HBM budgeting). */
+size_t gsim_batch_bytes_per_particle(void);
+/* Persistent force-launch width (waves per SIMD, 1..8); results unchanged. */
+gsim_status gsim_batch_set_waves(gsim_batch *batch, uint32_t waves_per_simd);
+/* Times each candidate width on the live batch (state is snapshotted and
+ * restored after every trial) and keeps the fastest. */
+gsim_status gsim_batch_autotune(gsim_batch *batch, uint32_t steps_per_trial,
+                                uint32_t *chosen_waves_per_simd);
+gsim_status gsim_batch_destroy(gsim_batch *batch);
+
 /* ---- particle sets: presets and the text format --------------------------- */
 
 /* Fills n particles (pos_xy[2n], vel_xy[2n], mass[n]) from a preset with an

@@ -1,0 +1,106 @@
+"""Batched form: many systems stepped together through gsim_batch_* must give
+each system exactly (bit for bit) what stepping it alone gives."""
+import sys
+
+import numpy as np
+import pytest
+
+import gsim
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _systems():
+    """A mixed set: small direct-sum systems, Barnes-Hut systems of uneven
+    size and parameters, one with particles leaving the domain."""
+    out = []
+    rng = np.random.default_rng(11)
+    specs = [("random", 1000, "direct-sum", {}), ("galaxy", 3000, "direct-sum", {"softening_len": 0.5}),
+             ("exponential-disk", 7000, "direct-sum", {}),
+             ("galaxy-collision", 9000, "barnes-hut", {"accuracy_parameter": 0.7}),
+             ("exponential-disk", 20000, "barnes-hut", {"accuracy_parameter": 0.7}),
+             ("random", 12345, "barnes-hut", {"accuracy_parameter": 1.0, "simulation_speed": 2.0}),
+             ("galaxy", 40000, "barnes-hut", {"accuracy_parameter": 0.5, "gravitational_const": 0.5})]
+    for k, (kind, n, algo, kw) in enumerate(specs):
+        p = gsim.generate(kind, n, 100.0, 1.0, 10.0, kw.get("gravitational_const", 1.0), seed=100 + k)
+        if k == 5:
+            p.pos[:7] = rng.uniform(600, 900, size=(7, 2))  # killed in the first build
+        out.append((p, algo, kw))
+    return out
+
+
+def _alone(p, algo, kw, steps):
+    with gsim.Simulation(p, algo, **kw) as s:
+        s.step(steps)
+        return s.read()
+
+
+def _same(a, b):
+    assert np.array_equal(a.pos, b.pos)
+    assert np.array_equal(a.vel, b.vel)
+    assert np.array_equal(a.mass, b.mass)
+
+
+def test_batch_matches_single_system_runs():
+    systems = _systems()
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    with gsim.Batch(cap + 4096, max_systems=16) as b:
+        ids = [b.add(p, algo, **kw) for p, algo, kw in systems]
+        b.step(6)
+        for sid, (p, algo, kw) in zip(ids, systems):
+            _say(f"batch vs alone: {algo} n={p.n} {kw}")
+            _same(b.read(sid), _alone(p, algo, kw, 6))
+
+
+def test_batch_membership_changes_between_steps():
+    systems = _systems()
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    with gsim.Batch(cap, max_systems=8) as b:
+        a = b.add(*systems[0][:2], **systems[0][2])
+        c = b.add(*systems[4][:2], **systems[4][2])
+        d = b.add(*systems[3][:2], **systems[3][2])
+        b.step(3)
+        b.remove(d)
+        e = b.add(*systems[6][:2], **systems[6][2])   # joins mid-run
+        f = b.add(*systems[1][:2], **systems[1][2])
+        b.step(4)
+        _same(b.read(a), _alone(*systems[0][:2], systems[0][2], 7))
+        _same(b.read(c), _alone(*systems[4][:2], systems[4][2], 7))
+        _same(b.read(e), _alone(*systems[6][:2], systems[6][2], 4))
+        _same(b.read(f), _alone(*systems[1][:2], systems[1][2], 4))
+        info = b.info()
+        assert info["systems"] == 4 and info["steps"] == 7
+
+
+def test_batch_autotune_leaves_state_unchanged():
+    systems = _systems()[2:6]
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    with gsim.Batch(cap, max_systems=8) as b:
+        ids = [b.add(p, algo, **kw) for p, algo, kw in systems]
+        b.step(2)
+        w = b.autotune(steps_per_trial=2)
+        assert 1 <= w <= 8
+        b.step(3)
+        for sid, (p, algo, kw) in zip(ids, systems):
+            _same(b.read(sid), _alone(p, algo, kw, 5))
+
+
+def test_batch_capacity_is_enforced():
+    p = gsim.generate("random", 5000, 50.0, 1.0, 2.0, seed=1)
+    with gsim.Batch(8192, max_systems=2) as b:
+        b.add(p, "barnes-hut")
+        with pytest.raises(gsim.GsimError) as e:
+            b.add(p, "barnes-hut", group_width=32)  # batched systems vote with W = 64
+        assert e.value.status == 1
+        with pytest.raises(gsim.GsimError) as e:
+            b.add(p, "direct-sum")  # 5056 more padded particles do not fit in 8192
+        assert e.value.status == 6
+        small = gsim.generate("random", 100, 50.0, 1.0, 2.0, seed=2)
+        b.add(small, "direct-sum")
+        with pytest.raises(gsim.GsimError):
+            b.add(small, "direct-sum")  # max_systems
