@@ -6,6 +6,7 @@ carries the Barnes-Hut step at 500,000 particles (two exponential disks,
 theta=1, W=64) because the headline metric names both.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--bh-steps K2] [--no-cpu]
+                    [--batch-steps K3] [--no-batch] [--serve-seconds S]
 
 Inputs are generated on the host and uploaded before the timed region; the
 timed region covers K steps bracketed by device synchronisation.  Kernel times
@@ -195,6 +196,85 @@ def bench_batched(steps: int, serial_steps: int = 5) -> dict:
     }
 
 
+def served_requests(seconds: float, rate: float = 50.0, seed: int = 7):
+    """The served workload: Poisson arrivals at `rate`/s for `seconds`; N
+    log-uniform in [2k, 2M]; preset random, galaxy or galaxy-collision (size
+    100, masses U[1,10]); Barnes-Hut with theta in {0.5, 0.7, 1.0}; steps
+    uniform in {10..1000}.  Returns (arrival time, particles, theta, steps)."""
+    rng = np.random.default_rng(seed)
+    kinds = ("random", "galaxy", "galaxy-collision")
+    out, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= seconds:
+            return out
+        n = int(np.exp(rng.uniform(np.log(2000), np.log(2_000_000))))
+        kind = kinds[int(rng.integers(3))]
+        theta = float((0.5, 0.7, 1.0)[int(rng.integers(3))])
+        steps = int(rng.integers(10, 1001))
+        p = gsim.generate(kind, n, 100.0, 1.0, 10.0, 1.0, seed=50_000 + len(out))
+        out.append((t, p, theta, steps))
+
+
+def bench_serve(seconds: float, budget_gb: float = 200.0, max_active: int = 0,
+                timeout_s: float = 900.0) -> dict:
+    import threading
+
+    reqs = served_requests(seconds)
+    offered = sum(p.n * steps for _, p, _, steps in reqs)
+    log(f"  {len(reqs)} requests, {offered / 1e9:.1f} G particle-steps offered over {seconds:g} s")
+    srv = gsim.Server(int(budget_gb * 1e9), max_systems=2048, max_active_particles=max_active)
+    rids = []
+
+    def client():
+        t0 = time.perf_counter()
+        for at, p, theta, steps in reqs:
+            wait = at - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            rids.append(srv.submit(p, steps, "barnes-hut", accuracy_parameter=theta))
+
+    th = threading.Thread(target=client)
+    t0 = time.perf_counter()
+    th.start()
+    th.join()
+    drained = srv.drain(timeout_s)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    lat, queue = [], []
+    for rid in rids:
+        if srv.wait(rid, 0.0):
+            _, info = srv.result(rid)
+            lat.append(info["latency_ms"])
+            queue.append(info["queue_ms"])
+    srv.close()
+    span = max(st["span_s"], 1e-9)
+    inter = st["bh_interactions"] + st["ds_interactions"]
+    return {
+        "metric": "served systems/s (continuous batching)",
+        "value": st["completed"] / span,
+        "unit": "systems/s",
+        "higher_is_better": True,
+        "completed": st["completed"],
+        "drained": drained,
+        "p50_ms": float(np.percentile(lat, 50)) if lat else None,
+        "p99_ms": float(np.percentile(lat, 99)) if lat else None,
+        "queue_p50_ms": float(np.percentile(queue, 50)) if queue else None,
+        "interactions_per_s": inter / span,
+        "particle_steps_per_s": offered / span if drained else None,
+        "offered_particle_steps_per_s": offered / seconds,
+        "span_s": span,
+        "wall_s": wall,
+        "batch_steps": st["batch_steps"],
+        "config": {"workload": f"Poisson arrivals 50/s for {seconds:g} s ({len(reqs)} requests), "
+                               "N log-uniform [2k, 2M], preset random/galaxy/galaxy-collision "
+                               "(size 100, masses U[1,10]), Barnes-Hut theta in {0.5,0.7,1.0}, "
+                               f"steps U{{10..1000}}, HBM budget {budget_gb:g} GB",
+                   "capacity_particles": st["capacity_particles"],
+                   "max_active_particles": max_active},
+    }
+
+
 def cpu_baseline(budget_s: float = 12.0) -> dict:
     """The oracle's FP32 direct sum (scalar C, one thread per core slice) on
     the same 50k workload, timed on this host: a bounded sample of whole
@@ -244,6 +324,10 @@ def main() -> int:
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="arrival window of the served workload (0: skip)")
+    ap.add_argument("--serve-max-active", type=int, default=0,
+                    help="cap on particles stepping at once in the served run (0: none)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run known-size device copies (counter calibration)")
     args = ap.parse_args()
@@ -251,9 +335,8 @@ def main() -> int:
         copy_calibration()
         return 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the simulation step has no multi-GPU path "
-              "in this build (see DESIGN.md for how it would shard); run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU only: the simulation step has no multi-GPU path; "
+              "run with --gpus 1.", file=sys.stderr)
         return 2
     log(f"direct-sum 50k: warmup {args.warmup}, steps {args.steps}")
     ds = bench_direct_sum(args.steps, args.warmup)
@@ -270,6 +353,12 @@ def main() -> int:
         bt = bench_batched(args.batch_steps)
         log(f"  {bt['value']:.0f} system-steps/s ({bt['ms_per_step']:.3f} ms/step), "
             f"{bt['speedup_vs_one_by_one']:.2f}x one by one")
+    sv = None
+    if args.serve_seconds > 0:
+        log(f"served: {args.serve_seconds:g} s of arrivals")
+        sv = bench_serve(args.serve_seconds, max_active=args.serve_max_active)
+        log(f"  {sv['value']:.2f} systems/s, p50 {sv['p50_ms']} ms, p99 {sv['p99_ms']} ms, "
+            f"{sv['interactions_per_s'] / 1e9:.1f} Ginteractions/s")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
@@ -295,7 +384,7 @@ def main() -> int:
                    "particles": DS_N, "global_batch": 1, "parallelism": "single-gpu"},
         "roofline": ds["roofline"],
         "cpu_baseline": cpu,
-        "secondary": [x for x in (bh, bt) if x],
+        "secondary": [x for x in (bh, bt, sv) if x],
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -39,6 +39,7 @@ SOURCES = [
     (os.path.join(CSRC, "bh_kernels.hip"), "hip"),
     (os.path.join(CSRC, "batch_kernels.hip"), "hip"),
     (os.path.join(CSRC, "batch.cpp"), "host"),
+    (os.path.join(CSRC, "server.cpp"), "host"),
     (os.path.join(CSRC, "api.cpp"), "host"),
     (os.path.join(HOST, "particles.cpp"), "host"),
 ]

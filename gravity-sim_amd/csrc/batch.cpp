@@ -44,7 +44,7 @@ void free_batch(gsim_batch *b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     destroy_graphs(b);
     gsim::BatchDevice &d = b->d;
-    void *ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.mass, d.partial, d.desc,
+    void *ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.mass, d.partial, d.desc, d.remaining,
                     d.bh_sys, d.bh_prefix, d.ds_sys, d.ds_prefix, d.items, d.counts, d.stats,
                     d.keys[0], d.keys[1], d.idx[0], d.idx[1], d.hist, d.digit_total,
                     d.block_cells, d.block_leaves, d.sdep, d.cpos, d.sorted_src, d.prod,
@@ -89,9 +89,16 @@ gsim_status upload_membership(gsim_batch *b) {
     std::vector<gsim::SysDesc> desc;
     std::vector<uint32_t> bh_sys, bh_prefix{0}, ds_sys, ds_prefix{0};
     std::vector<uint2> items;
+    std::vector<uint32_t> remaining;
     uint32_t part_off = 0;
+    GSIM_HIP(hipStreamSynchronize(b->stream));  // step counts below are exact from here
     for (auto &kv : b->systems) {
         gsim::BatchSystem &s = kv.second;
+        const uint64_t taken = b->steps - s.added_at;
+        s.steps = s.target == ~0ull ? taken : std::min<uint64_t>(taken, s.target);
+        remaining.push_back(s.target == ~0ull ? gsim::kUnlimitedSteps
+                                              : (uint32_t)std::min<uint64_t>(s.target - s.steps,
+                                                                             gsim::kUnlimitedSteps - 1));
         gsim::SysDesc d{};
         d.off = s.off;
         d.na = s.na;
@@ -135,12 +142,14 @@ gsim_status upload_membership(gsim_batch *b) {
     if (items.size() > b->item_capacity || part_off > b->partial_capacity)
         return fail(GSIM_ERROR_CAPACITY, "batch work list exceeds its capacity");
     gsim::BatchCounts c{};
+    c.n_sys = (uint32_t)desc.size();
     c.n_bh = (uint32_t)bh_sys.size();
     c.n_bh_particles = bh_prefix.back();
     c.n_ds = (uint32_t)ds_sys.size();
     c.n_ds_particles = ds_prefix.back();
     c.n_items = (uint32_t)items.size();
-    GSIM_HIP(hipStreamSynchronize(b->stream));
+    // the acceptance total carries over
+    GSIM_HIP(hipMemcpy(&c.accepts, &b->d.counts->accepts, sizeof(c.accepts), hipMemcpyDeviceToHost));
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
     };
@@ -150,6 +159,7 @@ gsim_status upload_membership(gsim_batch *b) {
     GSIM_HIP(up(b->d.ds_sys, ds_sys.data(), ds_sys.size() * 4));
     GSIM_HIP(up(b->d.ds_prefix, ds_prefix.data(), ds_prefix.size() * 4));
     GSIM_HIP(up(b->d.items, items.data(), items.size() * sizeof(uint2)));
+    GSIM_HIP(up(b->d.remaining, remaining.data(), remaining.size() * 4));
     GSIM_HIP(up(b->d.counts, &c, sizeof(c)));
     b->h_desc = desc;
     b->h_counts = c;
@@ -178,7 +188,6 @@ gsim_status launch_steps(gsim_batch *b, uint32_t count) {
         b->cur ^= 1;
         ++b->steps;
     }
-    for (auto &kv : b->systems) kv.second.steps += count;
     return GSIM_OK;
 }
 
@@ -195,8 +204,9 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
         return fail(GSIM_ERROR_INVALID_ARGUMENT, "capacity must be > 0 and max_systems in [1, %u]",
                     gsim::kBatchMaxSystems);
     const size_t cap = align64(capacity_particles);
-    if (cap > 0x7fffffffull / 8)
-        return fail(GSIM_ERROR_INVALID_ARGUMENT, "capacity %zu too large", capacity_particles);
+    if (cap > gsim::kBatchMaxParticles)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "capacity %zu exceeds the arena maximum of %zu particles",
+                    capacity_particles, gsim::kBatchMaxParticles);
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(GSIM_ERROR_NO_DEVICE, "no HIP device visible");
@@ -238,6 +248,7 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     GSIM_BALLOC(d.ds_sys, max_systems);
     GSIM_BALLOC(d.ds_prefix, max_systems + 1);
     GSIM_BALLOC(d.items, b->item_capacity);
+    GSIM_BALLOC(d.remaining, max_systems);
     GSIM_BALLOC(d.counts, 1);
     GSIM_BALLOC(d.stats, max_systems);
     GSIM_BALLOC(d.hist, 256 * b->tiles_cap);
@@ -326,6 +337,7 @@ gsim_status gsim_batch_add(gsim_batch *b, const gsim_params *p, const float *pos
     s.na = (uint32_t)na;
     s.n = (uint32_t)n;
     s.node_reserve = nodes;
+    s.added_at = b->steps;
     const uint32_t id = b->next_id++;
     b->systems[id] = s;
     b->nodes_reserved += nodes;
@@ -366,6 +378,45 @@ gsim_status gsim_batch_step(gsim_batch *b, uint32_t count) {
         if (st != GSIM_OK) return st;
     }
     return launch_steps(b, count);
+}
+
+gsim_status gsim_batch_set_steps(gsim_batch *b, uint32_t id, uint64_t more_steps) {
+    gsim::clear_error();
+    if (!b) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_set_steps: NULL handle");
+    auto it = b->systems.find(id);
+    if (it == b->systems.end()) return fail(GSIM_ERROR_INVALID_ARGUMENT, "no system %u in the batch", id);
+    GSIM_HIP(hipSetDevice(b->device));
+    GSIM_HIP(hipStreamSynchronize(b->stream));
+    gsim::BatchSystem &s = it->second;
+    const uint64_t taken = b->steps - s.added_at;
+    const uint64_t done = s.target == ~0ull ? taken : std::min<uint64_t>(taken, s.target);
+    // re-anchor so that `done` steps stay counted and `more_steps` remain
+    s.added_at = b->steps - done;
+    s.target = more_steps == ~0ull ? ~0ull : done + more_steps;
+    b->dirty = true;
+    return GSIM_OK;
+}
+
+gsim_status gsim_batch_steps_done(gsim_batch *b, uint32_t id, uint64_t *steps) {
+    gsim::clear_error();
+    if (!b || !steps) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_steps_done: NULL argument");
+    auto it = b->systems.find(id);
+    if (it == b->systems.end()) return fail(GSIM_ERROR_INVALID_ARGUMENT, "no system %u in the batch", id);
+    const gsim::BatchSystem &s = it->second;
+    const uint64_t taken = b->steps - s.added_at;
+    *steps = s.target == ~0ull ? taken : std::min<uint64_t>(taken, s.target);
+    return GSIM_OK;
+}
+
+gsim_status gsim_batch_accepts(gsim_batch *b, uint64_t *accepts) {
+    gsim::clear_error();
+    if (!b || !accepts) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_accepts: NULL argument");
+    GSIM_HIP(hipSetDevice(b->device));
+    GSIM_HIP(hipStreamSynchronize(b->stream));
+    unsigned long long a = 0;
+    GSIM_HIP(hipMemcpy(&a, &b->d.counts->accepts, sizeof(a), hipMemcpyDeviceToHost));
+    *accepts = a;
+    return GSIM_OK;
 }
 
 gsim_status gsim_batch_synchronize(gsim_batch *b) {
@@ -446,13 +497,17 @@ gsim_status gsim_batch_autotune(gsim_batch *b, uint32_t steps_per_trial, uint32_
     if (e == hipSuccess) e = hipMalloc((void **)&snap_mass, sizeof(float) * cap);
     const int cur0 = b->cur;
     const uint64_t steps0 = b->steps;
-    std::vector<uint64_t> sys_steps;
-    for (auto &kv : b->systems) sys_steps.push_back(kv.second.steps);
+    // trial steps also spend device step budgets: restore them with the state
+    std::vector<uint32_t> rem_snapshot(b->h_counts.n_sys);
+    if (!rem_snapshot.empty())
+        GSIM_HIP(hipMemcpy(rem_snapshot.data(), b->d.remaining, 4 * rem_snapshot.size(), hipMemcpyDeviceToHost));
     auto restore = [&]() -> hipError_t {
         hipError_t r = hipStreamSynchronize(b->stream);
         if (r == hipSuccess) r = hipMemcpy(b->d.pos[cur0], snap_pos, sizeof(float2) * cap, hipMemcpyDeviceToDevice);
         if (r == hipSuccess) r = hipMemcpy(b->d.vel[cur0], snap_vel, sizeof(float2) * cap, hipMemcpyDeviceToDevice);
         if (r == hipSuccess) r = hipMemcpy(b->d.mass, snap_mass, sizeof(float) * cap, hipMemcpyDeviceToDevice);
+        if (r == hipSuccess && !rem_snapshot.empty())
+            r = hipMemcpy(b->d.remaining, rem_snapshot.data(), 4 * rem_snapshot.size(), hipMemcpyHostToDevice);
         b->cur = cur0;
         b->steps = steps0;
         return r;
@@ -481,10 +536,6 @@ gsim_status gsim_batch_autotune(gsim_batch *b, uint32_t steps_per_trial, uint32_
             best = w;
         }
         if (e == hipSuccess) e = restore();
-    }
-    {
-        size_t k = 0;
-        for (auto &kv : b->systems) kv.second.steps = sys_steps[k++];
     }
     if (e == hipSuccess && st == GSIM_OK) st = gsim_batch_set_waves(b, best);
     (void)hipFree(snap_pos);

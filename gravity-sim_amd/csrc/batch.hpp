@@ -38,12 +38,18 @@ struct BatchCounts {
     uint32_t n_items;         // work items of the force launch
     uint32_t n_cells;         // cell nodes of all trees (emit)
     uint32_t n_leaves;        // occupied leaves of all trees (emit)
-    uint32_t pad;
+    uint32_t n_sys;           // all systems (direct sum and Barnes-Hut)
+    unsigned long long accepts;  // Barnes-Hut real-lane acceptances, all steps
 };
+
+constexpr uint32_t kUnlimitedSteps = 0xffffffffu;
 
 constexpr uint32_t kBatchRankShift = 21;                   // key = rank << 21 | cell (bit 20 = killed)
 constexpr uint32_t kBatchMaxSystems = 1u << (32 - kBatchRankShift);  // 2048
 constexpr uint32_t kBatchSentinel = 0xffffffffu;
+// Largest arena: node slots (3 per particle) and partial sums stay within
+// 32-bit indices.
+constexpr size_t kBatchMaxParticles = (0x7fffffffull / 8) / 64 * 64;
 
 struct BatchDevice {
     // arena
@@ -55,6 +61,7 @@ struct BatchDevice {
     uint32_t *bh_sys = nullptr, *bh_prefix = nullptr;  // [max_systems (+1)]
     uint32_t *ds_sys = nullptr, *ds_prefix = nullptr;
     uint2 *items = nullptr;
+    uint32_t *remaining = nullptr;  // steps each system may still take (kUnlimitedSteps = no limit)
     BatchCounts *counts = nullptr;
     RankStats *stats = nullptr;
     // Barnes-Hut scratch over all sorted positions
@@ -76,7 +83,9 @@ struct BatchSystem {
     gsim_params params;
     uint32_t off = 0, na = 0, n = 0;
     size_t node_reserve = 0;
-    uint64_t steps = 0;
+    uint64_t steps = 0;          // steps taken (exact after a synchronisation)
+    uint64_t added_at = 0;       // batch step count when the system joined
+    uint64_t target = ~0ull;     // total steps to take (~0 = unlimited)
 };
 
 }  // namespace gsim

@@ -40,7 +40,8 @@ __device__ __forceinline__ uint32_t rank_of(const uint32_t *__restrict__ prefix,
 
 __global__ __launch_bounds__(256) void bb_keys_kernel(
     const SysDesc *__restrict__ desc, const uint32_t *__restrict__ bh_sys,
-    const uint32_t *__restrict__ bh_prefix, BatchCounts *__restrict__ counts,
+    const uint32_t *__restrict__ bh_prefix, const uint32_t *__restrict__ remaining,
+    BatchCounts *__restrict__ counts,
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t *__restrict__ keys,
     uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
@@ -56,9 +57,20 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     }
     for (uint32_t f = tid; f < n; f += stride) {
         const uint32_t r = rank_of(bh_prefix, nb, f);
-        const SysDesc &d = desc[bh_sys[r]];
+        const uint32_t di = bh_sys[r];
+        const SysDesc &d = desc[di];
         const uint32_t i = d.off + (f - bh_prefix[r]);
         const float2 p = pos[i];
+        if (remaining[di] == 0u) {
+            // Budget spent: the system keeps its state exactly (no kill of
+            // particles that left the domain on its last step) and builds an
+            // empty tree, so the force launch skips it.
+            pos_out[i] = p;
+            vel_out[i] = vel[i];
+            keys[f] = (r << kBatchRankShift) | kKilledBit;
+            idx[f] = i;
+            continue;
+        }
         uint32_t c = cell20(p.x, p.y);
         if (c == 0xffffffffu) {  // InitShader.comp:110-117
             const float k = -kSimSize * 2.0f;
@@ -482,10 +494,12 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
                               const float4 *__restrict__ node4,
                               const uint32_t *__restrict__ ncount,
                               const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
-                              float2 *__restrict__ vel_out, uint32_t node_cap) {
+                              float2 *__restrict__ vel_out, uint32_t node_cap,
+                              unsigned long long &acc) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[d.sorted_base + g * 64 + lane];
     const bool real = src != kBatchSentinel;
+    const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
     float px = 0.0f, py = 0.0f;
     if (real) {
         const float2 p = pos[src];
@@ -521,6 +535,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
                 }
                 __builtin_amdgcn_wave_barrier();
             }
+            acc += (unsigned long long)n * nreal;
             ind = rend;
             continue;
         }
@@ -535,6 +550,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
             ax = __builtin_fmaf(dx, s, ax);
             ay = __builtin_fmaf(dy, s, ay);
             ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
+            acc += nreal;
         } else {
             ind = __builtin_amdgcn_readfirstlane(ind + 1u);
         }
@@ -590,7 +606,8 @@ __device__ void bb_ds_tile(uint32_t sub, const SysDesc &d, const float2 *__restr
 
 __global__ __launch_bounds__(256) void bb_force_kernel(
     const uint2 *__restrict__ items, const SysDesc *__restrict__ desc,
-    const RankStats *__restrict__ stats, const BatchCounts *__restrict__ counts,
+    const uint32_t *__restrict__ remaining, const RankStats *__restrict__ stats,
+    BatchCounts *__restrict__ counts,
     const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
     const float2 *__restrict__ vel, const float *__restrict__ mass,
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
@@ -600,32 +617,53 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t nitems = counts->n_items;
+    unsigned long long acc = 0;
     for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w); it < nitems;
          it += waves) {
         const uint2 item = items[it];
         const SysDesc d = desc[item.x];
+        const bool live = remaining[item.x] != 0u;
         if (d.algo == GSIM_BARNES_HUT) {
             const RankStats st = stats[d.bh_rank];
-            if (st.n_in == 0) continue;  // every particle of the system left the domain
+            // no particle inside the domain, or a spent budget (the keys
+            // kernel has already written the system's output)
+            if (!live || st.n_in == 0) continue;
             bb_walk_group(item.y, d, st, stage[w], sorted_src, pos, vel, node4, ncount, prun,
-                          pos_out, vel_out, node_cap);
-        } else {
+                          pos_out, vel_out, node_cap, acc);
+        } else if (live) {
             bb_ds_tile(item.y, d, pos, mass, partial);
         }
+    }
+    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(&counts->accepts, acc);
+}
+
+// End of a step: spend one step of every limited budget.
+__global__ __launch_bounds__(256) void bb_finish_kernel(uint32_t *__restrict__ remaining,
+                                                        const BatchCounts *__restrict__ counts) {
+    for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < counts->n_sys; s += gridDim.x * 256) {
+        const uint32_t r = remaining[s];
+        if (r != 0u && r != kUnlimitedSteps) remaining[s] = r - 1u;
     }
 }
 
 // Direct-sum combine + integrate (SimShader.comp:68-69) for every DS particle.
 __global__ __launch_bounds__(256) void bb_ds_combine_kernel(
-    const SysDesc *__restrict__ desc, const uint32_t *__restrict__ ds_sys,
+    const SysDesc *__restrict__ desc, const uint32_t *__restrict__ remaining,
+    const uint32_t *__restrict__ ds_sys,
     const uint32_t *__restrict__ ds_prefix, const BatchCounts *__restrict__ counts,
     const float2 *__restrict__ partial, const float2 *__restrict__ pos,
     const float2 *__restrict__ vel, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out) {
     const uint32_t nd = counts->n_ds, n = counts->n_ds_particles;
     for (uint32_t f = blockIdx.x * 256 + threadIdx.x; f < n; f += gridDim.x * 256) {
         const uint32_t r = rank_of(ds_prefix, nd, f);
-        const SysDesc d = desc[ds_sys[r]];
+        const uint32_t di = ds_sys[r];
+        const SysDesc d = desc[di];
         const uint32_t li = f - ds_prefix[r];
+        if (remaining[di] == 0u) {  // budget spent: hold the state
+            pos_out[d.off + li] = pos[d.off + li];
+            vel_out[d.off + li] = vel[d.off + li];
+            continue;
+        }
         float ax = 0.0f, ay = 0.0f;
         for (uint32_t s = 0; s < d.ds_slices; ++s) {
             const float2 a = partial[d.ds_part_off + (size_t)s * d.na + li];
@@ -650,7 +688,7 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     hipStream_t st = b->stream;
     const uint32_t g_part = (uint32_t)std::min<size_t>((b->capacity + 255) / 256, 4096);
     const uint32_t g_tiles = (uint32_t)std::min<size_t>(b->tiles_cap, 2048);
-    bb_keys_kernel<<<g_part, 256, 0, st>>>(d.desc, d.bh_sys, d.bh_prefix, d.counts, d.pos[in],
+    bb_keys_kernel<<<g_part, 256, 0, st>>>(d.desc, d.bh_sys, d.bh_prefix, d.remaining, d.counts, d.pos[in],
                                            d.mass, d.pos[out], d.vel[in], d.vel[out], d.keys[0],
                                            d.idx[0], d.tile_range, d.stats, d.digit_total);
     // 4 passes: keys[0] -> keys[1] -> keys[0] -> keys[1] -> keys[0]
@@ -679,13 +717,14 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     bb_top_kernel<<<std::min<uint32_t>(b->max_systems, 256), 1024, 0, st>>>(
         d.top, d.tile_range, d.sdep, d.cpos, d.counts, d.node4, d.ncount);
     const uint32_t waves = 1024u * b->waves_per_simd;
-    bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.stats, d.counts, d.sorted_src,
+    bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.remaining, d.stats, d.counts, d.sorted_src,
                                                d.pos[in], d.vel[in], d.mass, d.node4, d.ncount,
                                                d.prun, d.pos[out], d.vel[out], d.partial,
                                                (uint32_t)b->node_capacity);
-    bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.ds_sys, d.ds_prefix, d.counts,
-                                                 d.partial, d.pos[in], d.vel[in], d.pos[out],
-                                                 d.vel[out]);
+    bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.remaining, d.ds_sys, d.ds_prefix,
+                                                 d.counts, d.partial, d.pos[in], d.vel[in],
+                                                 d.pos[out], d.vel[out]);
+    bb_finish_kernel<<<8, 256, 0, st>>>(d.remaining, d.counts);
     return hipGetLastError();
 }
 

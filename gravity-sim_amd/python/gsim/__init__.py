@@ -91,6 +91,21 @@ class BatchInfo(C.Structure):
     ]
 
 
+class RequestInfo(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("steps", C.c_uint64),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [
+        ("completed", C.c_uint64), ("failed", C.c_uint64), ("queued", C.c_uint64),
+        ("running", C.c_uint64), ("batch_steps", C.c_uint64),
+        ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+        ("span_s", C.c_double), ("ds_interactions", C.c_uint64),
+        ("bh_interactions", C.c_uint64), ("capacity_particles", C.c_uint64),
+    ]
+
+
 _lib = None
 
 
@@ -136,6 +151,19 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_batch_set_waves": (st, [vp, C.c_uint32]),
         "gsim_batch_autotune": (st, [vp, C.c_uint32, C.POINTER(C.c_uint32)]),
         "gsim_batch_destroy": (st, [vp]),
+        "gsim_batch_set_steps": (st, [vp, C.c_uint32, C.c_uint64]),
+        "gsim_batch_steps_done": (st, [vp, C.c_uint32, C.POINTER(C.c_uint64)]),
+        "gsim_batch_accepts": (st, [vp, C.POINTER(C.c_uint64)]),
+        "gsim_server_create": (st, [C.c_int32, C.c_size_t, C.c_uint32, C.c_size_t,
+                                    C.POINTER(vp)]),
+        "gsim_server_submit": (st, [vp, C.POINTER(Params), fp, fp, fp, C.c_size_t,
+                                    C.c_uint32, C.POINTER(C.c_uint64)]),
+        "gsim_server_wait": (st, [vp, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+        "gsim_server_result": (st, [vp, C.c_uint64, fp, fp, fp, C.POINTER(RequestInfo)]),
+        "gsim_server_release": (st, [vp, C.c_uint64]),
+        "gsim_server_drain": (st, [vp, C.c_double, C.POINTER(C.c_int32)]),
+        "gsim_server_query": (st, [vp, C.POINTER(ServerInfo)]),
+        "gsim_server_destroy": (st, [vp]),
         "gsim_generate": (st, [C.c_int32, C.c_size_t, C.c_float, C.c_float, C.c_float,
                                C.c_float, C.c_uint64, fp, fp, fp]),
         "gsim_count_particles": (st, [C.c_char_p, C.POINTER(C.c_size_t)]),
@@ -406,6 +434,21 @@ class Batch:
     def set_waves(self, waves_per_simd: int) -> None:
         _check(_lib.gsim_batch_set_waves(self._h, waves_per_simd))
 
+    def set_steps(self, sid: int, more_steps: int | None) -> None:
+        """Limits a system to more_steps further steps (None: unlimited)."""
+        v = (1 << 64) - 1 if more_steps is None else int(more_steps)
+        _check(_lib.gsim_batch_set_steps(self._h, sid, v))
+
+    def steps_done(self, sid: int) -> int:
+        v = C.c_uint64(0)
+        _check(_lib.gsim_batch_steps_done(self._h, sid, C.byref(v)))
+        return v.value
+
+    def accepts(self) -> int:
+        v = C.c_uint64(0)
+        _check(_lib.gsim_batch_accepts(self._h, C.byref(v)))
+        return v.value
+
     def autotune(self, steps_per_trial: int = 3) -> int:
         w = C.c_uint32(0)
         _check(_lib.gsim_batch_autotune(self._h, steps_per_trial, C.byref(w)))
@@ -414,6 +457,81 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.gsim_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Server:
+    """Continuous-batching server (gsim_server_*): submit() queues a system for
+    a number of steps and returns a request id; a worker thread in the library
+    admits requests into one batch as HBM allows and retires each after exactly
+    its steps.  result() returns the final particles and the request's
+    latency."""
+
+    def __init__(self, hbm_budget_bytes: int, max_systems: int = 1024,
+                 max_active_particles: int = 0, device: int = 0):
+        lib = load_library()
+        h = C.c_void_p()
+        _check(lib.gsim_server_create(device, hbm_budget_bytes, max_systems,
+                                      max_active_particles, C.byref(h)))
+        self._h = h
+        self.n = {}
+
+    def submit(self, particles: Particles, steps: int, algorithm=DIRECT_SUM, **kw) -> int:
+        p = make_params(algorithm, **kw)
+        pos = np.ascontiguousarray(particles.pos, np.float32)
+        vel = np.ascontiguousarray(particles.vel, np.float32)
+        mass = np.ascontiguousarray(particles.mass, np.float32)
+        rid = C.c_uint64(0)
+        _check(_lib.gsim_server_submit(self._h, C.byref(p), _fptr(pos), _fptr(vel), _fptr(mass),
+                                       particles.n, steps, C.byref(rid)))
+        self.n[rid.value] = particles.n
+        return rid.value
+
+    def wait(self, rid: int, timeout_s: float = 60.0) -> bool:
+        done = C.c_int32(0)
+        _check(_lib.gsim_server_wait(self._h, rid, timeout_s, C.byref(done)))
+        return bool(done.value)
+
+    def result(self, rid: int, release: bool = True) -> tuple[Particles, dict]:
+        if rid not in self.n:
+            raise GsimError(1, f"unknown request {rid}")
+        n = self.n[rid]
+        pos = np.empty((n, 2), np.float32)
+        vel = np.empty((n, 2), np.float32)
+        mass = np.empty(n, np.float32)
+        info = RequestInfo()
+        _check(_lib.gsim_server_result(self._h, rid, _fptr(pos), _fptr(vel), _fptr(mass),
+                                       C.byref(info)))
+        if release:
+            _check(_lib.gsim_server_release(self._h, rid))
+            self.n.pop(rid, None)
+        return Particles(pos, vel, mass), {k: getattr(info, k) for k, _ in RequestInfo._fields_}
+
+    def drain(self, timeout_s: float = 600.0) -> bool:
+        idle = C.c_int32(0)
+        _check(_lib.gsim_server_drain(self._h, timeout_s, C.byref(idle)))
+        return bool(idle.value)
+
+    def stats(self) -> dict:
+        i = ServerInfo()
+        _check(_lib.gsim_server_query(self._h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in ServerInfo._fields_}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.gsim_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

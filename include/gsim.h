@@ -196,6 +196,12 @@ gsim_status gsim_batch_add(gsim_batch *batch, const gsim_params *params,
 gsim_status gsim_batch_remove(gsim_batch *batch, uint32_t system_id);
 /* Enqueues count steps of every system in the batch (asynchronous). */
 gsim_status gsim_batch_step(gsim_batch *batch, uint32_t count);
+/* Limits a system to `more_steps` further steps (UINT64_MAX: unlimited); once
+ * spent, the system holds its state while the others keep stepping. */
+gsim_status gsim_batch_set_steps(gsim_batch *batch, uint32_t system_id, uint64_t more_steps);
+gsim_status gsim_batch_steps_done(gsim_batch *batch, uint32_t system_id, uint64_t *steps);
+/* Barnes-Hut real-lane acceptances (interactions) over all steps so far. */
+gsim_status gsim_batch_accepts(gsim_batch *batch, uint64_t *accepts);
 gsim_status gsim_batch_synchronize(gsim_batch *batch);
 gsim_status gsim_batch_read(gsim_batch *batch, uint32_t system_id,
                             float *pos_xy, float *vel_xy, float *mass);
@@ -209,6 +215,56 @@ gsim_status gsim_batch_set_waves(gsim_batch *batch, uint32_t waves_per_simd);
 gsim_status gsim_batch_autotune(gsim_batch *batch, uint32_t steps_per_trial,
                                 uint32_t *chosen_waves_per_simd);
 gsim_status gsim_batch_destroy(gsim_batch *batch);
+
+/* ---- serving: continuous batching of independent requests ------------------
+ * A server owns a batch sized by an HBM budget and a worker thread.  Clients
+ * submit systems (any thread); the worker admits queued requests first-come
+ * first-served while the arena (and the optional active-particle cap) has
+ * room, steps every admitted system together, and retires each system after
+ * exactly its requested steps, freeing its space for the next request.  Each
+ * result is bit-identical to running the system alone.  No counterpart in the
+ * reference, which runs one system per process. */
+typedef struct gsim_server gsim_server;
+
+typedef struct gsim_request_info {
+    uint64_t n, steps;
+    double queue_ms;    /* submit -> admitted into the batch */
+    double latency_ms;  /* submit -> result on the host      */
+} gsim_request_info;
+
+typedef struct gsim_server_info {
+    uint64_t completed, failed, queued, running;
+    uint64_t batch_steps;          /* batched steps the worker has enqueued   */
+    double p50_ms, p99_ms, max_ms; /* submit -> result latency percentiles    */
+    double span_s;                 /* first submit -> last completion         */
+    uint64_t ds_interactions;      /* direct sum: padded N^2 per step, completed requests */
+    uint64_t bh_interactions;      /* Barnes-Hut accepted interactions, all steps so far  */
+    uint64_t capacity_particles;   /* arena capacity the budget bought        */
+} gsim_server_info;
+
+/* hbm_budget_bytes sizes the arena (gsim_batch_bytes_per_particle per padded
+ * particle, at most 268,435,392 particles: 32-bit node indices);
+ * max_active_particles caps the particles stepping at once (0: the arena
+ * capacity is the only limit). */
+gsim_status gsim_server_create(int32_t device, size_t hbm_budget_bytes, uint32_t max_systems,
+                               size_t max_active_particles, gsim_server **out);
+/* Queues one system for `steps` steps; inputs are copied.  GSIM_ERROR_CAPACITY
+ * when the system alone exceeds the budget. */
+gsim_status gsim_server_submit(gsim_server *server, const gsim_params *params,
+                               const float *pos_xy, const float *vel_xy, const float *mass,
+                               size_t n, uint32_t steps, uint64_t *request_id);
+/* Waits up to timeout_s seconds; *done = 1 when the request has completed. */
+gsim_status gsim_server_wait(gsim_server *server, uint64_t request_id, double timeout_s,
+                             int32_t *done);
+/* Copies a completed request's final state (any pointer may be NULL). */
+gsim_status gsim_server_result(gsim_server *server, uint64_t request_id, float *pos_xy,
+                               float *vel_xy, float *mass, gsim_request_info *info);
+/* Forgets a completed request and its host copy. */
+gsim_status gsim_server_release(gsim_server *server, uint64_t request_id);
+/* Waits until no request is queued or running; *idle = 1 if so. */
+gsim_status gsim_server_drain(gsim_server *server, double timeout_s, int32_t *idle);
+gsim_status gsim_server_query(gsim_server *server, gsim_server_info *out);
+gsim_status gsim_server_destroy(gsim_server *server);
 
 /* ---- particle sets: presets and the text format --------------------------- */
 

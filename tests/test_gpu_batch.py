@@ -104,3 +104,53 @@ def test_batch_capacity_is_enforced():
         b.add(small, "direct-sum")
         with pytest.raises(gsim.GsimError):
             b.add(small, "direct-sum")  # max_systems
+
+
+def _escaping(n=2000, seed=5):
+    """Barnes-Hut input whose particle 0 crosses the domain edge on step 1
+    (x = 499.5, vx = 1000, dt = 1e-3), so step 2 kills it."""
+    p = gsim.generate("random", n, 100.0, 1.0, 10.0, seed=seed)
+    p.pos[0] = (499.5, 0.0)
+    p.vel[0] = (1000.0, 0.0)
+    return p
+
+
+def test_batch_step_budgets_freeze_exactly():
+    systems = _systems()[:6] + [(_escaping(), "barnes-hut", {}), (_escaping(seed=6), "barnes-hut", {})]
+    budgets = [1, 3, None, 2, 5, 4, 1, 2]
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    with gsim.Batch(cap, max_systems=16) as b:
+        ids = []
+        for (p, algo, kw), k in zip(systems, budgets):
+            sid = b.add(p, algo, **kw)
+            b.set_steps(sid, k)
+            ids.append(sid)
+        b.step(3)
+        b.step(4)
+        for sid, (p, algo, kw), k in zip(ids, systems, budgets):
+            want = 7 if k is None else min(k, 7)
+            _say(f"budget {k}: {algo} n={p.n} {kw}")
+            assert b.steps_done(sid) == want
+            _same(b.read(sid), _alone(p, algo, kw, want))
+        # extend a spent budget: the system resumes from where it stopped
+        b.set_steps(ids[0], 2)
+        b.step(5)
+        assert b.steps_done(ids[0]) == 3
+        p, algo, kw = systems[0]
+        _same(b.read(ids[0]), _alone(p, algo, kw, 3))
+    esc = _alone(_escaping(), "barnes-hut", {}, 1)
+    assert esc.pos[0, 0] > 500.0 and esc.mass[0] > 0.0  # left the domain, not yet killed
+
+
+def test_batch_accepts_match_single_path_counters():
+    p = gsim.generate("galaxy", 20000, 100.0, 1.0, 10.0, seed=21)
+    with gsim.Simulation(p, "barnes-hut", accuracy_parameter=0.7) as s:
+        want = 0
+        for _ in range(3):
+            s.step(1)
+            want += int(s.dump(gsim.DUMP_COUNTERS)[1])
+    with gsim.Batch(32768, max_systems=4) as b:
+        sid = b.add(p, "barnes-hut", accuracy_parameter=0.7)
+        b.set_steps(sid, 3)
+        b.step(5)  # the two steps past the budget add nothing
+        assert b.accepts() == want
