@@ -31,8 +31,14 @@ import gsim  # noqa: E402
 PEAK_FP32_TFLOPS = 157.3          # vector FP32 (packed FMA), TFLOP/s
 PEAK_HBM_GBS = 8000.0             # HBM3E, GB/s
 VALU_LANE_OPS = 256 * 64 * 2.4e9  # one FP32 op per lane per clock, lane-ops/s
-DS_FLOP_PER_PAIR = 14             # 2 sub, 2 FMA, rsqrt, 3 mul, 2 FMA
+DS_FLOP_PER_PAIR = 14             # 2 sub, 2 FMA, rsqrt, 3 mul, 2 FMA (one-sided interaction)
 DS_LANEOPS_PER_PAIR = 10
+# symmetric kernel: per unordered pair 12 VALU + 1 rsq, plus 5 DPP rotates per
+# 8 pairs; B(B+1)/2 block pairs of 64 x 64 (diagonal blocks one-sided)
+DS_SYM_ISSUED_PER_PAIR = 13.625
+# HBM bytes per ds_sym_kernel launch at 50,048 particles (rocprofv3 FETCH_SIZE
+# x 2 + WRITE_SIZE x 1, calibrated on a 256 MiB copy); None until measured
+DS_TRAFFIC_BYTES = None
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
@@ -63,6 +69,8 @@ def bench_direct_sum(steps: int, warmup: int) -> dict:
     ms_step = elapsed * 1e3 / steps
     kernel_ms = t["force_ms"] / max(t["force_launches"], 1)
     achieved = DS_FLOP_PER_PAIR * pairs / (kernel_ms * 1e-3) / 1e12
+    blocks = (DS_N + 63) // 64
+    issued = DS_SYM_ISSUED_PER_PAIR * blocks * (blocks + 1) / 2 * 4096
     return {
         "value": pairs * steps / elapsed / 1e9,
         "ms_per_step": ms_step,
@@ -70,13 +78,15 @@ def bench_direct_sum(steps: int, warmup: int) -> dict:
         "combine_ms": t["integrate_ms"] / max(t["steps"], 1),
         "roofline": {
             "bound": "valu",
-            "kernel": "ds_partial_kernel",
+            "kernel": "ds_sym_kernel",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "valu_lane_op_frac": DS_LANEOPS_PER_PAIR * pairs / (kernel_ms * 1e-3) / VALU_LANE_OPS,
+            "traffic": DS_TRAFFIC_BYTES,
+            "note": "achieved counts the one-sided algorithm's 14 FLOP x N^2 per step; the "
+                    "symmetric kernel evaluates each unordered pair once",
+            "issued_valu_lane_op_frac": issued / (kernel_ms * 1e-3) / VALU_LANE_OPS,
         },
     }
 

@@ -100,12 +100,29 @@ gsim_status drain_timing(gsim_sim *s) {
     return GSIM_OK;
 }
 
-gsim_status ds_alloc_partial(gsim_sim *s) {
+// Grows the partial-sum buffer to hold `entries` float2 (the caller has
+// synchronised the stream).
+gsim_status ds_reserve_partial(gsim_sim *s, size_t entries) {
+    if (entries <= s->partial_cap) return GSIM_OK;
     (void)hipFree(s->partial);
     s->partial = nullptr;
-    GSIM_HIP(hipMalloc((void **)&s->partial, sizeof(float2) * s->na * s->ds_S));
-    s->ds_S_cap = s->ds_S;
+    s->partial_cap = 0;
+    hipError_t e = hipMalloc((void **)&s->partial, sizeof(float2) * entries);
+    if (e != hipSuccess)
+        return fail(GSIM_ERROR_OUT_OF_MEMORY, "direct-sum partial sums (%zu bytes): %s",
+                    sizeof(float2) * entries, hipGetErrorString(e));
+    s->partial_cap = entries;
     return GSIM_OK;
+}
+
+// Selects the symmetric or the one-sided kernel and sizes its partial sums.
+gsim_status ds_configure(gsim_sim *s, bool sym) {
+    s->ds_sym = sym ? 1u : 0u;
+    if (sym) {
+        s->ds_plan = gsim::ds_sym_plan(s->na, s->ds_sym_cols);
+        return ds_reserve_partial(s, s->ds_plan.ipart + s->ds_plan.jpart);
+    }
+    return ds_reserve_partial(s, s->na * s->ds_S);
 }
 
 gsim_status run_steps(gsim_sim *s, uint32_t count) {
@@ -263,7 +280,7 @@ gsim_status gsim_create(const gsim_params *params, const float *pos_xy, const fl
 
     if (params->algorithm == GSIM_DIRECT_SUM) {
         gsim::ds_pick_shape(na, &s->ds_R, &s->ds_S);
-        gsim_status st = ds_alloc_partial(s);
+        gsim_status st = ds_configure(s, gsim::ds_sym_default(na));
         if (st != GSIM_OK) return bail(st);
     } else {
         e = gsim::bh_alloc(&s->bh, na);
@@ -336,8 +353,7 @@ gsim_status gsim_set_ds_shape(gsim_sim *s, uint32_t R, uint32_t S) {
     GSIM_HIP(hipStreamSynchronize(s->stream));
     s->ds_R = R;
     s->ds_S = S;
-    if (S > s->ds_S_cap) return ds_alloc_partial(s);
-    return GSIM_OK;
+    return ds_configure(s, false);
 }
 
 gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
@@ -356,6 +372,20 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
             if (value > 3) return fail(GSIM_ERROR_INVALID_ARGUMENT, "traversal mode must be 0..3");
             s->trav_mode = value;
             return GSIM_OK;
+        case GSIM_TUNE_DS_SYMMETRIC:
+        case GSIM_TUNE_DS_SYM_COLS: {
+            if (s->params.algorithm != GSIM_DIRECT_SUM)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "not a direct-sum handle");
+            if (knob == GSIM_TUNE_DS_SYM_COLS && value != 0 && value != 1 && value != 2 &&
+                value != 4 && value != 8)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "column blocks per tile must be 0, 1, 2, 4 or 8");
+            GSIM_HIP(hipSetDevice(s->device));
+            GSIM_HIP(hipStreamSynchronize(s->stream));
+            bool sym = s->ds_sym != 0;
+            if (knob == GSIM_TUNE_DS_SYMMETRIC) sym = value != 0;
+            else s->ds_sym_cols = value;
+            return ds_configure(s, sym);
+        }
         default:
             return fail(GSIM_ERROR_INVALID_ARGUMENT, "unknown tuning knob %d", knob);
     }

@@ -298,6 +298,12 @@ gsim_status gsim_batch_add(gsim_batch *b, const gsim_params *p, const float *pos
     if (b->systems.size() >= b->max_systems)
         return fail(GSIM_ERROR_CAPACITY, "batch holds %u systems already", b->max_systems);
     const size_t na = align64(n);
+    // Large direct-sum systems fill the GPU alone and are stepped alone with
+    // the symmetric kernel; the batch keeps the one-sided kernel for the rest.
+    if (p->algorithm == GSIM_DIRECT_SUM && na >= gsim::kDsSymMinParticles)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "direct-sum systems of %zu or more padded particles are stepped alone (gsim_create)",
+                    gsim::kDsSymMinParticles);
     const size_t nodes = p->algorithm == GSIM_BARNES_HUT ? na + cell_bound(na) : 0;
     if (b->nodes_reserved + nodes > b->node_capacity)
         return fail(GSIM_ERROR_CAPACITY, "batch node arrays are full");

@@ -4,7 +4,7 @@
 // a_i = sum_j d * (m_j r^3) with d = p_j - p_i, r = rsqrt(d.d + eps2), then
 // v' = v + a*(dt*G), p' = p + ((v+v')*0.5)*dt.
 //
-// Work split: a target block of 256*R targets (R per lane) times a source
+// One-sided form -- work split: a target block of 256*R targets (R per lane) times a source
 // slice of `jlen` particles is one workgroup.  Source particles are read with
 // wave-uniform indices, so they arrive through scalar loads (s_load) and sit
 // in SGPRs: no LDS traffic and no VALU cycles spent on broadcast.  Each slice
@@ -83,7 +83,216 @@ __global__ __launch_bounds__(256) void ds_combine_kernel(
     if (acc_dbg) acc_dbg[i] = make_float2(ax, ay);
 }
 
+// ---- symmetric direct sum: every unordered pair evaluated once ----------------
+//
+// The 64-particle blocks of the padded set form a B x B block matrix of
+// interactions; only its upper triangle is evaluated, each pair (i, j) once,
+// adding  m_j d inv^3  to a_i and  -m_i d inv^3  to a_j (d = p_j - p_i; the
+// pair's r2 and inv are the same numbers from either side).  Row groups of
+// kDsSymRows blocks sit in one wave (each lane holds one particle of every row
+// block); a tile is one row group times `cols` column blocks.  For a column
+// block the wave holds one source particle per lane and rotates the sources,
+// with their accumulated reactions, one lane per step (DPP wave_ror), so after
+// 64 steps every lane has met every source once and each reaction is back in
+// its owner's lane.  Per pair: 12 VALU + 1 v_rsq_f32 for two interactions,
+// against 9 + 1 for one in the one-sided kernel.
+//
+// Partial sums, both in fixed order: per tile, each row particle's sum over
+// the tile's columns (ipart); per row group and column block, each column
+// particle's reactions from the group's rows (jpart).  The combine kernel adds
+// a particle's row partials in column order, then its column partials in
+// group order, and integrates.  Diagonal blocks are evaluated one-sided (so a
+// particle meets itself, as in the reference sum: with zero softening that
+// term is NaN there too).
+
+// Wave rotate right by one lane (DPP wave_ror:1, 0x13C): lane l receives lane
+// l-1.  Every source lane is valid, so no "old" value is needed.
+__device__ __forceinline__ float ror1(float v) {
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x13C, 0xF, 0xF, true));
+}
+
+// One column block against the wave's row blocks.  kInGroup: the column lies
+// inside the row group, so rows below it are skipped, the diagonal row is
+// one-sided and only rows above it react.
+template <bool kInGroup>
+__device__ __forceinline__ void ds_sym_column(const float (&ix)[kDsSymRows],
+                                              const float (&iy)[kDsSymRows],
+                                              const float (&im)[kDsSymRows],
+                                              float (&aix)[kDsSymRows], float (&aiy)[kDsSymRows],
+                                              float jx, float jy, float jm, float &ajx_out,
+                                              float &ajy_out, float eps2, uint32_t I0, uint32_t J) {
+    float ajx = 0.0f, ajy = 0.0f;
+#pragma unroll 2
+    for (int k = 0; k < 64; ++k) {
+#pragma unroll
+        for (int r = 0; r < kDsSymRows; ++r) {
+            if (kInGroup && I0 + r > J) continue;
+            const float dx = jx - ix[r], dy = jy - iy[r];
+            const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, eps2));
+            const float inv = __builtin_amdgcn_rsqf(r2);
+            const float inv3 = (inv * inv) * inv;
+            const float ti = jm * inv3;
+            aix[r] = __builtin_fmaf(dx, ti, aix[r]);
+            aiy[r] = __builtin_fmaf(dy, ti, aiy[r]);
+            if (!kInGroup || I0 + r < J) {
+                const float tj = im[r] * inv3;
+                ajx = __builtin_fmaf(-dx, tj, ajx);
+                ajy = __builtin_fmaf(-dy, tj, ajy);
+            }
+        }
+        jx = ror1(jx);
+        jy = ror1(jy);
+        jm = ror1(jm);
+        ajx = ror1(ajx);
+        ajy = ror1(ajy);
+    }
+    ajx_out = ajx;
+    ajy_out = ajy;
+}
+
+// Row group of tile t: the largest g with tile_base(g) <= t.
+__device__ __forceinline__ uint32_t ds_sym_group(uint32_t t, uint32_t G, uint32_t cb,
+                                                 uint32_t step) {
+    uint32_t lo = 0, hi = G;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ds_sym_tile_base(mid, cb, step) <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void ds_sym_kernel(const float2 *__restrict__ pos,
+                                                     const float *__restrict__ mass,
+                                                     float2 *__restrict__ ipart,
+                                                     float2 *__restrict__ jpart, uint32_t B,
+                                                     uint32_t G, uint32_t cols, uint32_t ntiles,
+                                                     float eps2) {
+    constexpr int R = kDsSymRows;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (t >= ntiles) return;  // whole wave
+    const uint32_t cb = (B + cols - 1) / cols, step = R / cols;
+    const uint32_t g = ds_sym_group(t, G, cb, step);
+    const uint32_t c = t - ds_sym_tile_base(g, cb, step);
+    const uint32_t I0 = g * R;
+    const uint32_t J0 = I0 + c * cols, J1 = min(B, J0 + cols);
+    float ix[R], iy[R], im[R], aix[R], aiy[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint32_t i = (I0 + r) * 64 + lane;
+        const bool ok = I0 + r < B;
+        const float2 p = ok ? pos[i] : make_float2(0.0f, 0.0f);
+        ix[r] = p.x;
+        iy[r] = p.y;
+        im[r] = ok ? mass[i] : 0.0f;
+        aix[r] = 0.0f;
+        aiy[r] = 0.0f;
+    }
+    const size_t jrow = ds_sym_jbase(g, B);
+    for (uint32_t J = J0; J < J1; ++J) {
+        const uint32_t j = J * 64 + lane;
+        const float2 q = pos[j];
+        const float qm = mass[j];
+        float ajx, ajy;
+        if (J >= I0 + R)
+            ds_sym_column<false>(ix, iy, im, aix, aiy, q.x, q.y, qm, ajx, ajy, eps2, I0, J);
+        else
+            ds_sym_column<true>(ix, iy, im, aix, aiy, q.x, q.y, qm, ajx, ajy, eps2, I0, J);
+        jpart[(jrow + (J - I0)) * 64 + lane] = make_float2(ajx, ajy);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (I0 + r < B) ipart[((size_t)t * R + r) * 64 + lane] = make_float2(aix[r], aiy[r]);
+}
+
+// One workgroup per 64-particle block: its four waves each add a quarter of
+// the block's partial list in order, then the quarters are added in order.
+__global__ __launch_bounds__(256) void ds_sym_combine_kernel(
+    const float2 *__restrict__ ipart, const float2 *__restrict__ jpart, uint32_t B, uint32_t cols,
+    const float2 *__restrict__ pos, const float2 *__restrict__ vel, float2 *__restrict__ pos_out,
+    float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, float kv, float dt) {
+    constexpr uint32_t R = kDsSymRows;
+    __shared__ float2 q[4][64];
+    const uint32_t P = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t g = P / R, r = P % R;
+    const uint32_t cb = (B + cols - 1) / cols, step = R / cols;
+    const uint32_t ni = cb - step * g;  // row partials: the group's tiles
+    const uint32_t nl = ni + g + 1;     // then one column partial per group 0..g
+    const uint32_t per = (nl + 3) / 4;
+    const uint32_t lo = min(nl, w * per), hi = min(nl, lo + per);
+    const size_t tb = ds_sym_tile_base(g, cb, step);
+    auto addr = [&](uint32_t e) -> size_t {
+        if (e < ni) return ((tb + e) * R + r) * 64 + lane;
+        const uint32_t h = e - ni;
+        return (ds_sym_jbase(h, B) + (P - h * R)) * 64 + lane;
+    };
+    float ax = 0.0f, ay = 0.0f;
+    uint32_t e = lo;
+    for (; e + 4 <= hi; e += 4) {
+        float2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const size_t a = addr(e + u);
+            v[u] = e + u < ni ? ipart[a] : jpart[a];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            ax = ax + v[u].x;
+            ay = ay + v[u].y;
+        }
+    }
+    for (; e < hi; ++e) {
+        const size_t a = addr(e);
+        const float2 v = e < ni ? ipart[a] : jpart[a];
+        ax = ax + v.x;
+        ay = ay + v.y;
+    }
+    q[w][lane] = make_float2(ax, ay);
+    __syncthreads();
+    if (w != 0) return;
+    ax = ((q[0][lane].x + q[1][lane].x) + q[2][lane].x) + q[3][lane].x;
+    ay = ((q[0][lane].y + q[1][lane].y) + q[2][lane].y) + q[3][lane].y;
+    const uint32_t i = P * 64 + lane;
+    const float2 p = pos[i], v = vel[i];
+    // SimShader.comp:68-69
+    const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
+    pos_out[i] = make_float2(p.x + ((v.x + nvx) * 0.5f) * dt, p.y + ((v.y + nvy) * 0.5f) * dt);
+    vel_out[i] = make_float2(nvx, nvy);
+    if (acc_dbg) acc_dbg[i] = make_float2(ax, ay);
+}
+
 }  // namespace
+
+DsSymPlan ds_sym_plan(size_t na, uint32_t cols) {
+    DsSymPlan p{};
+    p.B = (uint32_t)(na / 64);
+    p.G = (p.B + kDsSymRows - 1) / kDsSymRows;
+    auto tiles = [&](uint32_t b) -> size_t {
+        const uint32_t cb = (p.B + b - 1) / b, step = kDsSymRows / b;
+        return ds_sym_tile_base(p.G, cb, step);
+    };
+    if (cols == 0) {
+        // the widest tile that still gives ~16k waves; narrower tiles only
+        // add partial sums.  Measured (ms/step, 1/2/4/8 columns): 20k
+        // 0.111/0.125/0.131/0.155, 50k 0.447/0.446/0.463/0.461, 100k
+        // 1.725/1.641/1.626/1.620 -- this rule picks 1, 2 and 8.
+        cols = 1;
+        for (uint32_t b : {8u, 4u, 2u})
+            if (tiles(b) >= kDsSymMinTiles) {
+                cols = b;
+                break;
+            }
+    }
+    p.cols = cols;
+    p.ntiles = (uint32_t)tiles(cols);
+    p.ipart = (size_t)p.ntiles * kDsSymRows * 64;
+    p.jpart = ds_sym_jbase(p.G, p.B) * 64;
+    return p;
+}
+
+bool ds_sym_default(size_t na) { return na >= kDsSymMinParticles && na <= kDsSymMaxParticles; }
 
 // Launch shape by size: two targets per lane (each source load then feeds two
 // interactions), and enough source slices for ~6k workgroups so 256 CUs stay
@@ -103,6 +312,12 @@ void ds_pick_shape(size_t na, uint32_t *R, uint32_t *S) {
 
 hipError_t ds_launch_partial(gsim_sim *s, const float2 *pos, const float *mass) {
     const uint32_t na = (uint32_t)s->na;
+    if (s->ds_sym) {
+        const DsSymPlan &p = s->ds_plan;
+        ds_sym_kernel<<<(p.ntiles + 3) / 4, 256, 0, s->stream>>>(
+            pos, mass, s->partial, s->partial + p.ipart, p.B, p.G, p.cols, p.ntiles, s->eps2);
+        return hipGetLastError();
+    }
     const uint32_t R = s->ds_R, S = s->ds_S;
     // slice length: multiple of 64 covering na in S pieces
     uint32_t jlen = (uint32_t)(((na + S - 1) / S + 63) / 64 * 64);
@@ -126,6 +341,13 @@ hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
                              float2 *pos_out, float2 *vel_out) {
     const uint32_t na = (uint32_t)s->na;
     float2 *acc = (s->debug_flags & GSIM_DEBUG_ACCEL) ? s->accel : nullptr;
+    if (s->ds_sym) {
+        const DsSymPlan &p = s->ds_plan;
+        ds_sym_combine_kernel<<<p.B, 256, 0, s->stream>>>(s->partial, s->partial + p.ipart, p.B,
+                                                          p.cols, pos, vel, pos_out, vel_out, acc,
+                                                          s->kv, s->dt);
+        return hipGetLastError();
+    }
     ds_combine_kernel<<<(na + 255) / 256, 256, 0, s->stream>>>(
         s->partial, s->ds_S_used, pos, vel, pos_out, vel_out, acc, na, s->kv, s->dt);
     return hipGetLastError();

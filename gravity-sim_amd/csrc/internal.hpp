@@ -2,7 +2,9 @@
 // C ABI.  Device layout (HBM), per handle:
 //   particle state  pos float2[Na] x2 (ping-pong), vel float2[Na] x2, mass f32[Na]
 //                   (mass is only written by the Barnes-Hut kill, so one copy)
-//   direct sum      partial float2[S*Na] per-slice partial accelerations
+//   direct sum      one-sided: partial float2[S*Na] per-slice partial accelerations;
+//                   symmetric: row partials float2[tiles*8*64] then column
+//                   partials float2[(sum over groups of B - 8g)*64]
 //   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
 //                   cpos u32[Na], prod float4[Na], leaf records, tile results,
 //                   breaker u32[cap], prun u32[cap] (particle-run lengths),
@@ -77,6 +79,31 @@ struct PhaseEvents {
     bool bh;
 };
 
+// Symmetric direct sum (ds_kernels.hip): row groups of kDsSymRows 64-particle
+// blocks; a tile is a row group times `cols` column blocks (cols divides
+// kDsSymRows, so the tile count of each group has a closed form).
+constexpr uint32_t kDsSymRows = 8;
+constexpr size_t kDsSymMinParticles = 16384;   // default range of the symmetric kernel
+constexpr size_t kDsSymMaxParticles = 262144;  // (its partial sums grow as N^2 / 64)
+constexpr size_t kDsSymMinTiles = 16384;       // auto tile width: the widest with this many waves
+
+struct DsSymPlan {
+    uint32_t B = 0, G = 0;      // 64-particle blocks, row groups
+    uint32_t cols = 0;          // column blocks per tile
+    uint32_t ntiles = 0;        // waves of the force launch
+    size_t ipart = 0, jpart = 0;  // float2 entries of the two partial arrays
+};
+
+// First tile of row group g: sum over h < g of ceil((B - R h) / cols)
+// = g * cb - step * g (g - 1) / 2 with cb = ceil(B / cols), step = R / cols.
+__host__ __device__ inline size_t ds_sym_tile_base(uint32_t g, uint32_t cb, uint32_t step) {
+    return (size_t)g * cb - (size_t)step * g * (g - (g ? 1u : 0u)) / 2;
+}
+// First column-partial row of group g: sum over h < g of (B - R h).
+__host__ __device__ inline size_t ds_sym_jbase(uint32_t g, uint32_t B) {
+    return (size_t)g * B - (size_t)kDsSymRows * g * (g - (g ? 1u : 0u)) / 2;
+}
+
 }  // namespace gsim
 
 struct gsim_sim {
@@ -97,7 +124,10 @@ struct gsim_sim {
     int cur = 0;  // index of the input set
     // direct sum
     float2 *partial = nullptr;
-    uint32_t ds_R = 1, ds_S = 1, ds_S_cap = 0, ds_S_used = 1;
+    size_t partial_cap = 0;            // float2 entries allocated
+    uint32_t ds_R = 1, ds_S = 1, ds_S_used = 1;
+    uint32_t ds_sym = 0, ds_sym_cols = 0;  // symmetric kernel; column blocks per tile (0 auto)
+    gsim::DsSymPlan ds_plan{};
     // Barnes-Hut
     gsim::BhBuffers bh;
     // debug / timing
@@ -115,6 +145,8 @@ namespace gsim {
 
 // Direct sum (ds_kernels.hip)
 void ds_pick_shape(size_t na, uint32_t *R, uint32_t *S);
+bool ds_sym_default(size_t na);
+DsSymPlan ds_sym_plan(size_t na, uint32_t cols);
 hipError_t ds_launch_partial(gsim_sim *s, const float2 *pos, const float *mass);
 hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
                              float2 *pos_out, float2 *vel_out);

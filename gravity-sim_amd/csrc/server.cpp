@@ -210,6 +210,10 @@ gsim_status gsim_server_submit(gsim_server *s, const gsim_params *params, const 
     if (n == 0) return fail(GSIM_ERROR_INVALID_ARGUMENT, "The simulation must contain at least one particle!");
     if (padded(n) > s->capacity)
         return fail(GSIM_ERROR_CAPACITY, "a %zu-particle system exceeds the HBM budget", n);
+    if (params->algorithm == GSIM_DIRECT_SUM && padded(n) >= gsim::kDsSymMinParticles)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "direct-sum systems of %zu or more padded particles are stepped alone (gsim_create)",
+                    gsim::kDsSymMinParticles);
     std::unique_ptr<Request> r(new (std::nothrow) Request());
     if (!r) return fail(GSIM_ERROR_OUT_OF_MEMORY, "host allocation failed");
     r->params = *params;

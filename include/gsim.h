@@ -149,8 +149,8 @@ gsim_status gsim_set_debug(gsim_sim *sim, uint32_t flags);
 gsim_status gsim_set_timing(gsim_sim *sim, int32_t enable);
 gsim_status gsim_get_timing(gsim_sim *sim, gsim_timing *out, int32_t reset);
 
-/* Direct-sum launch shape: targets per lane (1, 2 or 4) and source slices
- * (0 = pick by size).  Results stay within the documented tolerance for every
+/* Selects the one-sided direct-sum kernel with this launch shape: targets per
+ * lane (1, 2 or 4) and source slices (0 = pick by size).  Results stay within the documented tolerance for every
  * shape; the partial sums are combined in slice order, so a shape is
  * deterministic. */
 gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
@@ -161,7 +161,11 @@ gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
 typedef enum gsim_tuning {
     GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
     GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
-    GSIM_TUNE_BH_TRAVERSAL = 3        /* bit 0 persistent groups, bit 1 LDS-staged runs */
+    GSIM_TUNE_BH_TRAVERSAL = 3,       /* bit 0 persistent groups, bit 1 LDS-staged runs */
+    GSIM_TUNE_DS_SYMMETRIC = 4,       /* 0/1: each pair once (Newton's third law); the
+                                         default for 16,384 <= Na <= 262,144           */
+    GSIM_TUNE_DS_SYM_COLS = 5         /* symmetric tiles: 1, 2, 4 or 8 column blocks,
+                                         0 = by size                                    */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 
@@ -173,6 +177,8 @@ gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
  * the whole step is replayed from a captured hipGraph.  Membership can change
  * between steps without re-capturing.  A system's results are bit-identical
  * to stepping it alone with gsim_step (group width 64, default launch shape).
+ * Direct-sum systems of 16,384 or more padded particles are refused: they
+ * fill the GPU alone and are stepped alone with the symmetric kernel.
  * No counterpart in the reference, which runs one system per process. */
 typedef struct gsim_batch gsim_batch;
 

@@ -100,6 +100,14 @@ def test_batch_capacity_is_enforced():
         with pytest.raises(gsim.GsimError) as e:
             b.add(p, "direct-sum")  # 5056 more padded particles do not fit in 8192
         assert e.value.status == 6
+    with gsim.Batch(40000, max_systems=4) as b:
+        big = gsim.generate("random", 16321, 50.0, 1.0, 2.0, seed=3)  # 16,384 padded
+        with pytest.raises(gsim.GsimError) as e:
+            b.add(big, "direct-sum")  # stepped alone with the symmetric kernel
+        assert e.value.status == 1
+        b.add(big, "barnes-hut")
+    with gsim.Batch(8192, max_systems=2) as b:
+        b.add(p, "barnes-hut")
         small = gsim.generate("random", 100, 50.0, 1.0, 2.0, seed=2)
         b.add(small, "direct-sum")
         with pytest.raises(gsim.GsimError):

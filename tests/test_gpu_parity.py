@@ -113,6 +113,84 @@ def test_ds_50k_one_step_all_shapes(R, S, rcp):
     assert np.array_equal(g.pos, np_)
 
 
+@pytest.mark.parametrize("cols", [0, 1, 2, 4, 8])
+def test_ds_50k_symmetric_every_tile_width(cols):
+    """The symmetric kernel (the default at 50,000): each pair evaluated once,
+    forces within the tolerance of the FP32 oracle for every tile width, and
+    the integration applied to exactly those forces."""
+    _say(f"ds 50k symmetric cols={cols}")
+    p = gsim.generate("exponential-disk", 50000, 100.0, 1.0, 10.0, seed=102)
+    s = _sim(p, "direct-sum")
+    s.set_tuning(gsim.TUNE_DS_SYM_COLS, cols)
+    s.set_tuning(gsim.TUNE_DS_SYMMETRIC, 1)
+    s.step(1)
+    a = s.dump(gsim.DUMP_ACCEL)[: p.n]
+    g = s.read()
+    s.close()
+    pp = p.padded()
+    ref = O.ds_accel_f32(pp.pos, pp.mass, EPS2)[: p.n]
+    ref64 = O.ds_accel_f64(pp.pos, pp.mass, EPS2)[: p.n]
+    sysr, eg, e32 = _force_vs_f64(a, ref, ref64)
+    _say(f"  system {sysr:.2e}; per-particle max vs FP64: gpu {eg:.2e}, oracle {e32:.2e}")
+    kv = np.float32(1e-3) * np.float32(1.0)
+    nv = (p.vel + a * kv).astype(np.float32)
+    assert np.array_equal(g.vel, nv)
+    np_ = (p.pos + ((p.vel + nv) * np.float32(0.5)) * np.float32(1e-3)).astype(np.float32)
+    assert np.array_equal(g.pos, np_)
+
+
+def test_ds_symmetric_is_default_in_its_range_and_deterministic():
+    p = gsim.generate("galaxy", 20000, 100.0, 1.0, 10.0, seed=105)
+    runs = []
+    for sym in (None, 1, 0):
+        s = _sim(p, "direct-sum")
+        if sym is not None:
+            s.set_tuning(gsim.TUNE_DS_SYMMETRIC, sym)
+        s.step(2)
+        runs.append(s.read())
+        s.close()
+    assert np.array_equal(runs[0].vel, runs[1].vel)       # default = symmetric
+    assert not np.array_equal(runs[0].vel, runs[2].vel)   # a different summation order
+    assert np.allclose(runs[0].vel, runs[2].vel, rtol=1e-4, atol=1e-3)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 500, 1000, 4103, 9000, 16384])
+def test_ds_symmetric_ragged_sizes(n):
+    """Block counts below, at and off a multiple of the row group (8 blocks),
+    forced onto the symmetric kernel: forces against the oracle, then three
+    steps against the oracle's trajectory."""
+    _say(f"ds symmetric n={n}")
+    p = gsim.generate("galaxy", n, 50.0, 1.0, 10.0, seed=n + 7)
+    s = _sim(p, "direct-sum")
+    s.set_tuning(gsim.TUNE_DS_SYMMETRIC, 1)
+    s.step(1)
+    a = s.dump(gsim.DUMP_ACCEL)[: p.n]
+    pp = p.padded()
+    ref = O.ds_accel_f32(pp.pos, pp.mass, EPS2)[: p.n]
+    ref64 = O.ds_accel_f64(pp.pos, pp.mass, EPS2)[: p.n]
+    if n > 1:
+        _force_vs_f64(a, ref, ref64)
+    else:
+        assert np.array_equal(a, np.zeros_like(a))  # alone: only itself and massless pads
+    s.step(2)
+    g = s.read()
+    s.close()
+    po, vo = pp.pos, pp.vel
+    for _ in range(3):
+        po, vo = O.ds_step(po, vo, pp.mass, 1.0, 1e-3, EPS2)
+    assert np.allclose(g.pos, po[:n], rtol=1e-6, atol=1e-6)
+    assert np.allclose(g.vel, vo[:n], rtol=1e-4, atol=1e-4)
+
+
+def test_ds_symmetric_zero_softening_is_nan():
+    p = gsim.generate("random", 700, 10.0, 1.0, 2.0, seed=4)
+    z = _sim(p, "direct-sum", softening_len=0.0)
+    z.set_tuning(gsim.TUNE_DS_SYMMETRIC, 1)
+    z.step(1)
+    assert np.isnan(z.read().vel).all()  # the self term, as in the reference sum
+    z.close()
+
+
 def test_ds_ragged_and_tiny_sizes():
     for n in (1, 2, 63, 64, 65, 1000):
         _say(f"ds n={n}")
