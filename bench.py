@@ -34,11 +34,12 @@ VALU_LANE_OPS = 256 * 64 * 2.4e9  # one FP32 op per lane per clock, lane-ops/s
 DS_FLOP_PER_PAIR = 14             # 2 sub, 2 FMA, rsqrt, 3 mul, 2 FMA (one-sided interaction)
 DS_LANEOPS_PER_PAIR = 10
 # symmetric kernel: per unordered pair 12 VALU + 1 rsq, plus 5 DPP rotates per
-# 8 pairs; B(B+1)/2 block pairs of 64 x 64 (diagonal blocks one-sided)
-DS_SYM_ISSUED_PER_PAIR = 13.625
-# HBM bytes per ds_sym_kernel launch at 50,048 particles (rocprofv3 FETCH_SIZE
-# x 2 + WRITE_SIZE x 1, calibrated on a 256 MiB copy); None until measured
-DS_TRAFFIC_BYTES = None
+# 16 pairs; B(B+1)/2 block pairs of 64 x 64 (diagonal blocks one-sided)
+DS_SYM_ISSUED_PER_PAIR = 13.3125
+# HBM bytes per ds_sym_kernel launch at 50,048 particles: rocprofv3 FETCH_SIZE
+# x 2 + WRITE_SIZE x 1 (factors from a 256 MiB copy), 2 x 2,417 + 48,950 KiB;
+# profiles/ds50k_bh500k_counters.md
+DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000

@@ -376,9 +376,8 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
         case GSIM_TUNE_DS_SYM_COLS: {
             if (s->params.algorithm != GSIM_DIRECT_SUM)
                 return fail(GSIM_ERROR_INVALID_ARGUMENT, "not a direct-sum handle");
-            if (knob == GSIM_TUNE_DS_SYM_COLS && value != 0 && value != 1 && value != 2 &&
-                value != 4 && value != 8)
-                return fail(GSIM_ERROR_INVALID_ARGUMENT, "column blocks per tile must be 0, 1, 2, 4 or 8");
+            if (knob == GSIM_TUNE_DS_SYM_COLS && value > 2)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "column blocks per wave must be 0, 1 or 2");
             GSIM_HIP(hipSetDevice(s->device));
             GSIM_HIP(hipStreamSynchronize(s->stream));
             bool sym = s->ds_sym != 0;

@@ -90,15 +90,19 @@ __global__ __launch_bounds__(256) void ds_combine_kernel(
 // adding  m_j d inv^3  to a_i and  -m_i d inv^3  to a_j (d = p_j - p_i; the
 // pair's r2 and inv are the same numbers from either side).  Row groups of
 // kDsSymRows blocks sit in one wave (each lane holds one particle of every row
-// block); a tile is one row group times `cols` column blocks.  For a column
+// block); a workgroup tile is one row group times 4 * `cols` column blocks,
+// `cols` per wave.  For a column
 // block the wave holds one source particle per lane and rotates the sources,
 // with their accumulated reactions, one lane per step (DPP wave_ror), so after
 // 64 steps every lane has met every source once and each reaction is back in
 // its owner's lane.  Per pair: 12 VALU + 1 v_rsq_f32 for two interactions,
-// against 9 + 1 for one in the one-sided kernel.
+// against 9 + 1 for one in the one-sided kernel; the 5 rotates per step are
+// shared by the group's 16 rows (16 rows beat 8 by 3-5%: 0.420 against 0.431
+// ms/step at 50k, 1.537 against 1.609 at 100k).
 //
 // Partial sums, both in fixed order: per tile, each row particle's sum over
-// the tile's columns (ipart); per row group and column block, each column
+// the tile's columns (ipart; the four waves' sums added in wave order through
+// LDS); per row group and column block, each column
 // particle's reactions from the group's rows (jpart).  The combine kernel adds
 // a particle's row partials in column order, then its column partials in
 // group order, and integrates.  Diagonal blocks are evaluated one-sided (so a
@@ -167,17 +171,18 @@ __global__ __launch_bounds__(256) void ds_sym_kernel(const float2 *__restrict__ 
                                                      const float *__restrict__ mass,
                                                      float2 *__restrict__ ipart,
                                                      float2 *__restrict__ jpart, uint32_t B,
-                                                     uint32_t G, uint32_t cols, uint32_t ntiles,
-                                                     float eps2) {
+                                                     uint32_t G, uint32_t cols, float eps2) {
     constexpr int R = kDsSymRows;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (t >= ntiles) return;  // whole wave
-    const uint32_t cb = (B + cols - 1) / cols, step = R / cols;
-    const uint32_t g = ds_sym_group(t, G, cb, step);
-    const uint32_t c = t - ds_sym_tile_base(g, cb, step);
+    __shared__ float2 red[3][R][64];  // row sums of waves 1..3
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t T = blockIdx.x;  // one tile per workgroup
+    const uint32_t width = 4 * cols;
+    const uint32_t cb = (B + width - 1) / width, step = R / width;
+    const uint32_t g = ds_sym_group(T, G, cb, step);
+    const uint32_t c = T - ds_sym_tile_base(g, cb, step);
     const uint32_t I0 = g * R;
-    const uint32_t J0 = I0 + c * cols, J1 = min(B, J0 + cols);
+    const uint32_t Jt = I0 + c * width;
+    const uint32_t J0 = min(B, Jt + w * cols), J1 = min(B, Jt + (w + 1) * cols);
     float ix[R], iy[R], im[R], aix[R], aiy[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -202,22 +207,37 @@ __global__ __launch_bounds__(256) void ds_sym_kernel(const float2 *__restrict__ 
             ds_sym_column<true>(ix, iy, im, aix, aiy, q.x, q.y, qm, ajx, ajy, eps2, I0, J);
         jpart[(jrow + (J - I0)) * 64 + lane] = make_float2(ajx, ajy);
     }
+    if (w > 0) {
 #pragma unroll
-    for (int r = 0; r < R; ++r)
-        if (I0 + r < B) ipart[((size_t)t * R + r) * 64 + lane] = make_float2(aix[r], aiy[r]);
+        for (int r = 0; r < R; ++r) red[w - 1][r][lane] = make_float2(aix[r], aiy[r]);
+    }
+    __syncthreads();
+    if (w != 0) return;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (I0 + r >= B) continue;
+        float sx = aix[r], sy = aiy[r];
+#pragma unroll
+        for (int u = 0; u < 3; ++u) {
+            const float2 v = red[u][r][lane];
+            sx = sx + v.x;
+            sy = sy + v.y;
+        }
+        ipart[((size_t)T * R + r) * 64 + lane] = make_float2(sx, sy);
+    }
 }
 
 // One workgroup per 64-particle block: its four waves each add a quarter of
 // the block's partial list in order, then the quarters are added in order.
 __global__ __launch_bounds__(256) void ds_sym_combine_kernel(
-    const float2 *__restrict__ ipart, const float2 *__restrict__ jpart, uint32_t B, uint32_t cols,
+    const float2 *__restrict__ ipart, const float2 *__restrict__ jpart, uint32_t B, uint32_t width,
     const float2 *__restrict__ pos, const float2 *__restrict__ vel, float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, float kv, float dt) {
     constexpr uint32_t R = kDsSymRows;
     __shared__ float2 q[4][64];
     const uint32_t P = blockIdx.x, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t g = P / R, r = P % R;
-    const uint32_t cb = (B + cols - 1) / cols, step = R / cols;
+    const uint32_t cb = (B + width - 1) / width, step = R / width;
     const uint32_t ni = cb - step * g;  // row partials: the group's tiles
     const uint32_t nl = ni + g + 1;     // then one column partial per group 0..g
     const uint32_t per = (nl + 3) / 4;
@@ -269,24 +289,17 @@ DsSymPlan ds_sym_plan(size_t na, uint32_t cols) {
     DsSymPlan p{};
     p.B = (uint32_t)(na / 64);
     p.G = (p.B + kDsSymRows - 1) / kDsSymRows;
-    auto tiles = [&](uint32_t b) -> size_t {
-        const uint32_t cb = (p.B + b - 1) / b, step = kDsSymRows / b;
+    auto tiles = [&](uint32_t width) -> size_t {
+        const uint32_t cb = (p.B + width - 1) / width, step = kDsSymRows / width;
         return ds_sym_tile_base(p.G, cb, step);
     };
-    if (cols == 0) {
-        // the widest tile that still gives ~16k waves; narrower tiles only
-        // add partial sums.  Measured (ms/step, 1/2/4/8 columns): 20k
-        // 0.111/0.125/0.131/0.155, 50k 0.447/0.446/0.463/0.461, 100k
-        // 1.725/1.641/1.626/1.620 -- this rule picks 1, 2 and 8.
-        cols = 1;
-        for (uint32_t b : {8u, 4u, 2u})
-            if (tiles(b) >= kDsSymMinTiles) {
-                cols = b;
-                break;
-            }
-    }
+    // Two columns per wave once that still gives ~32k waves.  Measured
+    // (ms/step, 1 / 2 columns): 20k 0.101 / 0.118, 50k 0.424 / 0.432, 100k
+    // 1.639 / 1.607.
+    if (cols == 0) cols = tiles(8) * 4 >= kDsSymMinWaves ? 2 : 1;
     p.cols = cols;
-    p.ntiles = (uint32_t)tiles(cols);
+    p.width = 4 * cols;
+    p.ntiles = (uint32_t)tiles(p.width);
     p.ipart = (size_t)p.ntiles * kDsSymRows * 64;
     p.jpart = ds_sym_jbase(p.G, p.B) * 64;
     return p;
@@ -314,8 +327,9 @@ hipError_t ds_launch_partial(gsim_sim *s, const float2 *pos, const float *mass) 
     const uint32_t na = (uint32_t)s->na;
     if (s->ds_sym) {
         const DsSymPlan &p = s->ds_plan;
-        ds_sym_kernel<<<(p.ntiles + 3) / 4, 256, 0, s->stream>>>(
-            pos, mass, s->partial, s->partial + p.ipart, p.B, p.G, p.cols, p.ntiles, s->eps2);
+        ds_sym_kernel<<<p.ntiles, 256, 0, s->stream>>>(pos, mass, s->partial,
+                                                       s->partial + p.ipart, p.B, p.G, p.cols,
+                                                       s->eps2);
         return hipGetLastError();
     }
     const uint32_t R = s->ds_R, S = s->ds_S;
@@ -344,7 +358,7 @@ hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
     if (s->ds_sym) {
         const DsSymPlan &p = s->ds_plan;
         ds_sym_combine_kernel<<<p.B, 256, 0, s->stream>>>(s->partial, s->partial + p.ipart, p.B,
-                                                          p.cols, pos, vel, pos_out, vel_out, acc,
+                                                          p.width, pos, vel, pos_out, vel_out, acc,
                                                           s->kv, s->dt);
         return hipGetLastError();
     }

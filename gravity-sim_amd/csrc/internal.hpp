@@ -3,8 +3,8 @@
 //   particle state  pos float2[Na] x2 (ping-pong), vel float2[Na] x2, mass f32[Na]
 //                   (mass is only written by the Barnes-Hut kill, so one copy)
 //   direct sum      one-sided: partial float2[S*Na] per-slice partial accelerations;
-//                   symmetric: row partials float2[tiles*8*64] then column
-//                   partials float2[(sum over groups of B - 8g)*64]
+//                   symmetric: row partials float2[tiles*16*64] then column
+//                   partials float2[(sum over groups of B - 16g)*64]
 //   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
 //                   cpos u32[Na], prod float4[Na], leaf records, tile results,
 //                   breaker u32[cap], prun u32[cap] (particle-run lengths),
@@ -80,22 +80,24 @@ struct PhaseEvents {
 };
 
 // Symmetric direct sum (ds_kernels.hip): row groups of kDsSymRows 64-particle
-// blocks; a tile is a row group times `cols` column blocks (cols divides
-// kDsSymRows, so the tile count of each group has a closed form).
-constexpr uint32_t kDsSymRows = 8;
+// blocks; a workgroup tile is a row group times `width` = 4 x cols column
+// blocks (width divides kDsSymRows, so the tile count of each group has a
+// closed form).
+constexpr uint32_t kDsSymRows = 16;
 constexpr size_t kDsSymMinParticles = 16384;   // default range of the symmetric kernel
 constexpr size_t kDsSymMaxParticles = 262144;  // (its partial sums grow as N^2 / 64)
-constexpr size_t kDsSymMinTiles = 16384;       // auto tile width: the widest with this many waves
+constexpr size_t kDsSymMinWaves = 32768;       // auto: 2 columns per wave from this many waves
 
 struct DsSymPlan {
     uint32_t B = 0, G = 0;      // 64-particle blocks, row groups
-    uint32_t cols = 0;          // column blocks per tile
-    uint32_t ntiles = 0;        // waves of the force launch
+    uint32_t cols = 0;          // column blocks per wave (1 or 2)
+    uint32_t width = 0;         // column blocks per workgroup tile (4 x cols)
+    uint32_t ntiles = 0;        // workgroups of the force launch
     size_t ipart = 0, jpart = 0;  // float2 entries of the two partial arrays
 };
 
-// First tile of row group g: sum over h < g of ceil((B - R h) / cols)
-// = g * cb - step * g (g - 1) / 2 with cb = ceil(B / cols), step = R / cols.
+// First tile of row group g: sum over h < g of ceil((B - R h) / width)
+// = g * cb - step * g (g - 1) / 2 with cb = ceil(B / width), step = R / width.
 __host__ __device__ inline size_t ds_sym_tile_base(uint32_t g, uint32_t cb, uint32_t step) {
     return (size_t)g * cb - (size_t)step * g * (g - (g ? 1u : 0u)) / 2;
 }

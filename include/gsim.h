@@ -164,8 +164,8 @@ typedef enum gsim_tuning {
     GSIM_TUNE_BH_TRAVERSAL = 3,       /* bit 0 persistent groups, bit 1 LDS-staged runs */
     GSIM_TUNE_DS_SYMMETRIC = 4,       /* 0/1: each pair once (Newton's third law); the
                                          default for 16,384 <= Na <= 262,144           */
-    GSIM_TUNE_DS_SYM_COLS = 5         /* symmetric tiles: 1, 2, 4 or 8 column blocks,
-                                         0 = by size                                    */
+    GSIM_TUNE_DS_SYM_COLS = 5         /* symmetric kernel: 1 or 2 column blocks per
+                                         wave, 0 = by size                              */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 

@@ -113,7 +113,7 @@ def test_ds_50k_one_step_all_shapes(R, S, rcp):
     assert np.array_equal(g.pos, np_)
 
 
-@pytest.mark.parametrize("cols", [0, 1, 2, 4, 8])
+@pytest.mark.parametrize("cols", [0, 1, 2])
 def test_ds_50k_symmetric_every_tile_width(cols):
     """The symmetric kernel (the default at 50,000): each pair evaluated once,
     forces within the tolerance of the FP32 oracle for every tile width, and
@@ -156,7 +156,7 @@ def test_ds_symmetric_is_default_in_its_range_and_deterministic():
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 500, 1000, 4103, 9000, 16384])
 def test_ds_symmetric_ragged_sizes(n):
-    """Block counts below, at and off a multiple of the row group (8 blocks),
+    """Block counts below, at and off a multiple of the row group (16 blocks),
     forced onto the symmetric kernel: forces against the oracle, then three
     steps against the oracle's trajectory."""
     _say(f"ds symmetric n={n}")
