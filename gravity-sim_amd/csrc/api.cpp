@@ -369,8 +369,14 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
             s->trav_waves_per_simd = value;
             return GSIM_OK;
         case GSIM_TUNE_BH_TRAVERSAL:
-            if (value > 3) return fail(GSIM_ERROR_INVALID_ARGUMENT, "traversal mode must be 0..3");
+            if (value > 3 && value != 6)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "traversal mode must be 0..3 or 6");
             s->trav_mode = value;
+            return GSIM_OK;
+        case GSIM_TUNE_BH_SEGMENTS:
+            if (value < 1 || value > 65536)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "segments must be in [1, 65536]");
+            s->trav_segments = value;
             return GSIM_OK;
         case GSIM_TUNE_DS_SYMMETRIC:
         case GSIM_TUNE_DS_SYM_COLS: {

@@ -470,9 +470,11 @@ __device__ __forceinline__ void walk_group(
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
     const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
-    float theta2, float kv, float kp, unsigned long long &vis, unsigned long long &acc) {
+    float theta2, float kv, float kp, unsigned long long &vis, unsigned long long &acc,
+    uint32_t *__restrict__ gcost = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[g * 64 + lane];
+    uint32_t cells = 0, runs = 0;  // walk cost: cell visits, streamed particle nodes
     const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
@@ -490,11 +492,16 @@ __device__ __forceinline__ void walk_group(
             n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), total - ind));
             const uint32_t end = ind + n;
             if (kStaged) {
+                // the next chunk's records are loaded while this one is summed
+                float4 next = lane < min(64u, n) ? node4[ind + lane]
+                                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 for (uint32_t base = ind; base < end; base += 64) {
                     const uint32_t cnt = min(64u, end - base);
-                    stage[lane] = lane < cnt ? node4[base + lane]
-                                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    stage[lane] = next;
                     __builtin_amdgcn_wave_barrier();
+                    const uint32_t nb = base + 64;
+                    const uint32_t ncnt = nb < end ? min(64u, end - nb) : 0u;
+                    next = lane < ncnt ? node4[nb + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     uint32_t k = 0;
                     for (; k + 8 <= cnt; k += 8) {
                         float4 q[8];
@@ -527,9 +534,11 @@ __device__ __forceinline__ void walk_group(
             }
             vis += (unsigned long long)n * nreal;
             acc += (unsigned long long)n * nreal;
+            runs += n;
             ind = end;
             continue;
         }
+        ++cells;
         const uint32_t cnt = ncount[ind];
         const float dx = nd.x - px;
         const float dy = nd.y - py;
@@ -553,6 +562,93 @@ __device__ __forceinline__ void walk_group(
         pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
         vel_out[src] = make_float2(nvx, nvy);
         if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+    }
+    // VALU weight of the walk (a cell visit ~12 instructions, a particle ~10),
+    // in units of 4, capped so 2^15 groups' prefix fits 32 bits
+    if (gcost && lane == 0) gcost[g] = min((cells * 12u + runs * 10u) >> 2, 65535u);
+}
+
+// Exclusive prefix of (cost + 1) over the groups, and the total at [groups]
+// (one workgroup).  The +1 makes the first step, with no costs yet, split the
+// groups evenly by count.
+__global__ __launch_bounds__(1024) void bh_cost_scan_kernel(const uint32_t *__restrict__ gcost,
+                                                            uint32_t groups,
+                                                            uint32_t *__restrict__ gpre) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t per = (groups + 1023) / 1024;
+    const uint32_t lo = min(groups, t * per), hi = min(groups, lo + per);
+    uint32_t sum = 0;
+    for (uint32_t g = lo; g < hi; ++g) sum += gcost[g] + 1u;
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (uint32_t g = lo; g < hi; ++g) {
+        gpre[g] = run;
+        run += gcost[g] + 1u;
+    }
+    if (t == 1023) gpre[groups] = part[1023];
+}
+
+// Cost-balanced traversal: workgroup k of P walks the contiguous groups whose
+// cost prefix starts in [k T / P, (k+1) T / P) (T = total cost of the last
+// walk), so every workgroup gets the same estimated work while neighbouring
+// groups -- which share their near field in the caches -- stay together; its
+// 16 waves take the segment's groups in order from an LDS counter.  Which wave
+// walks a group never changes the group's result.  Measured at 500k (two
+// disks, theta 1): per-CU work max/mean 1.45 -> 1.13, but the launch then ends
+// on single dense-core groups (one wave walks ~5,400 particle nodes), so the
+// gain is ~2.5%; the static launch stays the default.
+template <bool kDebugAccel>
+__global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
+    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun,
+    BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
+    float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
+    float kv, float kp, const uint32_t *__restrict__ gpre, uint32_t *__restrict__ gcost) {
+    __shared__ float4 stage[16][64];
+    __shared__ uint32_t seg[2];  // next group, end of segment
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t groups = na / 64;
+    if (threadIdx.x == 0) {
+        const unsigned long long total = gpre[groups];
+        const uint32_t tl = (uint32_t)(total * blockIdx.x / gridDim.x);
+        const uint32_t th = (uint32_t)(total * (blockIdx.x + 1) / gridDim.x);
+        auto first_at_least = [&](uint32_t v) -> uint32_t {  // first g with gpre[g] >= v
+            uint32_t a = 0, b = groups;
+            while (a < b) {
+                const uint32_t m = (a + b) >> 1;
+                if (gpre[m] < v) a = m + 1;
+                else b = m;
+            }
+            return a;
+        };
+        seg[0] = first_at_least(tl);
+        seg[1] = blockIdx.x + 1 == gridDim.x ? groups : first_at_least(th);
+    }
+    __syncthreads();
+    const uint32_t total_nodes = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
+    const uint32_t end = seg[1];
+    unsigned long long vis = 0, acc = 0;
+    while (true) {
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(&seg[0], 1u);
+        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
+        if (g >= end) break;
+        walk_group<kDebugAccel, true>(g, total_nodes, stage[w], sorted_src, pos, vel, node4,
+                                      ncount, prun, pos_out, vel_out, acc_dbg, na, eps2, theta2,
+                                      kv, kp, vis, acc, gcost);
+    }
+    if (lane == 0 && vis) {
+        atomicAdd(&ctr->visits, vis);
+        atomicAdd(&ctr->accepts, acc);
     }
 }
 
@@ -727,8 +823,11 @@ hipError_t bh_alloc(BhBuffers *b, size_t na) {
     GSIM_ALLOC(b->ncount, b->cap);
     GSIM_ALLOC(b->sorted_src, na);
     GSIM_ALLOC(b->ctr, 1);
+    GSIM_ALLOC(b->gcost, na / 64);
+    GSIM_ALLOC(b->gpre, na / 64 + 1);
 #undef GSIM_ALLOC
     if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
+    if (e == hipSuccess) e = hipMemset(b->gcost, 0, sizeof(uint32_t) * (na / 64));
     if (e == hipSuccess) e = hipMemset(b->ncount, 0, sizeof(uint32_t) * b->cap);
     if (e == hipSuccess) e = hipMemset(b->node4, 0, sizeof(float4) * b->cap);
     return e;
@@ -741,7 +840,7 @@ void bh_free(BhBuffers *b) {
     }
     void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->prun, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
-                    b->node4, b->ncount, b->sorted_src, b->ctr};
+                    b->node4, b->ncount, b->sorted_src, b->ctr, b->gcost, b->gpre};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
 }
@@ -796,6 +895,19 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     float2 *acc = dbg ? s->accel : nullptr;
     if (s->W == 64) {
         const uint32_t groups = na / 64;
+        if (s->trav_mode & 4u) {
+            bh_cost_scan_kernel<<<1, 1024, 0, s->stream>>>(b.gcost, groups, b.gpre);
+            const uint32_t P = s->trav_segments;
+            if (dbg)
+                bh_traverse_bal_kernel<true><<<P, 1024, 0, s->stream>>>(
+                    b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc,
+                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost);
+            else
+                bh_traverse_bal_kernel<false><<<P, 1024, 0, s->stream>>>(
+                    b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc,
+                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost);
+            return hipGetLastError();
+        }
         const bool persistent = (s->trav_mode & 1u) != 0, staged = (s->trav_mode & 2u) != 0;
         uint32_t g = grid;
         if (persistent) {

@@ -69,6 +69,10 @@ struct BhBuffers {
     uint32_t *sorted_src = nullptr;  // [Na]
     BhCounters *ctr = nullptr;
     uint32_t sorted_slot = 0;        // which keys/idx buffer holds the sorted result
+    // balanced traversal: each group's walk cost from the previous step and
+    // the exclusive prefix of (cost + 1) that cuts the groups into segments
+    uint32_t *gcost = nullptr;       // [Na/64], zero before the first walk
+    uint32_t *gpre = nullptr;        // [Na/64 + 1]
 };
 
 // Events recorded around one step: e[0] start, e[1] between the two phases,
@@ -118,7 +122,9 @@ struct gsim_sim {
     float eps2 = 0, theta2 = 0, dt = 0, kv = 0, kp = 0;
     uint32_t W = 64;
     uint32_t trav_waves_per_simd = 8;  // persistent traversal width
-    uint32_t trav_mode = 2;            // bit 0 persistent, bit 1 LDS-staged runs
+    uint32_t trav_mode = 2;            // bit 0 persistent, bit 1 LDS-staged runs,
+                                       // bit 2 cost-balanced segments (staged)
+    uint32_t trav_segments = 512;      // balanced traversal: workgroups (2 per CU)
     uint32_t ds_rcp = 0;               // direct sum: 1/r2 from the transcendental unit
     // state
     float2 *pos[2] = {}, *vel[2] = {};

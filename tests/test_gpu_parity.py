@@ -325,11 +325,12 @@ def test_bh_traversal_counts_exact_forces_within_tol(W, case):
     assert np.array_equal(g.pos[real], (pk[real] + (pp.vel[real] + nv) * kp).astype(np.float32))
 
 
-@pytest.mark.parametrize("mode", [0, 1, 2, 3])
+@pytest.mark.parametrize("mode", [0, 1, 2, 3, 6])
 @pytest.mark.parametrize("case", [2, 4])
 def test_bh_traversal_variants_agree(mode, case):
     """Every traversal launch variant (static or persistent groups, scalar or
-    LDS-staged leaf runs) visits exactly the oracle's nodes."""
+    LDS-staged leaf runs, cost-balanced segments) visits exactly the oracle's
+    nodes."""
     name, pos, mass = list(_bh_inputs())[case]
     _say(f"bh traversal mode={mode} {name}")
     p = gsim.Particles(pos, np.zeros_like(pos), mass)
@@ -347,6 +348,27 @@ def test_bh_traversal_variants_agree(mode, case):
     assert (V, A) == (oV, oA)
     inside = t["sorted_src"][t["sorted_src"] < len(mk)]
     _force_ok(a[inside], oa[inside])
+
+
+@pytest.mark.parametrize("segments", [1, 7, 512, 4096])
+def test_bh_balanced_traversal_matches_static(segments):
+    """The cost-balanced traversal (segments cut by the previous step's walk
+    costs) gives bit for bit the static launch's results over several steps,
+    for any number of segments, including more segments than groups."""
+    _say(f"bh balanced segments={segments}")
+    p = gsim.generate("two-exponential-disks", 100000, 33.0, 1.0, 10.0, seed=208)
+    out = []
+    for mode in (2, 6):
+        s = gsim.Simulation(p, "barnes-hut", accuracy_parameter=0.7)
+        s.set_tuning(gsim.TUNE_BH_TRAVERSAL, mode)
+        s.set_tuning(gsim.TUNE_BH_SEGMENTS, segments)
+        s.step(4)
+        out.append((s.read(), tuple(int(x) for x in s.dump(gsim.DUMP_COUNTERS))))
+        s.close()
+    (a, ca), (b, cb) = out
+    assert ca == cb
+    assert np.array_equal(a.pos, b.pos) and np.array_equal(a.vel, b.vel)
+    assert np.array_equal(a.mass, b.mass)
 
 
 def test_bh_theta_zero_equals_sorted_direct_sum():
