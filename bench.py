@@ -66,6 +66,12 @@ def bench_direct_sum(steps: int, warmup: int) -> dict:
     elapsed = time.perf_counter() - t0
     t = sim.timing(reset=True)
     sim.close()
+    # PCIe-inclusive: host arrays in, one step, host arrays out
+    t0 = time.perf_counter()
+    with gsim.Simulation(p, "direct-sum") as once:
+        once.step(1)
+        once.read()
+    pcie_ms = (time.perf_counter() - t0) * 1e3
     pairs = float(DS_N) * DS_N
     ms_step = elapsed * 1e3 / steps
     kernel_ms = t["force_ms"] / max(t["force_launches"], 1)
@@ -77,6 +83,7 @@ def bench_direct_sum(steps: int, warmup: int) -> dict:
         "ms_per_step": ms_step,
         "kernel_ms": kernel_ms,
         "combine_ms": t["integrate_ms"] / max(t["steps"], 1),
+        "pcie_inclusive_ms": pcie_ms,
         "roofline": {
             "bound": "valu",
             "kernel": "ds_sym_kernel",
@@ -393,6 +400,7 @@ def main() -> int:
                                "scale 20, circular velocities + 5% dispersion, masses "
                                "U[1,10]), G=1, softening 0.2, dt=1e-3",
                    "particles": DS_N, "global_batch": 1, "parallelism": "single-gpu"},
+        "pcie_inclusive_ms_one_step": ds["pcie_inclusive_ms"],
         "roofline": ds["roofline"],
         "cpu_baseline": cpu,
         "secondary": [x for x in (bh, bt, sv) if x],

@@ -346,46 +346,48 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
     const uint32_t waves = gridDim.x * 4;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
-    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    for (uint32_t L = wave_id; L < nl; L += waves) {
-        const uint32_t key = leaf_key[L];
-        const uint32_t r = key >> kBatchRankShift;
-        const uint32_t sys_end = bh_prefix[r] + stats[r].n_in;
-        const uint32_t e = __builtin_amdgcn_readfirstlane(min(leaf_first[L + 1], sys_end));
-        const uint32_t f = __builtin_amdgcn_readfirstlane(min(leaf_first[L], e));
-        float sx = 0.0f, sy = 0.0f, sm = 0.0f;
-        float4 next = f + lane < e ? prod[f + lane] : zero4;
-        for (uint32_t base = f; base < e; base += 64) {
-            const uint32_t cnt = min(64u, e - base);
-            stage[w][lane] = next;
-            __builtin_amdgcn_wave_barrier();
-            const uint32_t nb = base + 64 + lane;
-            next = nb < e ? prod[nb] : zero4;
-            uint32_t k = 0;
-            for (; k + 8 <= cnt; k += 8) {
-                float4 q[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) q[u] = stage[w][k + u];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    sx = sx + q[u].x;
-                    sy = sy + q[u].y;
-                    sm = sm + q[u].z;
-                }
-            }
-            for (; k < cnt; ++k) {
-                const float4 q = stage[w][k];
-                sx = sx + q.x;
-                sy = sy + q.y;
-                sm = sm + q.z;
-            }
-            __builtin_amdgcn_wave_barrier();
+    // small leaves: each wave takes 64 consecutive leaves, one per lane
+    for (uint32_t base = wave_id * 64; base < nl; base += waves * 64) {
+        const uint32_t L = base + lane;
+        const bool valid = L < nl;
+        uint32_t f = 0, e = 0, key = 0;
+        if (valid) {
+            key = leaf_key[L];
+            const uint32_t r = key >> kBatchRankShift;
+            const uint32_t sys_end = bh_prefix[r] + stats[r].n_in;
+            e = min(leaf_first[L + 1], sys_end);
+            f = min(leaf_first[L], e);
         }
-        if (lane == 0) {
-            leaf_sum[L] = make_float4(sx, sy, sm, __uint_as_float(e - f));
+        const bool small = valid && e - f <= kSmallLeaf;
+        if (small) {
+            const float4 r = lane_leaf_sum(prod, f, e);
+            leaf_sum[L] = make_float4(r.x, r.y, r.z, __uint_as_float(e - f));
+        }
+        if (valid) {
             const uint32_t T = tile_of(key);
             if (L == 0 || tile_of(leaf_key[L - 1]) != T) tile_range[2 * T] = L;
             if (L + 1 == nl || tile_of(leaf_key[L + 1]) != T) tile_range[2 * T + 1] = L + 1;
+        }
+    }
+    // larger leaves: leaf L goes to wave L mod waves (neighbouring dense
+    // leaves to different waves), streamed by the whole wave
+    for (uint32_t i0 = 0; wave_id + i0 * waves < nl; i0 += 64) {
+        const uint32_t L = wave_id + (i0 + lane) * waves;
+        uint32_t f = 0, e = 0;
+        if (L < nl) {
+            const uint32_t r = leaf_key[L] >> kBatchRankShift;
+            e = min(leaf_first[L + 1], bh_prefix[r] + stats[r].n_in);
+            f = min(leaf_first[L], e);
+        }
+        unsigned long long big = __ballot(L < nl && e - f > kSmallLeaf);
+        while (big) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(big);
+            big &= big - 1;
+            const uint32_t bf = __builtin_amdgcn_readfirstlane(__shfl(f, j, 64));
+            const uint32_t be = __builtin_amdgcn_readfirstlane(__shfl(e, j, 64));
+            const float4 r = wave_leaf_sum(stage[w], prod, bf, be, lane);
+            if (lane == 0)
+                leaf_sum[wave_id + (i0 + j) * waves] = make_float4(r.x, r.y, r.z, __uint_as_float(be - bf));
         }
     }
     for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n; k += gridDim.x * 256) {

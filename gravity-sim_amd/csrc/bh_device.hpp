@@ -113,4 +113,60 @@ __device__ __forceinline__ void wave_scan2(uint32_t &a, uint32_t &b) {
     }
 }
 
+// ---- leaf sums: m*x, m*y, m of a leaf's particles in ascending sorted order --
+// Both forms add the products one after another in the same order, so a leaf
+// gets the same bits whichever sums it.
+
+// Leaves of at most kSmallLeaf particles are summed by one lane.
+constexpr uint32_t kSmallLeaf = 24;
+
+__device__ __forceinline__ float4 lane_leaf_sum(const float4 *__restrict__ prod, uint32_t f,
+                                                uint32_t e) {
+    float sx = 0.0f, sy = 0.0f, sm = 0.0f;
+    for (uint32_t k = f; k < e; ++k) {
+        const float4 q = prod[k];
+        sx = sx + q.x;
+        sy = sy + q.y;
+        sm = sm + q.z;
+    }
+    return make_float4(sx, sy, sm, 0.0f);
+}
+
+// A larger leaf [f, e) (wave-uniform bounds) by the whole wave: 64 products
+// per coalesced load into the wave's LDS stage, the next chunk in flight while
+// the current one is summed (every lane computes the same running sum).
+__device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__restrict__ prod,
+                                                uint32_t f, uint32_t e, uint32_t lane) {
+    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float sx = 0.0f, sy = 0.0f, sm = 0.0f;
+    float4 next = f + lane < e ? prod[f + lane] : zero4;
+    for (uint32_t base = f; base < e; base += 64) {
+        const uint32_t cnt = min(64u, e - base);
+        stage[lane] = next;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t nb = base + 64 + lane;
+        next = nb < e ? prod[nb] : zero4;
+        uint32_t k = 0;
+        for (; k + 8 <= cnt; k += 8) {
+            float4 q[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) q[u] = stage[k + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                sx = sx + q[u].x;
+                sy = sy + q[u].y;
+                sm = sm + q[u].z;
+            }
+        }
+        for (; k < cnt; ++k) {
+            const float4 q = stage[k];
+            sx = sx + q.x;
+            sy = sy + q.y;
+            sm = sm + q.z;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    return make_float4(sx, sy, sm, 0.0f);
+}
+
 }  // namespace gsim
