@@ -48,7 +48,7 @@ void free_batch(gsim_batch *b) {
                     d.bh_sys, d.bh_prefix, d.ds_sys, d.ds_prefix, d.items, d.counts, d.stats,
                     d.keys[0], d.keys[1], d.idx[0], d.idx[1], d.hist, d.digit_total,
                     d.block_cells, d.block_leaves, d.sdep, d.cpos, d.sorted_src, d.prod,
-                    d.leaf_sum, d.leaf_first, d.leaf_key, d.tile_range, d.top, d.node4,
+                    d.leaf_sum, d.leaf_first, d.leaf_key, d.tile_range, d.live_tiles, d.top, d.node4,
                     d.ncount, d.prun, d.breaker};
     for (void *p : ptrs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -263,6 +263,7 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     GSIM_BALLOC(d.leaf_first, cap + 1);
     GSIM_BALLOC(d.leaf_key, cap);
     GSIM_BALLOC(d.tile_range, 2 * (size_t)gsim::kTiles * max_systems);
+    GSIM_BALLOC(d.live_tiles, (size_t)gsim::kTiles * max_systems);
     GSIM_BALLOC(d.top, (size_t)gsim::kTiles * max_systems);
     GSIM_BALLOC(d.node4, b->node_capacity);
     GSIM_BALLOC(d.ncount, b->node_capacity);

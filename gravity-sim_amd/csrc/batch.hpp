@@ -39,6 +39,8 @@ struct BatchCounts {
     uint32_t n_cells;         // cell nodes of all trees (emit)
     uint32_t n_leaves;        // occupied leaves of all trees (emit)
     uint32_t n_sys;           // all systems (direct sum and Barnes-Hut)
+    uint32_t n_live_tiles;    // (system, depth-5 tile) pairs holding leaves (leaf kernel)
+    uint32_t pad;
     unsigned long long accepts;  // Barnes-Hut real-lane acceptances, all steps
 };
 
@@ -73,6 +75,7 @@ struct BatchDevice {
     float4 *prod = nullptr, *leaf_sum = nullptr;
     uint32_t *leaf_first = nullptr, *leaf_key = nullptr;
     uint32_t *tile_range = nullptr;
+    uint32_t *live_tiles = nullptr;  // tiles with leaves, in no particular order
     TileCell *top = nullptr;
     // nodes of all trees
     float4 *node4 = nullptr;

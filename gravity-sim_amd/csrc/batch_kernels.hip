@@ -54,6 +54,7 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     if (tid == 0) {
         counts->n_cells = 0;
         counts->n_leaves = 0;
+        counts->n_live_tiles = 0;
     }
     for (uint32_t f = tid; f < n; f += stride) {
         const uint32_t r = rank_of(bh_prefix, nb, f);
@@ -338,9 +339,10 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
     const uint32_t *__restrict__ leaf_first, const uint32_t *__restrict__ leaf_key,
     const float4 *__restrict__ prod, const uint32_t *__restrict__ keys,
     const uint32_t *__restrict__ cpos, const uint32_t *__restrict__ breaker,
-    const BatchCounts *__restrict__ counts, const uint32_t *__restrict__ bh_prefix,
+    BatchCounts *counts, const uint32_t *__restrict__ bh_prefix,
     const RankStats *__restrict__ stats, float4 *__restrict__ leaf_sum,
-    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ prun) {
+    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ live_tiles,
+    uint32_t *__restrict__ prun) {
     __shared__ float4 stage[4][64];
     const uint32_t n = counts->n_bh_particles, nl = counts->n_leaves, ncells = counts->n_cells;
     const uint32_t waves = gridDim.x * 4;
@@ -365,7 +367,10 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
         }
         if (valid) {
             const uint32_t T = tile_of(key);
-            if (L == 0 || tile_of(leaf_key[L - 1]) != T) tile_range[2 * T] = L;
+            if (L == 0 || tile_of(leaf_key[L - 1]) != T) {
+                tile_range[2 * T] = L;
+                live_tiles[atomicAdd(&counts->n_live_tiles, 1u)] = T;
+            }
             if (L + 1 == nl || tile_of(leaf_key[L + 1]) != T) tile_range[2 * T + 1] = L + 1;
         }
     }
@@ -410,15 +415,15 @@ __global__ __launch_bounds__(256) void bb_tile_kernel(
     const uint32_t *__restrict__ tile_range, const uint32_t *__restrict__ leaf_first,
     const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
     const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
-    const BatchCounts *__restrict__ counts, float4 *__restrict__ node4,
-    uint32_t *__restrict__ ncount, TileCell *__restrict__ top) {
+    const BatchCounts *__restrict__ counts, const uint32_t *__restrict__ live_tiles,
+    float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top) {
     __shared__ TileCell l10[1024];
     __shared__ TileCell up[341];
-    const uint32_t ntiles = counts->n_bh * kTiles;
+    const uint32_t nlive = counts->n_live_tiles;
     const uint32_t t = threadIdx.x;
-    for (uint32_t T = blockIdx.x; T < ntiles; T += gridDim.x) {
+    for (uint32_t i = blockIdx.x; i < nlive; i += gridDim.x) {  // only tiles holding leaves
+        const uint32_t T = live_tiles[i];
         const uint32_t lb = tile_range[2 * T], le = tile_range[2 * T + 1];
-        if (lb == le) continue;  // uniform across the workgroup
         for (uint32_t c = t; c < 1024; c += 256)
             l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
         __syncthreads();
@@ -712,10 +717,11 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
                                           d.sorted_src, d.breaker, d.stats);
     bb_leaf_kernel<<<1024, 256, 0, st>>>(d.leaf_first, d.leaf_key, d.prod, sk, d.cpos, d.breaker,
                                          d.counts, d.bh_prefix, d.stats, d.leaf_sum,
-                                         d.tile_range, d.prun);
+                                         d.tile_range, d.live_tiles, d.prun);
     const uint32_t g_tile = (uint32_t)std::min<size_t>((size_t)b->max_systems * kTiles, 4096);
     bb_tile_kernel<<<g_tile, 256, 0, st>>>(d.tile_range, d.leaf_first, d.leaf_key, d.leaf_sum,
-                                           d.sdep, d.cpos, d.counts, d.node4, d.ncount, d.top);
+                                           d.sdep, d.cpos, d.counts, d.live_tiles, d.node4,
+                                           d.ncount, d.top);
     bb_top_kernel<<<std::min<uint32_t>(b->max_systems, 256), 1024, 0, st>>>(
         d.top, d.tile_range, d.sdep, d.cpos, d.counts, d.node4, d.ncount);
     const uint32_t waves = 1024u * b->waves_per_simd;
