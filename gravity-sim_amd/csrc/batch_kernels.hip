@@ -513,56 +513,15 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
         px = p.x;
         py = p.y;
     }
-    const float eps2 = d.eps2, theta2 = d.theta2;
     const uint32_t end = __builtin_amdgcn_readfirstlane(min(st.end_slot, node_cap));
-    float ax = 0.0f, ay = 0.0f;
-    uint32_t ind = __builtin_amdgcn_readfirstlane(st.root);
-    while (ind < end) {
-        const float4 nd = node4[ind];
-        if (nd.w == 0.0f) {
-            uint32_t n = prun[ind];
-            n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), end - ind));
-            const uint32_t rend = ind + n;
-            for (uint32_t base = ind; base < rend; base += 64) {
-                const uint32_t cnt = min(64u, rend - base);
-                stage[lane] = lane < cnt ? node4[base + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                __builtin_amdgcn_wave_barrier();
-                uint32_t k = 0;
-                for (; k + 8 <= cnt; k += 8) {
-                    float4 q[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) q[u] = stage[k + u];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
-                }
-                for (; k < cnt; ++k) {
-                    const float4 q = stage[k];
-                    accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
-            acc += (unsigned long long)n * nreal;
-            ind = rend;
-            continue;
-        }
-        const uint32_t cnt = ncount[ind];
-        const float dx = nd.x - px;
-        const float dy = nd.y - py;
-        const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
-        const bool ok = (nd.w <= r2 * theta2) || !real;
-        if (__ballot(ok) == ~0ull) {
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float s = (nd.z * inv) * (inv * inv);
-            ax = __builtin_fmaf(dx, s, ax);
-            ay = __builtin_fmaf(dy, s, ay);
-            ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
-            acc += nreal;
-        } else {
-            ind = __builtin_amdgcn_readfirstlane(ind + 1u);
-        }
-    }
+    const uint32_t root = __builtin_amdgcn_readfirstlane(st.root);
+    Partials P;
+    unsigned long long vis = 0;
+    (void)walk_partials<true>(root, end, stage, node4, ncount, prun, px, py, real, nreal, d.eps2,
+                              d.theta2, 0xfu, P, vis, acc);
     if (real) {
+        float ax, ay;
+        partial_sum(P, ax, ay);
         const float2 v = vel[src];
         const float nvx = v.x + ax * d.kv, nvy = v.y + ay * d.kv;
         pos_out[src] = make_float2(px + (v.x + nvx) * d.kp, py + (v.y + nvy) * d.kp);

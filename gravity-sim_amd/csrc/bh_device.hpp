@@ -137,15 +137,17 @@ __device__ __forceinline__ float4 lane_leaf_sum(const float4 *__restrict__ prod,
 // the current one is summed (every lane computes the same running sum).
 __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__restrict__ prod,
                                                 uint32_t f, uint32_t e, uint32_t lane) {
-    const float4 zero4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    typedef float v4 __attribute__((ext_vector_type(4)));  // stays in registers
+    const v4 *vp = reinterpret_cast<const v4 *>(prod);
+    const v4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     float sx = 0.0f, sy = 0.0f, sm = 0.0f;
-    float4 next = f + lane < e ? prod[f + lane] : zero4;
+    v4 next = f + lane < e ? vp[f + lane] : zero;
     for (uint32_t base = f; base < e; base += 64) {
         const uint32_t cnt = min(64u, e - base);
-        stage[lane] = next;
+        reinterpret_cast<v4 *>(stage)[lane] = next;
         __builtin_amdgcn_wave_barrier();
         const uint32_t nb = base + 64 + lane;
-        next = nb < e ? prod[nb] : zero4;
+        next = nb < e ? vp[nb] : zero;
         uint32_t k = 0;
         for (; k + 8 <= cnt; k += 8) {
             float4 q[8];
@@ -167,6 +169,165 @@ __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__r
         __builtin_amdgcn_wave_barrier();
     }
     return make_float4(sx, sy, sm, 0.0f);
+}
+
+// ---- the walk of one W = 64 group --------------------------------------------
+// Summation order, the same in every W = 64 launch, single-system or batched:
+// the particle runs (leaf buckets) are cut into 64-node chunks numbered in
+// walk order across runs; chunk c is accumulated node by node into partial
+// c mod 4, every accepted cell into partial 0, and the acceleration is
+// ((p0 + p1) + p2) + p3.  One wave can compute all four partials, or four
+// waves one each (a dense-core group split across waves), with the same bits.
+// Native 4-float vector: conditional copies stay in registers (HIP's float4
+// is a union-based struct whose selects the compiler may route through
+// private memory).
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4v load_node(const float4 *__restrict__ node4, uint32_t i, bool ok) {
+    return ok ? reinterpret_cast<const f4v *>(node4)[i] : f4v{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+struct Partials {
+    float x0 = 0.0f, y0 = 0.0f, x1 = 0.0f, y1 = 0.0f, x2 = 0.0f, y2 = 0.0f, x3 = 0.0f, y3 = 0.0f;
+};
+
+// Runs f(ax, ay) on partial k (wave-uniform), each case on named registers so
+// the partials never leave them.
+template <class F>
+__device__ __forceinline__ void with_partial(Partials &P, uint32_t k, F &&f) {
+    switch (k) {
+        case 0: f(P.x0, P.y0); break;
+        case 1: f(P.x1, P.y1); break;
+        case 2: f(P.x2, P.y2); break;
+        default: f(P.x3, P.y3); break;
+    }
+}
+
+__device__ __forceinline__ void partial_sum(const Partials &P, float &ax, float &ay) {
+    ax = ((P.x0 + P.x1) + P.x2) + P.x3;
+    ay = ((P.y0 + P.y1) + P.y2) + P.y3;
+}
+
+// Walks the pre-order nodes [ind, end) for the group's lanes (ForceShader.comp
+// :111-141): a cell is accepted iff every real lane has radius <= (dx*dx +
+// dy*dy + eps2) * theta2, rounded as the shader writes it, then ind += count,
+// else ind++.  Particle nodes are always accepted, so a run of them is a plain
+// direct sum.  `parts` (bit k) selects the partials this wave computes; the
+// wave computing partial 0 adds the cells and counts the real-lane visits
+// and acceptances.  kStaged: run chunks come through the wave's LDS stage (one
+// coalesced load per chunk, broadcast reads; walking all four partials, the
+// next chunk of a run is loaded while the current one is summed); otherwise
+// through scalar loads.  Returns
+// the walk's VALU weight (a cell visit ~12 instructions, a particle ~10) in
+// units of 4, capped at 65535.
+template <bool kStaged>
+__device__ __forceinline__ uint32_t walk_partials(
+    uint32_t ind, uint32_t end, float4 *stage, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun, float px, float py,
+    bool real, uint32_t nreal, float eps2, float theta2, uint32_t parts, Partials &P,
+    unsigned long long &vis, unsigned long long &acc) {
+    const uint32_t lane = threadIdx.x & 63;
+    const bool first = (parts & 1u) != 0;
+    uint32_t chunk = 0, cells = 0, runs = 0;
+    while (ind < end) {
+        const float4 nd = node4[ind];
+        if (nd.w == 0.0f) {
+            uint32_t n = prun[ind];
+            n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), end - ind));
+            const uint32_t rend = ind + n;
+            if (kStaged) {
+                // all partials: every chunk of the run is this wave's, so the
+                // next one is loaded while the current one is summed
+                // the stage holds (x, y, m) records packed at 12 bytes, so a
+                // broadcast read of 48 bytes brings four particles
+                const bool all = parts == 0xfu;
+                f4v next = load_node(node4, ind + lane, all && lane < min(64u, n));
+                float *st = reinterpret_cast<float *>(stage);
+                const f4v *st4 = reinterpret_cast<const f4v *>(stage);
+                for (uint32_t base = ind; base < rend; base += 64, ++chunk) {
+                    const uint32_t k = chunk & 3u;
+                    if (!((parts >> k) & 1u)) continue;
+                    const uint32_t cnt = min(64u, rend - base);
+                    f4v cur;
+                    if (all) {
+                        cur = next;
+                        const uint32_t nb = base + 64;
+                        const uint32_t ncnt = nb < rend ? min(64u, rend - nb) : 0u;
+                        next = load_node(node4, nb + lane, lane < ncnt);
+                    } else {
+                        cur = load_node(node4, base + lane, lane < cnt);
+                    }
+                    st[3 * lane] = cur.x;
+                    st[3 * lane + 1] = cur.y;
+                    st[3 * lane + 2] = cur.z;
+                    __builtin_amdgcn_wave_barrier();
+                    with_partial(P, k, [&](float &ax, float &ay) {
+                        uint32_t j = 0;
+                        for (; j + 8 <= cnt; j += 8) {
+                            f4v r[6];
+#pragma unroll
+                            for (int u = 0; u < 6; ++u) r[u] = st4[3 * (j / 4) + u];
+                            const float *q = reinterpret_cast<const float *>(r);
+#pragma unroll
+                            for (int u = 0; u < 8; ++u)
+                                accumulate(q[3 * u], q[3 * u + 1], q[3 * u + 2], px, py, eps2, ax, ay);
+                        }
+                        for (; j < cnt; ++j)
+                            accumulate(st[3 * j], st[3 * j + 1], st[3 * j + 2], px, py, eps2, ax, ay);
+                    });
+                    __builtin_amdgcn_wave_barrier();
+                }
+            } else {
+                for (uint32_t base = ind; base < rend; base += 64, ++chunk) {
+                    const uint32_t k = chunk & 3u;
+                    if (!((parts >> k) & 1u)) continue;
+                    const uint32_t ce = base + min(64u, rend - base);
+                    with_partial(P, k, [&](float &ax, float &ay) {
+                        uint32_t j = base;
+                        for (; j + 4 <= ce; j += 4) {
+                            float4 q[4];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) q[u] = node4[j + u];
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
+                        }
+                        for (; j < ce; ++j) {
+                            const float4 q = node4[j];
+                            accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
+                        }
+                    });
+                }
+            }
+            if (first) {
+                vis += (unsigned long long)n * nreal;
+                acc += (unsigned long long)n * nreal;
+            }
+            runs += n;
+            ind = rend;
+            continue;
+        }
+        ++cells;
+        const uint32_t cnt = ncount[ind];
+        const float dx = nd.x - px;
+        const float dy = nd.y - py;
+        const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
+        const bool ok = (nd.w <= r2 * theta2) || !real;
+        if (first) vis += nreal;
+        if (__ballot(ok) == ~0ull) {
+            if (first) {
+                const float inv = __builtin_amdgcn_rsqf(r2);
+                const float sc = (nd.z * inv) * (inv * inv);
+                P.x0 = __builtin_fmaf(dx, sc, P.x0);
+                P.y0 = __builtin_fmaf(dy, sc, P.y0);
+                acc += nreal;
+            }
+            ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
+        } else {
+            ind = __builtin_amdgcn_readfirstlane(ind + 1u);
+        }
+    }
+    return min((cells * 12u + runs * 10u) >> 2, 65535u);
 }
 
 }  // namespace gsim

@@ -459,9 +459,7 @@ __global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict
 // streamed as a plain direct sum without votes; cell nodes are voted on with
 // r2 = (dx*dx + dy*dy) + eps2 rounded exactly as the reference's shader, so
 // the visited and accepted node sets are the reference's.
-// One group's walk and integrate.  kStaged: leaf-bucket runs come through a
-// per-wave LDS stage (one coalesced 1 KB load per 64 nodes, broadcast reads)
-// instead of scalar loads.
+// One group's walk (all four partials) and integrate.
 template <bool kDebugAccel, bool kStaged>
 __device__ __forceinline__ void walk_group(
     uint32_t g, uint32_t total, float4 *stage, const uint32_t *__restrict__ sorted_src,
@@ -473,7 +471,6 @@ __device__ __forceinline__ void walk_group(
     uint32_t *__restrict__ gcost = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[g * 64 + lane];
-    uint32_t cells = 0, runs = 0;  // walk cost: cell visits, streamed particle nodes
     const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
@@ -482,89 +479,19 @@ __device__ __forceinline__ void walk_group(
         py = p.y;
     }
     const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
-    float ax = 0.0f, ay = 0.0f;
-    uint32_t ind = 0;
-    while (ind < total) {
-        const float4 nd = node4[ind];
-        if (nd.w == 0.0f) {
-            uint32_t n = prun[ind];
-            n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), total - ind));
-            const uint32_t end = ind + n;
-            if (kStaged) {
-                // the next chunk's records are loaded while this one is summed
-                float4 next = lane < min(64u, n) ? node4[ind + lane]
-                                                  : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                for (uint32_t base = ind; base < end; base += 64) {
-                    const uint32_t cnt = min(64u, end - base);
-                    stage[lane] = next;
-                    __builtin_amdgcn_wave_barrier();
-                    const uint32_t nb = base + 64;
-                    const uint32_t ncnt = nb < end ? min(64u, end - nb) : 0u;
-                    next = lane < ncnt ? node4[nb + lane] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    uint32_t k = 0;
-                    for (; k + 8 <= cnt; k += 8) {
-                        float4 q[8];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) q[u] = stage[k + u];
-#pragma unroll
-                        for (int u = 0; u < 8; ++u)
-                            accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
-                    }
-                    for (; k < cnt; ++k) {
-                        const float4 q = stage[k];
-                        accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
-                    }
-                    __builtin_amdgcn_wave_barrier();
-                }
-            } else {
-                uint32_t j = ind;
-                for (; j + 4 <= end; j += 4) {
-                    float4 q[4];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) q[u] = node4[j + u];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u)
-                        accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
-                }
-                for (; j < end; ++j) {
-                    const float4 q = node4[j];
-                    accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
-                }
-            }
-            vis += (unsigned long long)n * nreal;
-            acc += (unsigned long long)n * nreal;
-            runs += n;
-            ind = end;
-            continue;
-        }
-        ++cells;
-        const uint32_t cnt = ncount[ind];
-        const float dx = nd.x - px;
-        const float dy = nd.y - py;
-        const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
-        const bool ok = (nd.w <= r2 * theta2) || !real;
-        vis += nreal;
-        if (__ballot(ok) == ~0ull) {
-            const float inv = __builtin_amdgcn_rsqf(r2);
-            const float s = (nd.z * inv) * (inv * inv);
-            ax = __builtin_fmaf(dx, s, ax);
-            ay = __builtin_fmaf(dy, s, ay);
-            ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
-            acc += nreal;
-        } else {
-            ind = __builtin_amdgcn_readfirstlane(ind + 1u);
-        }
-    }
+    Partials P;
+    const uint32_t cost = walk_partials<kStaged>(0u, total, stage, node4, ncount, prun, px, py, real,
+                                                 nreal, eps2, theta2, 0xfu, P, vis, acc);
     if (real) {
+        float ax, ay;
+        partial_sum(P, ax, ay);
         const float2 v = vel[src];
         const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
         pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
         vel_out[src] = make_float2(nvx, nvy);
         if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
     }
-    // VALU weight of the walk (a cell visit ~12 instructions, a particle ~10),
-    // in units of 4, capped so 2^15 groups' prefix fits 32 bits
-    if (gcost && lane == 0) gcost[g] = min((cells * 12u + runs * 10u) >> 2, 65535u);
+    if (gcost && lane == 0) gcost[g] = cost;
 }
 
 // Exclusive prefix of (cost + 1) over the groups, and the total at [groups]
@@ -598,12 +525,21 @@ __global__ __launch_bounds__(1024) void bh_cost_scan_kernel(const uint32_t *__re
 // Cost-balanced traversal: workgroup k of P walks the contiguous groups whose
 // cost prefix starts in [k T / P, (k+1) T / P) (T = total cost of the last
 // walk), so every workgroup gets the same estimated work while neighbouring
-// groups -- which share their near field in the caches -- stay together; its
-// 16 waves take the segment's groups in order from an LDS counter.  Which wave
-// walks a group never changes the group's result.  Measured at 500k (two
-// disks, theta 1): per-CU work max/mean 1.45 -> 1.13, but the launch then ends
-// on single dense-core groups (one wave walks ~5,400 particle nodes), so the
-// gain is ~2.5%; the static launch stays the default.
+// groups -- which share their near field in the caches -- stay together.
+// Dense-core groups (last cost above `split_x` times the mean) would still
+// bound the launch alone on one wave (one walks ~5,400 particle nodes at the
+// 500k workload), so up to kSplitSlots of them per segment are split into
+// four units, one per partial of the canonical summation (walk_partials);
+// the four waves leave their partials in LDS and the last to finish adds them
+// in order and integrates -- the same bits as one wave walking the group.
+// The 16 waves take units from an LDS counter: split parts first, then the
+// segment's other groups.  Which wave walks what never changes a result.
+// Measured at 500k (two disks, theta 1), traversal ms: static one wave per
+// group 0.336, balanced 0.320, balanced with groups above 5x the mean split
+// 0.311-0.316 (per-CU work max/mean 1.45 -> 1.13).
+constexpr uint32_t kSplitSlots = 16;
+constexpr uint32_t kSegList = 1024;
+
 template <bool kDebugAccel>
 __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
@@ -611,15 +547,19 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun,
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
-    float kv, float kp, const uint32_t *__restrict__ gpre, uint32_t *__restrict__ gcost) {
+    float kv, float kp, const uint32_t *__restrict__ gpre, uint32_t *__restrict__ gcost,
+    uint32_t split_x) {
     __shared__ float4 stage[16][64];
-    __shared__ uint32_t seg[2];  // next group, end of segment
+    __shared__ uint32_t seg[4];            // first group, end, next unit, list fill
+    __shared__ uint32_t list[kSegList];    // split groups first, then the others
+    __shared__ float2 hpart[kSplitSlots][4][64];
+    __shared__ uint32_t hdone[kSplitSlots];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t groups = na / 64;
+    const unsigned long long total_cost = gpre[groups];
     if (threadIdx.x == 0) {
-        const unsigned long long total = gpre[groups];
-        const uint32_t tl = (uint32_t)(total * blockIdx.x / gridDim.x);
-        const uint32_t th = (uint32_t)(total * (blockIdx.x + 1) / gridDim.x);
+        const uint32_t tl = (uint32_t)(total_cost * blockIdx.x / gridDim.x);
+        const uint32_t th = (uint32_t)(total_cost * (blockIdx.x + 1) / gridDim.x);
         auto first_at_least = [&](uint32_t v) -> uint32_t {  // first g with gpre[g] >= v
             uint32_t a = 0, b = groups;
             while (a < b) {
@@ -631,19 +571,92 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         };
         seg[0] = first_at_least(tl);
         seg[1] = blockIdx.x + 1 == gridDim.x ? groups : first_at_least(th);
+        seg[2] = 0;
+        seg[3] = 0;
+    }
+    if (threadIdx.x < kSplitSlots) hdone[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t lo = seg[0], hi = seg[1];
+    const uint32_t count = hi - lo, listed = min(count, kSegList);
+    const uint32_t i = threadIdx.x;  // one listed group per thread (kSegList == blockDim)
+    bool split = false;
+    if (split_x && i < listed &&
+        (unsigned long long)(gcost[lo + i] + 1u) * groups > split_x * total_cost) {
+        const uint32_t slot = atomicAdd(&seg[3], 1u);
+        if (slot < kSplitSlots) {
+            list[slot] = lo + i;
+            split = true;
+        }
     }
     __syncthreads();
+    const uint32_t nsplit = min(seg[3], kSplitSlots);
+    __syncthreads();
+    if (threadIdx.x == 0) seg[3] = nsplit;
+    __syncthreads();
+    if (i < listed && !split) list[atomicAdd(&seg[3], 1u)] = lo + i;
+    __syncthreads();
     const uint32_t total_nodes = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
-    const uint32_t end = seg[1];
+    const uint32_t units = 4 * nsplit + (count - nsplit);
     unsigned long long vis = 0, acc = 0;
     while (true) {
-        uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(&seg[0], 1u);
-        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
-        if (g >= end) break;
-        walk_group<kDebugAccel, true>(g, total_nodes, stage[w], sorted_src, pos, vel, node4,
-                                      ncount, prun, pos_out, vel_out, acc_dbg, na, eps2, theta2,
-                                      kv, kp, vis, acc, gcost);
+        uint32_t u = 0;
+        if (lane == 0) u = atomicAdd(&seg[2], 1u);
+        u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
+        if (u >= units) break;
+        if (u >= 4 * nsplit) {  // a whole group
+            const uint32_t v = u - 4 * nsplit + nsplit;
+            const uint32_t g = __builtin_amdgcn_readfirstlane(v < listed ? list[v] : lo + v);
+            walk_group<kDebugAccel, true>(g, total_nodes, stage[w], sorted_src, pos, vel, node4,
+                                          ncount, prun, pos_out, vel_out, acc_dbg, na, eps2,
+                                          theta2, kv, kp, vis, acc, gcost);
+            continue;
+        }
+        // one partial of a split group
+        const uint32_t h = u >> 2, part = u & 3u;
+        const uint32_t g = __builtin_amdgcn_readfirstlane(list[h]);
+        const uint32_t src = sorted_src[g * 64 + lane];
+        const bool real = src < na;
+        float px = 0.0f, py = 0.0f;
+        if (real) {
+            const float2 p = pos[src];
+            px = p.x;
+            py = p.y;
+        }
+        const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
+        Partials P;
+        const uint32_t cost = walk_partials<true>(0u, total_nodes, stage[w], node4, ncount, prun,
+                                                  px, py, real, nreal, eps2, theta2, 1u << part,
+                                                  P, vis, acc);
+        float ax = 0.0f, ay = 0.0f;
+        with_partial(P, part, [&](float &px_, float &py_) {
+            ax = px_;
+            ay = py_;
+        });
+        hpart[h][part][lane] = make_float2(ax, ay);
+        if (part == 0 && lane == 0) gcost[g] = cost;
+        __threadfence_block();
+        uint32_t done = 0;
+        if (lane == 0) done = atomicAdd(&hdone[h], 1u);
+        done = __builtin_amdgcn_readfirstlane(__shfl(done, 0, 64));
+        if (done != 3u) continue;
+        __threadfence_block();  // the last of the four adds the partials in order
+        Partials Q;
+        Q.x0 = hpart[h][0][lane].x;
+        Q.y0 = hpart[h][0][lane].y;
+        Q.x1 = hpart[h][1][lane].x;
+        Q.y1 = hpart[h][1][lane].y;
+        Q.x2 = hpart[h][2][lane].x;
+        Q.y2 = hpart[h][2][lane].y;
+        Q.x3 = hpart[h][3][lane].x;
+        Q.y3 = hpart[h][3][lane].y;
+        if (real) {
+            partial_sum(Q, ax, ay);
+            const float2 v = vel[src];
+            const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
+            pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+            vel_out[src] = make_float2(nvx, nvy);
+            if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+        }
     }
     if (lane == 0 && vis) {
         atomicAdd(&ctr->visits, vis);
@@ -900,11 +913,13 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
             if (dbg)
                 bh_traverse_bal_kernel<true><<<P, 1024, 0, s->stream>>>(
                     b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc,
-                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost);
+                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost,
+                    s->trav_split);
             else
                 bh_traverse_bal_kernel<false><<<P, 1024, 0, s->stream>>>(
                     b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc,
-                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost);
+                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost,
+                    s->trav_split);
             return hipGetLastError();
         }
         const bool persistent = (s->trav_mode & 1u) != 0, staged = (s->trav_mode & 2u) != 0;

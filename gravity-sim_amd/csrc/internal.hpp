@@ -122,9 +122,10 @@ struct gsim_sim {
     float eps2 = 0, theta2 = 0, dt = 0, kv = 0, kp = 0;
     uint32_t W = 64;
     uint32_t trav_waves_per_simd = 8;  // persistent traversal width
-    uint32_t trav_mode = 2;            // bit 0 persistent, bit 1 LDS-staged runs,
+    uint32_t trav_mode = 6;            // bit 0 persistent, bit 1 LDS-staged runs,
                                        // bit 2 cost-balanced segments (staged)
     uint32_t trav_segments = 512;      // balanced traversal: workgroups (2 per CU)
+    uint32_t trav_split = 5;           // balanced traversal: split groups above this x mean cost
     uint32_t ds_rcp = 0;               // direct sum: 1/r2 from the transcendental unit
     // state
     float2 *pos[2] = {}, *vel[2] = {};

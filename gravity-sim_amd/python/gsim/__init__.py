@@ -47,7 +47,7 @@ DUMP_COUNTS, DUMP_RADII, DUMP_NODE_POS, DUMP_NODE_MASS = 4, 5, 6, 7
 DUMP_NODE_TOTAL, DUMP_ACCEL, DUMP_COUNTERS = 8, 9, 10
 DEBUG_ACCEL = 1
 TUNE_DS_RCP, TUNE_BH_WAVES_PER_SIMD, TUNE_BH_TRAVERSAL = 1, 2, 3
-TUNE_DS_SYMMETRIC, TUNE_DS_SYM_COLS, TUNE_BH_SEGMENTS = 4, 5, 6
+TUNE_DS_SYMMETRIC, TUNE_DS_SYM_COLS, TUNE_BH_SEGMENTS, TUNE_BH_SPLIT = 4, 5, 6, 7
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
           4: "NO_DEVICE", 5: "IO", 6: "CAPACITY"}

@@ -162,13 +162,17 @@ typedef enum gsim_tuning {
     GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
     GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
     GSIM_TUNE_BH_TRAVERSAL = 3,       /* bit 0 persistent groups, bit 1 LDS-staged runs;
-                                         6 = cost-balanced contiguous segments        */
+                                         6 = cost-balanced contiguous segments (the
+                                         default)                                       */
     GSIM_TUNE_DS_SYMMETRIC = 4,       /* 0/1: each pair once (Newton's third law); the
                                          default for 16,384 <= Na <= 262,144           */
     GSIM_TUNE_DS_SYM_COLS = 5,        /* symmetric kernel: 1 or 2 column blocks per
                                          wave, 0 = by size                              */
-    GSIM_TUNE_BH_SEGMENTS = 6         /* balanced traversal: workgroups of 16 waves
+    GSIM_TUNE_BH_SEGMENTS = 6,        /* balanced traversal: workgroups of 16 waves
                                          (default 512, two per CU)                      */
+    GSIM_TUNE_BH_SPLIT = 7            /* balanced traversal: groups whose last walk
+                                         cost exceeds this multiple of the mean are
+                                         split across 4 waves (0 = never; default 5)    */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 

@@ -360,6 +360,7 @@ typedef struct {
     size_t n_nodes;
     float eps2, theta2;
     float *acc;
+    double *acc64;              /* may be NULL */
     uint64_t *visits, *accepts; /* per group */
     uint8_t *ambiguous;         /* per group, may be NULL */
 } trav_ctx;
@@ -370,6 +371,7 @@ static void trav_range(size_t glo, size_t ghi, void *p) {
     float *lx = (float *)malloc(sizeof(float) * W * 6);
     float *ly = lx + W, *ax = lx + 2 * W, *ay = lx + 3 * W, *ddx = lx + 4 * W,
           *ddy = lx + 5 * W;
+    double *bx = (double *)malloc(sizeof(double) * W * 2), *by = bx + W;
     uint8_t *real = (uint8_t *)malloc(W);
     uint32_t *src = (uint32_t *)malloc(sizeof(uint32_t) * W);
     const oracle_tree *t = c->tree;
@@ -382,6 +384,7 @@ static void trav_range(size_t glo, size_t ghi, void *p) {
             lx[l] = real[l] ? c->pos[2 * (size_t)src[l]] : 0.0f;
             ly[l] = real[l] ? c->pos[2 * (size_t)src[l] + 1] : 0.0f;
             ax[l] = 0.0f; ay[l] = 0.0f;
+            bx[l] = 0.0; by[l] = 0.0;
             nreal += real[l];
         }
         uint64_t vis = 0, acc_n = 0;
@@ -414,6 +417,15 @@ static void trav_range(size_t glo, size_t ghi, void *p) {
                     const float s = ((m * r) * r) * r;     /* :134 */
                     ax[l] = ax[l] + dx * s;
                     ay[l] = ay[l] + dy * s;
+                    if (c->acc64) {
+                        /* the same accepted node in double precision */
+                        const double ex = (double)nx - (double)lx[l];
+                        const double ey = (double)ny - (double)ly[l];
+                        const double q = 1.0 / sqrt(ex * ex + ey * ey + (double)c->eps2);
+                        const double w = (double)m * q * q * q;
+                        bx[l] += ex * w;
+                        by[l] += ey * w;
+                    }
                 }
                 acc_n += nreal;
                 ind += t->counts[ind];                     /* :136 */
@@ -425,18 +437,22 @@ static void trav_range(size_t glo, size_t ghi, void *p) {
             if (!real[l]) continue;
             c->acc[2 * (size_t)src[l]] = ax[l];
             c->acc[2 * (size_t)src[l] + 1] = ay[l];
+            if (c->acc64) {
+                c->acc64[2 * (size_t)src[l]] = bx[l];
+                c->acc64[2 * (size_t)src[l] + 1] = by[l];
+            }
         }
         c->visits[g] = vis;
         c->accepts[g] = acc_n;
         if (c->ambiguous) c->ambiguous[g] = amb;
     }
-    free(lx); free(real); free(src);
+    free(lx); free(bx); free(real); free(src);
 }
 
 void oracle_bh_traverse(size_t na, uint32_t W, const float *pos,
                         const float *mass, const oracle_tree *tree,
                         size_t n_nodes, float eps2, float theta2, float *acc,
-                        uint64_t *visits, uint64_t *accepts,
+                        double *acc64, uint64_t *visits, uint64_t *accepts,
                         uint64_t *ambiguous, uint8_t *group_ambiguous,
                         int nthreads) {
     if (W == 0 || na % W != 0) return;
@@ -444,7 +460,7 @@ void oracle_bh_traverse(size_t na, uint32_t W, const float *pos,
     uint64_t *gv = (uint64_t *)calloc(groups ? groups : 1, sizeof(uint64_t));
     uint64_t *ga = (uint64_t *)calloc(groups ? groups : 1, sizeof(uint64_t));
     uint8_t *gamb = (uint8_t *)calloc(groups ? groups : 1, 1);
-    trav_ctx c = {na, W, pos, mass, tree, n_nodes, eps2, theta2, acc, gv, ga, gamb};
+    trav_ctx c = {na, W, pos, mass, tree, n_nodes, eps2, theta2, acc, acc64, gv, ga, gamb};
     if (n_nodes > 0) parallel_for(groups, nthreads, trav_range, &c);
     uint64_t V = 0, A = 0, AM = 0;
     for (size_t g = 0; g < groups; ++g) { V += gv[g]; A += ga[g]; AM += gamb[g]; }
@@ -498,7 +514,7 @@ int oracle_bh_step(size_t na, float *pos_in, float *vel_in, float *mass_in,
             rc = -1;
         } else {
             oracle_bh_traverse(na, W, pos_in, mass_in, &t, (size_t)nn, eps2, theta2,
-                               acc, visits, accepts, NULL, NULL, nthreads);
+                               acc, NULL, visits, accepts, NULL, NULL, nthreads);
             oracle_bh_integrate(na, pos_in, vel_in, mass_in, acc, pos_out, vel_out,
                                 mass_out, G, dt);
         }

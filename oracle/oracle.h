@@ -92,10 +92,13 @@ long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
  * only.  visits/accepts (may be NULL) receive the real-lane node visits and
  * acceptances; ambiguous (may be NULL) counts the groups that had a vote with
  * a real lane within 4 ulp of the threshold, and group_ambiguous (may be
- * NULL, [na/W]) flags them. */
+ * NULL, [na/W]) flags them.  acc64 (may be NULL) receives the same accepted
+ * node set summed in double precision: the error yardstick for a GPU walk
+ * that adds the same terms in another order. */
 void oracle_bh_traverse(size_t na, uint32_t W, const float *pos,
                         const float *mass, const oracle_tree *tree,
                         size_t n_nodes, float eps2, float theta2, float *acc,
+                        double *acc64,
                         uint64_t *visits, uint64_t *accepts,
                         uint64_t *ambiguous, uint8_t *group_ambiguous,
                         int nthreads);

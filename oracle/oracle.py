@@ -47,8 +47,8 @@ def lib() -> C.CDLL:
                                       C.POINTER(C.c_size_t)]
         L.oracle_bh_build.restype = C.c_long
         L.oracle_bh_traverse.argtypes = [C.c_size_t, C.c_uint32, fp, fp, C.c_void_p,
-                                         C.c_size_t, C.c_float, C.c_float, fp, u64, u64, u64,
-                                         C.POINTER(C.c_uint8), C.c_int]
+                                         C.c_size_t, C.c_float, C.c_float, fp, dp, u64, u64,
+                                         u64, C.POINTER(C.c_uint8), C.c_int]
         L.oracle_bh_integrate.argtypes = [C.c_size_t, fp, fp, fp, fp, fp, fp, fp, C.c_float,
                                           C.c_float]
         L.oracle_bh_step.argtypes = [C.c_size_t, fp, fp, fp, fp, fp, fp, C.c_float, C.c_float,
@@ -146,8 +146,9 @@ def bh_build(pos, mass):
 
 
 def bh_traverse(tree: dict, pos, mass, eps2: float, theta2: float, W: int = 64,
-                nthreads: int = 0):
-    """Returns (acc[na,2], visits, accepts, ambiguous_groups, group_flags)."""
+                nthreads: int = 0, f64: bool = False):
+    """Returns (acc[na,2], visits, accepts, ambiguous_groups, group_flags), and
+    with f64=True also the same accepted terms summed in double, acc64[na,2]."""
     pos, mass = _f32(pos), _f32(mass)
     na = mass.shape[0]
     nn = int(tree["total"])
@@ -159,15 +160,18 @@ def bh_traverse(tree: dict, pos, mass, eps2: float, theta2: float, W: int = 64,
     t = _Tree(counts.ctypes.data_as(C.POINTER(C.c_uint32)), _f(radii), _f(npos), _f(nmass),
               src.ctypes.data_as(C.POINTER(C.c_uint32)), max(nn, 1))
     acc = np.zeros((na, 2), np.float32)
+    acc64 = np.zeros((na, 2), np.float64) if f64 else None
     v = C.c_uint64(0)
     a = C.c_uint64(0)
     am = C.c_uint64(0)
     flags = np.zeros(max(na // W, 1), np.uint8)
     lib().oracle_bh_traverse(na, W, _f(pos), _f(mass), C.byref(t), nn, eps2, theta2, _f(acc),
+                             acc64.ctypes.data_as(C.POINTER(C.c_double)) if f64 else None,
                              C.byref(v), C.byref(a), C.byref(am),
                              flags.ctypes.data_as(C.POINTER(C.c_uint8)),
                              nthreads or nthreads_default())
-    return acc, int(v.value), int(a.value), int(am.value), flags
+    out = (acc, int(v.value), int(a.value), int(am.value), flags)
+    return out + (acc64,) if f64 else out
 
 
 def bh_step(pos, vel, mass, G: float, dt: float, eps2: float, theta2: float, W: int = 64,

@@ -276,6 +276,30 @@ def test_bh_theta_infinite_is_pull_of_global_com():
     assert V == A == nin  # one visit per lane: the root
 
 
+def test_bh_traverse_f64_accumulation_is_exact_on_the_same_terms():
+    """The double-precision sum of the walk's accepted terms (the yardstick
+    for GPU walks that add the same terms in another order): at theta = 0 it
+    is the FP64 direct sum; at theta -> inf it is the exact pull of the root
+    node the tree built; in between it stays within FP32 rounding of the FP32
+    walk."""
+    pos, mass = _input(n=512, seed=12)
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass)
+    eps2 = H.sq32(0.2)
+    inside = tree["sorted_src"][tree["sorted_src"] < len(mk)]
+    acc, V, A, _, _, a64 = O.bh_traverse(tree, pk, mk, eps2, 0.0, W=64, f64=True)
+    ds64 = O.ds_accel_f64(pk[inside], mk[inside], eps2)
+    assert np.allclose(a64[inside], ds64, rtol=1e-12, atol=1e-12)
+    acc, V, A, _, _, a64 = O.bh_traverse(tree, pk, mk, eps2, 1e30, W=64, f64=True)
+    root = tree["node_pos"][0].astype(np.float64)
+    d = root[None, :] - pk[inside].astype(np.float64)
+    q = 1.0 / np.sqrt((d ** 2).sum(1) + np.float64(eps2))
+    ref = d * (np.float64(tree["node_mass"][0]) * q ** 3)[:, None]
+    assert np.allclose(a64[inside], ref, rtol=1e-14, atol=0)
+    acc, V, A, _, _, a64 = O.bh_traverse(tree, pk, mk, eps2, 1.0, W=64, f64=True)
+    assert H.system_rel_err(acc[inside], a64[inside]) < 1e-6
+    assert H.per_particle_rel(acc[inside], a64[inside]).max() < 1e-5
+
+
 def test_bh_error_against_f64_shrinks_with_theta():
     pos, mass = _input(n=1024, seed=10, spread=20.0)
     tree, nn, nin, pk, mk = O.bh_build(pos, mass)
