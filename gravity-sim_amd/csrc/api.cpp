@@ -373,6 +373,13 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
                 return fail(GSIM_ERROR_INVALID_ARGUMENT, "traversal mode must be 0..3 or 6");
             s->trav_mode = value;
             return GSIM_OK;
+        case GSIM_TUNE_BH_ADAPTIVE_BOX:
+            if (s->params.algorithm != GSIM_BARNES_HUT)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "not a Barnes-Hut handle");
+            GSIM_HIP(hipSetDevice(s->device));
+            GSIM_HIP(hipStreamSynchronize(s->stream));
+            s->bh_box = value ? 1u : 0u;
+            return GSIM_OK;
         case GSIM_TUNE_BH_SPLIT:
             if (value > 1000) return fail(GSIM_ERROR_INVALID_ARGUMENT, "split threshold must be in [0, 1000]");
             s->trav_split = value;
@@ -445,7 +452,15 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
                 if (u[i] >= gsim::kKilledKey) u[i] = 0xffffffffu;
             return GSIM_OK;
         }
+        case GSIM_DUMP_BOX:
+            *written = 4 * sizeof(float);
+            if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
+            if (!s->bh_box) return fail(GSIM_ERROR_INVALID_ARGUMENT, "the adaptive box is off");
+            GSIM_HIP(hipMemcpy(buf, s->bh.box, *written, hipMemcpyDeviceToHost));
+            return GSIM_OK;
         case GSIM_DUMP_MORTON30: {
+            if (s->bh_box)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "30-bit codes are of the fixed domain only");
             *written = sizeof(uint32_t) * na;
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
             // positions the last step read (its input set)

@@ -33,9 +33,24 @@ __device__ __forceinline__ int shared_depth(uint32_t a, uint32_t b) {
     return 9 - ((31 - __clz(x)) >> 1);
 }
 
-__device__ __forceinline__ float cell_radius(int d) {
-    // TreeMoveShader.comp:70-71: S*S*2 / 4^d, exact in FP32
-    return (kSimSize * kSimSize * 2.0f) / (float)(1u << (2 * d));
+// Radius base of the fixed domain: S*S*2 (TreeMoveShader.comp:70-71).
+constexpr float kFixedR0 = kSimSize * kSimSize * 2.0f;
+
+// r0 / 4^d: S*S*2 / 4^d for the fixed domain, (h*h)*2 / 4^d for the adaptive
+// box; exact in FP32 (division by a power of two).
+__device__ __forceinline__ float cell_radius(int d, float r0) {
+    return r0 / (float)(1u << (2 * d));
+}
+
+// Adaptive box {cx, cy, h, r0}: ((p - c) / h + 1) * 0.5 * 1024, clamped to
+// [0, 1023]; only non-finite positions are outside.
+__device__ __forceinline__ uint32_t cell20_box(float x, float y, float cx, float cy, float h) {
+    if (!__builtin_isfinite(x) || !__builtin_isfinite(y)) return 0xffffffffu;
+    const float qx = ((((x - cx) / h) + 1.0f) * 0.5f) * (float)kTreeSize;
+    const float qy = ((((y - cy) / h) + 1.0f) * 0.5f) * (float)kTreeSize;
+    const uint32_t ix = min(qx > 0.0f ? (uint32_t)qx : 0u, kTreeSize - 1);
+    const uint32_t iy = min(qy > 0.0f ? (uint32_t)qy : 0u, kTreeSize - 1);
+    return part1by1(ix) | (part1by1(iy) << 1);
 }
 
 __device__ __forceinline__ void accumulate(float nx, float ny, float m, float px, float py,
@@ -67,7 +82,8 @@ __device__ __forceinline__ void emit_cell(const TileCell &c, int d,
                                           const int8_t *__restrict__ sdep,
                                           const uint32_t *__restrict__ cpos,
                                           float4 *__restrict__ node4,
-                                          uint32_t *__restrict__ ncount) {
+                                          uint32_t *__restrict__ ncount,
+                                          float r0 = kFixedR0) {
     if (c.y <= 1u) return;
     const uint32_t f = c.f;
     const uint32_t slot = f + cpos[f] + (uint32_t)(d - (int)sdep[f] - 1);
@@ -77,7 +93,7 @@ __device__ __forceinline__ void emit_cell(const TileCell &c, int d,
         py = py / c.sm;
     }
     ncount[slot] = c.y;
-    node4[slot] = make_float4(px, py, c.sm, cell_radius(d));
+    node4[slot] = make_float4(px, py, c.sm, cell_radius(d, r0));
 }
 
 // Inclusive scan of one value across a 256-thread workgroup; returns the

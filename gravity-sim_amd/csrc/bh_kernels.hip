@@ -37,11 +37,91 @@ constexpr int kSortTile = kSortThreads * kSortRounds;
 
 // ---- 1. keys ----------------------------------------------------------------
 
+// ---- 0. adaptive bounding box (opt-in; beyond the reference) -------------------
+// Extremes of the finite positions: per-block partials, then one block that
+// also applies the box formula {cx, cy, h, r0} (order-independent, so exact).
+
+__device__ __forceinline__ void minmax4_wave(float &a, float &b, float &c, float &d) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        a = fminf(a, __shfl_xor(a, off, 64));
+        b = fmaxf(b, __shfl_xor(b, off, 64));
+        c = fminf(c, __shfl_xor(c, off, 64));
+        d = fmaxf(d, __shfl_xor(d, off, 64));
+    }
+}
+
+__global__ __launch_bounds__(256) void bh_box_partial_kernel(const float2 *__restrict__ pos,
+                                                             uint32_t na,
+                                                             float4 *__restrict__ part) {
+    __shared__ float4 wv[4];
+    float x0 = __builtin_inff(), x1 = -__builtin_inff(), y0 = __builtin_inff(),
+          y1 = -__builtin_inff();
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < na; i += gridDim.x * 256) {
+        const float2 p = pos[i];
+        if (!__builtin_isfinite(p.x) || !__builtin_isfinite(p.y)) continue;
+        x0 = fminf(x0, p.x);
+        x1 = fmaxf(x1, p.x);
+        y0 = fminf(y0, p.y);
+        y1 = fmaxf(y1, p.y);
+    }
+    minmax4_wave(x0, x1, y0, y1);
+    if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = make_float4(x0, x1, y0, y1);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float4 r = wv[0];
+        for (int w = 1; w < 4; ++w) {
+            r.x = fminf(r.x, wv[w].x);
+            r.y = fmaxf(r.y, wv[w].y);
+            r.z = fminf(r.z, wv[w].z);
+            r.w = fmaxf(r.w, wv[w].w);
+        }
+        part[blockIdx.x] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restrict__ part,
+                                                           uint32_t nparts,
+                                                           float *__restrict__ box) {
+    __shared__ float4 wv[4];
+    float x0 = __builtin_inff(), x1 = -__builtin_inff(), y0 = __builtin_inff(),
+          y1 = -__builtin_inff();
+    for (uint32_t i = threadIdx.x; i < nparts; i += 256) {
+        const float4 r = part[i];
+        x0 = fminf(x0, r.x);
+        x1 = fmaxf(x1, r.y);
+        y0 = fminf(y0, r.z);
+        y1 = fmaxf(y1, r.w);
+    }
+    minmax4_wave(x0, x1, y0, y1);
+    if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = make_float4(x0, x1, y0, y1);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int w = 1; w < 4; ++w) {
+        wv[0].x = fminf(wv[0].x, wv[w].x);
+        wv[0].y = fmaxf(wv[0].y, wv[w].y);
+        wv[0].z = fminf(wv[0].z, wv[w].z);
+        wv[0].w = fmaxf(wv[0].w, wv[w].w);
+    }
+    float xmin = wv[0].x, xmax = wv[0].y, ymin = wv[0].z, ymax = wv[0].w;
+    if (!(xmin <= xmax)) xmin = xmax = ymin = ymax = 0.0f;  // no finite particle
+    const float cx = (xmin + xmax) * 0.5f, cy = (ymin + ymax) * 0.5f;
+    const float w = fmaxf(xmax - xmin, ymax - ymin);
+    float h = w * 0.501953125f;  // half the extent plus 2^-9: the farthest maps below 1024
+    if (!(h >= 1e-3f)) h = 1e-3f;
+    box[0] = cx;
+    box[1] = cy;
+    box[2] = h;
+    box[3] = (h * h) * 2.0f;
+}
+
+template <bool kBox>
 __global__ __launch_bounds__(256) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
-    uint32_t *__restrict__ digit_total, BhCounters *__restrict__ ctr) {
+    uint32_t *__restrict__ digit_total, BhCounters *__restrict__ ctr,
+    const float *__restrict__ box) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < 2 * kTiles) tile_range[i] = 0;
     if (i < 3 * 256) digit_total[i] = 0;
@@ -52,7 +132,7 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     }
     if (i >= na) return;
     const float2 p = pos[i];
-    uint32_t c = cell20(p.x, p.y);
+    uint32_t c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
     if (c == 0xffffffffu) {
         // InitShader.comp:110-117; the reference leaves vel_out stale, here it
         // keeps the input velocity.
@@ -300,6 +380,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
 // ---- 5. leaf sums and particle runs ------------------------------------------
 
 constexpr uint32_t kLeafBlocks = 1024;
+constexpr uint32_t kBoxParts = 256;  // blocks of the box reduction
 
 // Phase A, one wave per leaf: the sum runs over the leaf's particles in
 // sorted (= ascending index) order, the canonical order of the reference's
@@ -378,7 +459,8 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
     const uint32_t *__restrict__ tile_range, const uint32_t *__restrict__ leaf_first,
     const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
     const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
-    float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top) {
+    float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top,
+    const float *__restrict__ box) {
     __shared__ TileCell l10[1024];
     __shared__ TileCell up[341];
     const uint32_t T = blockIdx.x, t = threadIdx.x;
@@ -390,15 +472,14 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
     for (uint32_t c = t; c < 1024; c += 256)
         l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
     __syncthreads();
-    const float r10 = cell_radius(10);
+    const float r0 = box ? box[3] : kFixedR0;
     for (uint32_t L = lb + t; L < le; L += 256) {
         const uint32_t c = leaf_key[L] & 1023u;
         const float4 s = leaf_sum[L];
         const uint32_t n = __float_as_uint(s.w);
         TileCell cell{n, n + (n >= 2u ? 1u : 0u), leaf_first[L], 0u, s.x, s.y, s.z, 0.0f};
         l10[c] = cell;
-        (void)r10;
-        emit_cell(cell, 10, sdep, cpos, node4, ncount);
+        emit_cell(cell, 10, sdep, cpos, node4, ncount, r0);
     }
     __syncthreads();
     {
@@ -416,7 +497,7 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
     }
     for (uint32_t c = t; c < 341; c += 256) {
         const int d = c < 256 ? 9 : c < 320 ? 8 : c < 336 ? 7 : c < 340 ? 6 : 5;
-        emit_cell(up[c], d, sdep, cpos, node4, ncount);
+        emit_cell(up[c], d, sdep, cpos, node4, ncount, r0);
     }
     if (t == 0) top[T] = up[340];
 }
@@ -431,8 +512,10 @@ __global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict
                                                       const int8_t *__restrict__ sdep,
                                                       const uint32_t *__restrict__ cpos,
                                                       float4 *__restrict__ node4,
-                                                      uint32_t *__restrict__ ncount) {
+                                                      uint32_t *__restrict__ ncount,
+                                                      const float *__restrict__ box) {
     __shared__ TileCell lv[1365];
+    const float r0 = box ? box[3] : kFixedR0;
     const uint32_t t = threadIdx.x;
     lv[t] = top[t];
     __syncthreads();
@@ -447,7 +530,7 @@ __global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict
     if (t < 341) {
         const uint32_t c = 1024 + t;
         const int d = c < 1280 ? 4 : c < 1344 ? 3 : c < 1360 ? 2 : c < 1364 ? 1 : 0;
-        emit_cell(lv[c], d, sdep, cpos, node4, ncount);
+        emit_cell(lv[c], d, sdep, cpos, node4, ncount, r0);
     }
 }
 
@@ -835,6 +918,8 @@ hipError_t bh_alloc(BhBuffers *b, size_t na) {
     GSIM_ALLOC(b->ncount, b->cap);
     GSIM_ALLOC(b->sorted_src, na);
     GSIM_ALLOC(b->ctr, 1);
+    GSIM_ALLOC(b->box, 4);
+    GSIM_ALLOC(b->box_part, kBoxParts);
     GSIM_ALLOC(b->gcost, na / 64);
     GSIM_ALLOC(b->gpre, na / 64 + 1);
 #undef GSIM_ALLOC
@@ -852,7 +937,7 @@ void bh_free(BhBuffers *b) {
     }
     void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->prun, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
-                    b->node4, b->ncount, b->sorted_src, b->ctr, b->gcost, b->gpre};
+                    b->node4, b->ncount, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box, b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
 }
@@ -863,9 +948,20 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     const uint32_t na = (uint32_t)s->na;
     const uint32_t g256 = (na + 255) / 256;
     const uint32_t gkeys = g256 > (2 * kTiles + 255) / 256 ? g256 : (2 * kTiles + 255) / 256;
-    bh_keys_kernel<<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
-                                                 b.keys[0], b.idx[0], b.tile_range,
-                                                 b.digit_total, b.ctr);
+    const float *box = nullptr;
+    if (s->bh_box) {
+        const uint32_t nparts = std::min<uint32_t>(g256, kBoxParts);
+        bh_box_partial_kernel<<<nparts, 256, 0, s->stream>>>(pos, na, b.box_part);
+        bh_box_final_kernel<<<1, 256, 0, s->stream>>>(b.box_part, nparts, b.box);
+        box = b.box;
+        bh_keys_kernel<true><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
+                                                           b.keys[0], b.idx[0], b.tile_range,
+                                                           b.digit_total, b.ctr, box);
+    } else {
+        bh_keys_kernel<false><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
+                                                            b.keys[0], b.idx[0], b.tile_range,
+                                                            b.digit_total, b.ctr, nullptr);
+    }
     // keys[0] -> keys[1] -> keys[2] -> keys[1]; keys[0] keeps the per-particle cells
     const int in_of[3] = {0, 1, 2}, out_of[3] = {1, 2, 1};
     const uint32_t nblk = (uint32_t)b.sort_blocks;
@@ -893,8 +989,8 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
                                               b.breaker, b.ctr, b.leaf_sum, b.tile_range, b.prun);
     bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
                                                   b.leaf_sum, b.sdep, b.cpos, b.node4,
-                                                  b.ncount, b.top);
-    bh_top_kernel<<<1, 1024, 0, s->stream>>>(b.top, b.sdep, b.cpos, b.node4, b.ncount);
+                                                  b.ncount, b.top, box);
+    bh_top_kernel<<<1, 1024, 0, s->stream>>>(b.top, b.sdep, b.cpos, b.node4, b.ncount, box);
     return hipGetLastError();
 }
 

@@ -73,6 +73,9 @@ struct BhBuffers {
     // the exclusive prefix of (cost + 1) that cuts the groups into segments
     uint32_t *gcost = nullptr;       // [Na/64], zero before the first walk
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
+    // adaptive bounding box (opt-in): {cx, cy, h, r0} and per-block extremes
+    float *box = nullptr;            // [4]
+    float4 *box_part = nullptr;      // [256] (xmin, xmax, ymin, ymax)
 };
 
 // Events recorded around one step: e[0] start, e[1] between the two phases,
@@ -126,6 +129,7 @@ struct gsim_sim {
                                        // bit 2 cost-balanced segments (staged)
     uint32_t trav_segments = 512;      // balanced traversal: workgroups (2 per CU)
     uint32_t trav_split = 5;           // balanced traversal: split groups above this x mean cost
+    uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
     uint32_t ds_rcp = 0;               // direct sum: 1/r2 from the transcendental unit
     // state
     float2 *pos[2] = {}, *vel[2] = {};

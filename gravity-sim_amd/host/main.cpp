@@ -9,7 +9,9 @@
 //   --no-graphics without --simulation-count is rejected (the reference tests
 //   simulationCount, which is always 0, so it loops forever).
 // Additions: --seed= (generation seed; the reference seeds from the clock),
-// --group-width= (Barnes-Hut vote width W, 64 or 32), --device=.
+// --group-width= (Barnes-Hut vote width W, 64 or 32), --device=, and
+// --adaptive-box (Barnes-Hut tree over the particles' bounding square instead
+// of the fixed [-500, 500) domain; beyond the reference, off by default).
 #include <chrono>
 #include <cstdarg>
 #include <cstdint>
@@ -45,6 +47,8 @@ const char *const kHelp =
     "\t--seed: The seed of the particle generation. Defaulted to the current time.\n"
     "\t--group-width: The Barnes-Hut traversal group width, 64 or 32. Defaulted to 64.\n"
     "\t--device: The GPU to run on. Defaulted to 0.\n"
+    "\t--adaptive-box: Barnes-Hut only: build the tree over the particles' bounding square\n"
+    "\t\tinstead of the fixed [-500, 500) domain (no particle leaves it). Off by default.\n"
     "Available options:\n"
     "\t--help: Displays the current message and exits the program.\n"
     "\t--log-detailed: Outputs non-crucial logs that might be useful for debugging or additional information.\n"
@@ -116,7 +120,7 @@ struct ProgramInfo {
     uint64_t seed = (uint64_t)std::time(nullptr);
     uint32_t groupWidth = 64;
     int32_t device = 0;
-    bool logDetailed = false, noGraphics = false, benchmark = false;
+    bool logDetailed = false, noGraphics = false, benchmark = false, adaptiveBox = false;
 };
 
 bool starts(const char *arg, const char *prefix, const char **value) {
@@ -164,6 +168,7 @@ int main(int argc, char **argv) {
         else if (!std::strcmp(a, "--log-detailed")) pi.logDetailed = true;
         else if (!std::strcmp(a, "--no-graphics")) pi.noGraphics = true;
         else if (!std::strcmp(a, "--benchmark")) pi.benchmark = true;
+        else if (!std::strcmp(a, "--adaptive-box")) pi.adaptiveBox = true;
     }
 
     Logger logger(pi.logFile, pi.logDetailed ? (DEBUG | INFO | WARNING | ERROR | FATAL)
@@ -206,7 +211,11 @@ int main(int argc, char **argv) {
         if (algo == gsim::ParticleSystem::SIMULATION_ALGORITHM_DIRECT_SUM)
             ds.reset(new gsim::DirectSimulation(ps.get(), pi.groupWidth, pi.device));
         else
+        {
             bh.reset(new gsim::BarnesHutSimulation(ps.get(), pi.groupWidth, pi.device));
+            if (pi.adaptiveBox)
+                GSIM_CHECK(gsim_set_tuning(bh->GetHandle(), GSIM_TUNE_BH_ADAPTIVE_BOX, 1));
+        }
         logger.Log(INFO, "Device library: %s", gsim_version());
 
         const auto start = std::chrono::steady_clock::now();

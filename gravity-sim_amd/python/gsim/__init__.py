@@ -44,10 +44,11 @@ ALGORITHMS = {"direct-sum": DIRECT_SUM, "barnes-hut": BARNES_HUT}
 
 DUMP_CELLS, DUMP_MORTON30, DUMP_SORTED_SRC = 1, 2, 3
 DUMP_COUNTS, DUMP_RADII, DUMP_NODE_POS, DUMP_NODE_MASS = 4, 5, 6, 7
-DUMP_NODE_TOTAL, DUMP_ACCEL, DUMP_COUNTERS = 8, 9, 10
+DUMP_NODE_TOTAL, DUMP_ACCEL, DUMP_COUNTERS, DUMP_BOX = 8, 9, 10, 11
 DEBUG_ACCEL = 1
 TUNE_DS_RCP, TUNE_BH_WAVES_PER_SIMD, TUNE_BH_TRAVERSAL = 1, 2, 3
 TUNE_DS_SYMMETRIC, TUNE_DS_SYM_COLS, TUNE_BH_SEGMENTS, TUNE_BH_SPLIT = 4, 5, 6, 7
+TUNE_BH_ADAPTIVE_BOX = 8
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
           4: "NO_DEVICE", 5: "IO", 6: "CAPACITY"}
@@ -329,7 +330,7 @@ class Simulation:
         nbytes = written.value
         dtype = {DUMP_RADII: np.float32, DUMP_NODE_POS: np.float32,
                  DUMP_NODE_MASS: np.float32, DUMP_ACCEL: np.float32,
-                 DUMP_COUNTERS: np.uint64}.get(what, np.uint32)
+                 DUMP_COUNTERS: np.uint64, DUMP_BOX: np.float32}.get(what, np.uint32)
         buf = np.zeros(max(nbytes, 1) // np.dtype(dtype).itemsize, dtype)
         _check(_lib.gsim_debug_dump(self._h, what, buf.ctypes.data_as(C.c_void_p),
                                     buf.nbytes, C.byref(written)))

@@ -93,8 +93,10 @@ typedef enum gsim_dump {
     GSIM_DUMP_NODE_TOTAL = 8,  /* u32[1]   node count of the last tree (counts[0])      */
     GSIM_DUMP_ACCEL = 9,       /* f32[2*Na] accelerations of the last step, by particle
                                   index (needs gsim_set_debug(sim, GSIM_DEBUG_ACCEL))  */
-    GSIM_DUMP_COUNTERS = 10    /* u64[2]   real-lane node visits and acceptances of
+    GSIM_DUMP_COUNTERS = 10,   /* u64[2]   real-lane node visits and acceptances of
                                   the last Barnes-Hut step                              */
+    GSIM_DUMP_BOX = 11         /* f32[4]   adaptive box {cx, cy, h, r0} of the last
+                                  step (GSIM_TUNE_BH_ADAPTIVE_BOX only)                 */
 } gsim_dump;
 
 #define GSIM_DEBUG_ACCEL 1u
@@ -157,7 +159,8 @@ gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
                               uint32_t slices);
 
 /* Launch tuning that never changes results beyond the documented tolerance
- * (visit/accept counts and trees are unaffected by every knob). */
+ * (visit/accept counts and trees are unaffected by every knob), and one
+ * opt-in option beyond the reference, GSIM_TUNE_BH_ADAPTIVE_BOX. */
 typedef enum gsim_tuning {
     GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
     GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
@@ -170,9 +173,16 @@ typedef enum gsim_tuning {
                                          wave, 0 = by size                              */
     GSIM_TUNE_BH_SEGMENTS = 6,        /* balanced traversal: workgroups of 16 waves
                                          (default 512, two per CU)                      */
-    GSIM_TUNE_BH_SPLIT = 7            /* balanced traversal: groups whose last walk
+    GSIM_TUNE_BH_SPLIT = 7,           /* balanced traversal: groups whose last walk
                                          cost exceeds this multiple of the mean are
                                          split across 4 waves (0 = never; default 5)    */
+    GSIM_TUNE_BH_ADAPTIVE_BOX = 8     /* 0/1, beyond the reference and off by default:
+                                         each step's tree spans the bounding square of
+                                         the particles (centre c, half-size h = extent
+                                         x (0.5 + 2^-9)) instead of [-500, 500), so
+                                         only non-finite positions are killed; cell
+                                         radius (h*h)*2 / 4^d.  Changes the topology,
+                                         so results differ from the reference's.        */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 

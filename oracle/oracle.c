@@ -221,13 +221,55 @@ uint32_t oracle_cell20(float x, float y) {
     return 0xffffffffu;
 }
 
+/* ---- adaptive bounding box (beyond the reference; opt-in) ------------------- */
+
+/* The square that holds every particle with finite coordinates: centre at the
+ * midpoint of the extremes, half-size h = max extent x (0.5 + 2^-9) (so the
+ * farthest particle maps below cell 1024), at least 1e-3; r0 = (h*h)*2 plays
+ * the role of S*S*2 in the cell radius r0 / 4^d.  FP32, operations as
+ * written.  No finite particle: the box is (0, 0, 1e-3, 2e-6). */
+void oracle_bh_box(size_t na, const float *pos, float box[4]) {
+    float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (size_t i = 0; i < na; ++i) {
+        const float x = pos[2 * i], y = pos[2 * i + 1];
+        if (!isfinite(x) || !isfinite(y)) continue;
+        xmin = fminf(xmin, x); xmax = fmaxf(xmax, x);
+        ymin = fminf(ymin, y); ymax = fmaxf(ymax, y);
+    }
+    if (!(xmin <= xmax)) { xmin = xmax = ymin = ymax = 0.0f; }
+    const float cx = (xmin + xmax) * 0.5f, cy = (ymin + ymax) * 0.5f;
+    const float w = fmaxf(xmax - xmin, ymax - ymin);
+    float h = w * 0.501953125f;
+    if (!(h >= 1e-3f)) h = 1e-3f;
+    box[0] = cx; box[1] = cy; box[2] = h; box[3] = (h * h) * 2.0f;
+}
+
+/* Cell of a particle in the box: ((p - c) / h + 1) * 0.5 * 1024, clamped to
+ * [0, 1023]; only non-finite positions are outside. */
+uint32_t oracle_cell20_box(float x, float y, const float box[4]) {
+    if (!isfinite(x) || !isfinite(y)) return 0xffffffffu;
+    float qx = (((x - box[0]) / box[2]) + 1.0f) * 0.5f;
+    float qy = (((y - box[1]) / box[2]) + 1.0f) * 0.5f;
+    qx = qx * (float)ORACLE_TREE_SIZE;
+    qy = qy * (float)ORACLE_TREE_SIZE;
+    uint32_t ix = qx > 0.0f ? (uint32_t)qx : 0u, iy = qy > 0.0f ? (uint32_t)qy : 0u;
+    if (ix > ORACLE_TREE_SIZE - 1) ix = ORACLE_TREE_SIZE - 1;
+    if (iy > ORACLE_TREE_SIZE - 1) iy = ORACLE_TREE_SIZE - 1;
+    return part1by1(ix) | (part1by1(iy) << 1);
+}
+
+static uint32_t cell_of(float x, float y, const float *box) {
+    return box ? oracle_cell20_box(x, y, box) : oracle_cell20(x, y);
+}
+
 size_t oracle_bh_capacity(size_t na) { return na + ORACLE_PYRAMID_CELLS; }
 
 /* Offsets of level d inside one dense pyramid array: (4^d - 1)/3. */
 static size_t level_offset(unsigned d) { return ((size_t)1 << (2 * d)) / 3; }
 
-long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
-                     float *mass_out, oracle_tree *tree, size_t *n_inside) {
+static long build_impl(size_t na, float *pos, float *mass, float *pos_out,
+                       float *mass_out, const float *box, oracle_tree *tree,
+                       size_t *n_inside) {
     const size_t cells = ORACLE_PYRAMID_CELLS;
     /* dense per-level arrays (BarnesHutSimulation.cpp:143-234) */
     uint32_t *cn = (uint32_t *)calloc(cells, sizeof(uint32_t));
@@ -243,6 +285,13 @@ long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
     }
     const size_t L10 = level_offset(10);
     size_t inside = 0;
+    /* each particle's cell from Init, reused by ParticleSort (a killed
+     * particle's rewritten position must not be mapped again) */
+    uint32_t *cell = (uint32_t *)malloc(sizeof(uint32_t) * (na ? na : 1));
+    if (!cell) {
+        free(cn); free(cy); free(stx); free(sty); free(sx); free(sy); free(sm);
+        return -1;
+    }
 
     /* ClearShader.comp:81-82: every sorted slot starts as the sentinel. */
     for (size_t i = 0; i < na; ++i) tree->sorted_src[i] = (uint32_t)na;
@@ -251,7 +300,8 @@ long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
      * the float atomics). */
     for (size_t i = 0; i < na; ++i) {
         const float x = pos[2 * i], y = pos[2 * i + 1], m = mass[i];
-        const uint32_t c = oracle_cell20(x, y);
+        const uint32_t c = cell_of(x, y, box);
+        cell[i] = c;
         if (c != 0xffffffffu) {
             const size_t k = L10 + c;
             cn[k] += 1u;
@@ -308,8 +358,8 @@ long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
     for (unsigned d = 0; d <= 10; ++d) {
         const size_t Ld = level_offset(d);
         const size_t nc = (size_t)1 << (2 * d);
-        const float radius = (ORACLE_SIM_SIZE * ORACLE_SIM_SIZE * 2.0f) /
-                             (float)(1u << (2 * d));               /* :70-71 */
+        const float r0 = box ? box[3] : ORACLE_SIM_SIZE * ORACLE_SIM_SIZE * 2.0f;
+        const float radius = r0 / (float)(1u << (2 * d));          /* :70-71 */
         for (size_t c = 0; c < nc; ++c) {
             const size_t k = Ld + c;
             if (cy[k] > 1u) {
@@ -331,7 +381,7 @@ long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
     /* ParticleSortShader.comp:71-110, ascending index (canonical). */
     for (size_t i = 0; i < na && status == 0; ++i) {
         const float x = pos[2 * i], y = pos[2 * i + 1];
-        const uint32_t c = oracle_cell20(x, y);
+        const uint32_t c = cell[i];
         if (c == 0xffffffffu) continue;
         const size_t k = L10 + c;
         const uint32_t dsrc = stx[k]++;
@@ -345,9 +395,20 @@ long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
         tree->node_mass[slot] = mass[i];
     }
 
-    free(cn); free(cy); free(stx); free(sty); free(sx); free(sy); free(sm);
+    free(cn); free(cy); free(stx); free(sty); free(sx); free(sy); free(sm); free(cell);
     if (status != 0) return -1;
     return inside ? (long)tree->counts[0] : 0;
+}
+
+long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
+                     float *mass_out, oracle_tree *tree, size_t *n_inside) {
+    return build_impl(na, pos, mass, pos_out, mass_out, NULL, tree, n_inside);
+}
+
+long oracle_bh_build_box(size_t na, float *pos, float *mass, float *pos_out,
+                         float *mass_out, const float box[4], oracle_tree *tree,
+                         size_t *n_inside) {
+    return build_impl(na, pos, mass, pos_out, mass_out, box, tree, n_inside);
 }
 
 /* ---- Barnes-Hut: traversal ------------------------------------------------- */

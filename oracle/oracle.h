@@ -86,6 +86,17 @@ typedef struct {
 long oracle_bh_build(size_t na, float *pos, float *mass, float *pos_out,
                      float *mass_out, oracle_tree *tree, size_t *n_inside);
 
+/* Adaptive bounding box (beyond the reference, opt-in): box = {cx, cy, h, r0}
+ * from the extremes of the finite positions (see oracle.c), the cell of a
+ * particle in it, and the build of oracle_bh_build over that box instead of
+ * the fixed [-500, 500) domain: only non-finite positions are killed, and the
+ * cell radius is r0 / 4^d. */
+void oracle_bh_box(size_t na, const float *pos, float box[4]);
+uint32_t oracle_cell20_box(float x, float y, const float box[4]);
+long oracle_bh_build_box(size_t na, float *pos, float *mass, float *pos_out,
+                         float *mass_out, const float box[4], oracle_tree *tree,
+                         size_t *n_inside);
+
 /* Group traversal (ForceShader.comp:78-141): groups of W consecutive sorted
  * slots walk one path; a node is accepted iff every non-sentinel lane has
  * radius <= (d.d + eps2)*theta2.  Writes acc[2*src] for non-sentinel lanes

@@ -43,6 +43,12 @@ def lib() -> C.CDLL:
         L.oracle_cell20.restype = C.c_uint32
         L.oracle_bh_capacity.argtypes = [C.c_size_t]
         L.oracle_bh_capacity.restype = C.c_size_t
+        L.oracle_bh_box.argtypes = [C.c_size_t, fp, fp]
+        L.oracle_cell20_box.argtypes = [C.c_float, C.c_float, fp]
+        L.oracle_cell20_box.restype = C.c_uint32
+        L.oracle_bh_build_box.argtypes = [C.c_size_t, fp, fp, fp, fp, fp, C.c_void_p,
+                                          C.POINTER(C.c_size_t)]
+        L.oracle_bh_build_box.restype = C.c_long
         L.oracle_bh_build.argtypes = [C.c_size_t, fp, fp, fp, fp, C.c_void_p,
                                       C.POINTER(C.c_size_t)]
         L.oracle_bh_build.restype = C.c_long
@@ -123,8 +129,23 @@ def cell20(x: float, y: float) -> int:
     return int(lib().oracle_cell20(x, y))
 
 
-def bh_build(pos, mass):
-    """Returns (tree dict, n_nodes, n_inside, pos_after_kill, mass_after_kill)."""
+def bh_box(pos) -> np.ndarray:
+    """Adaptive bounding box {cx, cy, h, r0} of the finite positions."""
+    pos = _f32(pos)
+    box = np.zeros(4, np.float32)
+    lib().oracle_bh_box(pos.shape[0], _f(pos), _f(box))
+    return box
+
+
+def cell20_box(x: float, y: float, box) -> int:
+    box = _f32(box)
+    return int(lib().oracle_cell20_box(x, y, _f(box)))
+
+
+def bh_build(pos, mass, box=None):
+    """Returns (tree dict, n_nodes, n_inside, pos_after_kill, mass_after_kill);
+    with box ({cx, cy, h, r0}) the build is over that box instead of the fixed
+    [-500, 500) domain."""
     pos, mass = _f32(pos).copy(), _f32(mass).copy()
     na = mass.shape[0]
     cap = int(lib().oracle_bh_capacity(na))
@@ -136,7 +157,12 @@ def bh_build(pos, mass):
     t = _Tree(counts.ctypes.data_as(C.POINTER(C.c_uint32)), _f(radii), _f(npos), _f(nmass),
               src.ctypes.data_as(C.POINTER(C.c_uint32)), cap)
     inside = C.c_size_t(0)
-    nn = lib().oracle_bh_build(na, _f(pos), _f(mass), None, None, C.byref(t), C.byref(inside))
+    if box is None:
+        nn = lib().oracle_bh_build(na, _f(pos), _f(mass), None, None, C.byref(t), C.byref(inside))
+    else:
+        bx = _f32(box)
+        nn = lib().oracle_bh_build_box(na, _f(pos), _f(mass), None, None, _f(bx), C.byref(t),
+                                       C.byref(inside))
     if nn < 0:
         raise RuntimeError("oracle_bh_build: capacity exceeded")
     tree = {"counts": counts[:nn].copy(), "radii": radii[:nn].copy(),

@@ -52,18 +52,45 @@ class _Cell:
         self.children = []
 
 
-def brute_tree(pos: np.ndarray, mass: np.ndarray):
+def box_numpy(pos: np.ndarray) -> np.ndarray:
+    """The adaptive box {cx, cy, h, r0} restated in numpy FP32: centre of the
+    finite extremes, half-size = max extent x (0.5 + 2^-9), at least 1e-3."""
+    p = pos.astype(np.float32)
+    fin = np.isfinite(p).all(axis=1)
+    if not fin.any():
+        lo = hi = np.zeros(2, np.float32)
+    else:
+        lo, hi = p[fin].min(axis=0), p[fin].max(axis=0)
+    c = ((lo + hi) * np.float32(0.5)).astype(np.float32)
+    w = np.float32(max(np.float32(hi[0] - lo[0]), np.float32(hi[1] - lo[1])))
+    h = np.float32(w * np.float32(0.501953125))
+    if not h >= np.float32(1e-3):
+        h = np.float32(1e-3)
+    return np.array([c[0], c[1], h, np.float32(np.float32(h * h) * np.float32(2.0))], np.float32)
+
+
+def brute_tree(pos: np.ndarray, mass: np.ndarray, box=None):
     """Builds the reference topology by recursive subdivision of the domain
     (explicit geometry, no Morton codes): quadrant by comparing the normalised
     coordinate against the cell's midpoint.  Returns node arrays in pre-order
     and the sorted source list, with FP32 sums in the reference association."""
     na = len(mass)
-    cells = cells_numpy(pos)
-    inside = [i for i in range(na) if cells[i] != 0xFFFFFFFF]
-    # integer leaf coordinates from the same normalisation
     p = pos.astype(np.float32)
-    q = (((p / SIM_SIZE) + np.float32(1.0)) * np.float32(0.5)) * np.float32(TREE)
-    ij = {i: (int(q[i, 0]), int(q[i, 1])) for i in inside}
+    if box is None:
+        cells = cells_numpy(pos)
+        inside = [i for i in range(na) if cells[i] != 0xFFFFFFFF]
+        # integer leaf coordinates from the same normalisation
+        q = (((p / SIM_SIZE) + np.float32(1.0)) * np.float32(0.5)) * np.float32(TREE)
+        ij = {i: (int(q[i, 0]), int(q[i, 1])) for i in inside}
+        r0 = np.float32(500000.0)
+    else:
+        b = np.asarray(box, np.float32)
+        inside = [i for i in range(na) if np.isfinite(p[i]).all()]
+        with np.errstate(invalid="ignore"):
+            q = ((((p - b[:2]) / b[2]) + np.float32(1.0)) * np.float32(0.5)) * np.float32(TREE)
+        ij = {i: (min(int(max(q[i, 0], 0)), TREE - 1), min(int(max(q[i, 1], 0)), TREE - 1))
+              for i in inside}
+        r0 = b[3]
 
     def split(members, depth, x0, y0):
         half = TREE >> (depth + 1)
@@ -112,7 +139,7 @@ def brute_tree(pos: np.ndarray, mass: np.ndarray):
         sx, sy, sm = sums(members, depth, x0, y0)
         cx, cy = (sx / sm, sy / sm) if sm != 0 else (sx, sy)
         counts.append(0)
-        radii.append(np.float32(np.float32(500000.0) / np.float32(4 ** depth)))
+        radii.append(np.float32(np.float32(r0) / np.float32(4 ** depth)))
         npos.append((np.float32(cx), np.float32(cy))); nmass.append(sm)
         size = 1
         if depth == 10:

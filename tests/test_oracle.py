@@ -237,6 +237,44 @@ def _input(n=640, seed=7, spread=30.0):
     return pos, mass
 
 
+# ---- adaptive bounding box (opt-in, beyond the reference) ---------------------
+
+def test_bh_box_formula_and_cells():
+    rng = np.random.default_rng(21)
+    pos = rng.normal(0, 300, size=(3000, 2)).astype(np.float32)
+    pos[:5] = [[2500, -40], [-1800, 900], [np.nan, 0], [0, np.inf], [700, 700]]
+    box = O.bh_box(pos)
+    assert np.array_equal(box, H.box_numpy(pos))
+    fin = np.isfinite(pos).all(1)
+    lo, hi = pos[fin].min(0), pos[fin].max(0)
+    assert box[2] >= 0.5 * max(hi - lo)  # the square holds every finite particle
+    cells = [O.cell20_box(float(x), float(y), box) for x, y in pos]
+    assert cells[2] == cells[3] == 0xFFFFFFFF          # non-finite: outside
+    assert all(c != 0xFFFFFFFF for c, f in zip(cells, fin) if f)
+    # the extreme particles map strictly inside the 1024 x 1024 grid, without the clamp
+    q = ((((pos[fin] - box[:2]) / box[2]) + np.float32(1)) * np.float32(0.5)) * np.float32(1024)
+    assert q.min() >= 0 and q.max() < 1023.99
+    # degenerate inputs
+    assert np.array_equal(O.bh_box(np.zeros((64, 2), np.float32)), H.box_numpy(np.zeros((64, 2))))
+    assert O.bh_box(np.zeros((64, 2), np.float32))[2] == np.float32(1e-3)
+
+
+@pytest.mark.parametrize("seed", [22, 23])
+def test_bh_build_box_matches_recursive_quadtree(seed):
+    """Over the adaptive box the dense build equals the recursive quadtree of
+    the same box (explicit geometry, radii r0/4^d); particles far beyond the
+    reference's +-500 stay in, non-finite ones are killed."""
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(0, 400, size=(1500, 2)).astype(np.float32)
+    pos[:3] = [[1900, 10], [-1200, -1500], [np.nan, 4]]
+    mass = rng.uniform(1, 10, 1500).astype(np.float32)
+    box = O.bh_box(pos)
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass, box=box)
+    assert nin == 1499 and mk[2] == 0 and mk[0] > 0 and mk[1] > 0
+    _compare_tree(tree, H.brute_tree(pos, mass, box=box))
+    assert tree["radii"][0] == box[3]
+
+
 @pytest.mark.parametrize("W", [1, 32, 64])
 @pytest.mark.parametrize("theta", [0.5, 1.0])
 def test_bh_traverse_matches_recursive_group_vote(W, theta):
