@@ -40,6 +40,9 @@ DS_SYM_ISSUED_PER_PAIR = 13.3125
 # x 2 + WRITE_SIZE x 1 (factors from a 256 MiB copy), 2 x 2,417 + 48,950 KiB;
 # profiles/ds50k_bh500k_counters.md
 DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
+# HBM bytes per bh_traverse_bal_kernel launch at 500,032 particles (same
+# counters): 2 x 67,720 + 34,130 KiB
+BH_TRAFFIC_BYTES = (2 * 67720 + 34130) * 1024
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
@@ -140,12 +143,12 @@ def bench_barnes_hut(steps: int, warmup: int) -> dict:
                    "theta": 1.0, "group_width": 64},
         "roofline": {
             "bound": "valu",
-            "kernel": "bh_traverse64_kernel",
+            "kernel": "bh_traverse_bal_kernel",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": BH_TRAFFIC_BYTES,
             "valu_lane_op_frac": lane_ops / (trav_ms * 1e-3) / VALU_LANE_OPS,
             "step_frac": t_star / (ms_step * 1e-3),
         },
