@@ -14,6 +14,10 @@
  *   oracle_ds_step_f64     pinned: two-body orbit energy and momentum drift.
  *   oracle_bh_build        pinned: recursive pointer-quadtree brute force on
  *                          small N, y = n + #cells invariants, dyadic COM.
+ *   oracle_bh_build_sparse pinned: bit-identical to oracle_bh_build /
+ *                          oracle_bh_build_box at 10 bits; recursive
+ *                          pointer-quadtree brute force at 15 bits; the
+ *                          15-bit code >> 10 is the 20-bit cell.
  *   oracle_bh_traverse     pinned: recursive group-vote brute force for
  *                          W in {1,32,64}, theta = 0 == sorted direct sum,
  *                          theta -> inf == pull of the global COM.
@@ -409,6 +413,165 @@ long oracle_bh_build_box(size_t na, float *pos, float *mass, float *pos_out,
                          float *mass_out, const float box[4], oracle_tree *tree,
                          size_t *n_inside) {
     return build_impl(na, pos, mass, pos_out, mass_out, box, tree, n_inside);
+}
+
+/* ---- Barnes-Hut: sparse build at 10 or 15 bits per axis (opt-in) ----------- */
+
+/* Cell of a position on a 2^bits grid per axis, from the same normalised
+ * coordinate as oracle_cell20 / oracle_cell20_box scaled by 2^bits instead of
+ * 1024.  Scaling by a power of two is exact and truncation commutes with it,
+ * so the 15-bit code shifted right by 10 is the 20-bit cell. */
+uint32_t oracle_cell_bits(float x, float y, const float *box, unsigned bits) {
+    const uint32_t T = 1u << bits;
+    float qx, qy;
+    if (box) {
+        if (!isfinite(x) || !isfinite(y)) return 0xffffffffu;
+        qx = (((x - box[0]) / box[2]) + 1.0f) * 0.5f;
+        qy = (((y - box[1]) / box[2]) + 1.0f) * 0.5f;
+        qx = qx * (float)T;
+        qy = qy * (float)T;
+        uint32_t ix = qx > 0.0f ? (uint32_t)qx : 0u, iy = qy > 0.0f ? (uint32_t)qy : 0u;
+        if (ix > T - 1) ix = T - 1;
+        if (iy > T - 1) iy = T - 1;
+        return part1by1(ix) | (part1by1(iy) << 1);
+    }
+    qx = ((x / ORACLE_SIM_SIZE) + 1.0f) * 0.5f;
+    qy = ((y / ORACLE_SIM_SIZE) + 1.0f) * 0.5f;
+    qx = qx * (float)T;
+    qy = qy * (float)T;
+    if (qx >= 0.0f && qx < (float)T && qy >= 0.0f && qy < (float)T)
+        return part1by1((uint32_t)qx) | (part1by1((uint32_t)qy) << 1);
+    return 0xffffffffu;
+}
+
+size_t oracle_bh_capacity_sparse(size_t na, unsigned bits) {
+    /* every particle plus, per depth, at most min(4^d, na/2) cells holding
+     * two or more particles */
+    size_t cells = 0;
+    for (unsigned d = 0; d <= bits; ++d) {
+        const size_t lvl = (size_t)1 << (2 * d);
+        cells += lvl < na / 2 ? lvl : na / 2;
+    }
+    return na + cells + 1;
+}
+
+typedef struct {
+    uint32_t key, idx;
+} kv_pair;
+
+static int kv_cmp(const void *a, const void *b) {
+    const kv_pair *x = (const kv_pair *)a, *y = (const kv_pair *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->idx < y->idx ? -1 : (x->idx > y->idx ? 1 : 0);
+}
+
+typedef struct {
+    const kv_pair *kv;
+    const float *pos, *mass;
+    unsigned bits;
+    float r0;
+    oracle_tree *tree;
+    size_t next; /* next free pre-order slot */
+    int overflow;
+} sparse_ctx;
+
+typedef struct {
+    uint32_t n, y;
+    float sx, sy, sm;
+} sparse_sum;
+
+/* The cell of depth d holding sorted positions [lo, hi): the dense pyramid's
+ * rule evaluated only where particles are.  A depth-`bits` leaf adds its
+ * particles' m*p in ascending index from zero; a shallower cell is
+ * ((c0 + c1) + c2) + c3 over its quadrants, an empty quadrant being +0 (the
+ * pyramid's untouched cell); y = sum of the children's y, + 1 when above 1.
+ * A cell of two or more particles takes the pre-order slot before its
+ * subtree; particles are nodes of their own. */
+static sparse_sum sparse_rec(sparse_ctx *c, unsigned d, size_t lo, size_t hi) {
+    sparse_sum r = {0u, 0u, 0.0f, 0.0f, 0.0f};
+    const size_t n = hi - lo;
+    if (n == 0) return r;
+    oracle_tree *t = c->tree;
+    const size_t slot = c->next;
+    if (n >= 2) c->next += 1;
+    if (d == c->bits) {
+        for (size_t k = lo; k < hi; ++k) {
+            const uint32_t i = c->kv[k].idx;
+            const float x = c->pos[2 * (size_t)i], y = c->pos[2 * (size_t)i + 1];
+            const float m = c->mass[i];
+            r.sx = r.sx + x * m;
+            r.sy = r.sy + y * m;
+            r.sm = r.sm + m;
+            const size_t ps = c->next++;
+            if (ps >= t->cap) { c->overflow = 1; continue; }
+            t->sorted_src[k] = i;
+            t->counts[ps] = 1u;
+            t->radii[ps] = 0.0f;
+            t->node_pos[2 * ps] = x;
+            t->node_pos[2 * ps + 1] = y;
+            t->node_mass[ps] = m;
+        }
+        r.n = (uint32_t)n;
+        r.y = (uint32_t)n + (n >= 2 ? 1u : 0u);
+    } else {
+        const unsigned shift = 2u * (c->bits - 1u - d);
+        sparse_sum v[4];
+        size_t a = lo;
+        for (uint32_t q = 0; q < 4; ++q) {
+            size_t b = a;
+            while (b < hi && ((c->kv[b].key >> shift) & 3u) == q) ++b;
+            v[q] = sparse_rec(c, d + 1, a, b);
+            a = b;
+        }
+        r.n = v[0].n + v[1].n + v[2].n + v[3].n;
+        r.y = v[0].y + v[1].y + v[2].y + v[3].y;
+        r.y += r.y > 1u ? 1u : 0u;
+        r.sx = ((v[0].sx + v[1].sx) + v[2].sx) + v[3].sx;
+        r.sy = ((v[0].sy + v[1].sy) + v[2].sy) + v[3].sy;
+        r.sm = ((v[0].sm + v[1].sm) + v[2].sm) + v[3].sm;
+    }
+    if (n >= 2) {
+        if (slot >= t->cap) { c->overflow = 1; return r; }
+        float px = r.sx, py = r.sy;
+        if (r.sm != 0.0f) { px = px / r.sm; py = py / r.sm; }
+        t->counts[slot] = r.y;
+        t->radii[slot] = c->r0 / (float)(1u << (2 * d));
+        t->node_pos[2 * slot] = px;
+        t->node_pos[2 * slot + 1] = py;
+        t->node_mass[slot] = r.sm;
+    }
+    return r;
+}
+
+long oracle_bh_build_sparse(size_t na, float *pos, float *mass, float *pos_out,
+                            float *mass_out, const float *box, unsigned bits,
+                            oracle_tree *tree, size_t *n_inside) {
+    if (bits != 10 && bits != 15) return -1;
+    kv_pair *kv = (kv_pair *)malloc(sizeof(kv_pair) * (na ? na : 1));
+    if (!kv) return -1;
+    size_t inside = 0;
+    for (size_t i = 0; i < na; ++i) {
+        tree->sorted_src[i] = (uint32_t)na;
+        const uint32_t c = oracle_cell_bits(pos[2 * i], pos[2 * i + 1], box, bits);
+        if (c != 0xffffffffu) {
+            kv[inside].key = c;
+            kv[inside].idx = (uint32_t)i;
+            ++inside;
+        } else { /* the kill of oracle_bh_build */
+            const float kill = -ORACLE_SIM_SIZE * 2.0f;
+            pos[2 * i] = kill; pos[2 * i + 1] = kill; mass[i] = 0.0f;
+            if (pos_out) { pos_out[2 * i] = kill; pos_out[2 * i + 1] = kill; }
+            if (mass_out) mass_out[i] = 0.0f;
+        }
+    }
+    if (n_inside) *n_inside = inside;
+    qsort(kv, inside, sizeof(kv_pair), kv_cmp);
+    sparse_ctx c = {kv, pos, mass, bits,
+                    box ? box[3] : ORACLE_SIM_SIZE * ORACLE_SIM_SIZE * 2.0f, tree, 0, 0};
+    (void)sparse_rec(&c, 0, 0, inside);
+    free(kv);
+    if (c.overflow) return -1;
+    return (long)c.next;
 }
 
 /* ---- Barnes-Hut: traversal ------------------------------------------------- */

@@ -97,6 +97,21 @@ long oracle_bh_build_box(size_t na, float *pos, float *mass, float *pos_out,
                          float *mass_out, const float box[4], oracle_tree *tree,
                          size_t *n_inside);
 
+/* Sparse build at `bits` = 10 or 15 bits per axis (15: beyond the reference,
+ * opt-in).  The dense pyramid's rule evaluated recursively over the occupied
+ * cells only: a depth-`bits` leaf adds its particles' m*p in ascending index
+ * from zero, a shallower cell is ((c0+c1)+c2)+c3 over its quadrants with an
+ * empty quadrant +0, y = sum + [sum > 1], pre-order slots.  At 10 bits it is
+ * bit-identical to oracle_bh_build (box == NULL) / oracle_bh_build_box; at 15
+ * bits the cells are those of oracle_cell_bits and the leaves sit at depth
+ * 15 (radius r0 / 4^15).  box == NULL means the fixed [-500, 500) domain with
+ * its kill.  Returns the node count, or -1 when cap is too small or bits is
+ * neither 10 nor 15. */
+uint32_t oracle_cell_bits(float x, float y, const float *box, unsigned bits);
+size_t oracle_bh_capacity_sparse(size_t na, unsigned bits);
+long oracle_bh_build_sparse(size_t na, float *pos, float *mass, float *pos_out,
+                            float *mass_out, const float *box, unsigned bits,
+                            oracle_tree *tree, size_t *n_inside);
 /* Group traversal (ForceShader.comp:78-141): groups of W consecutive sorted
  * slots walk one path; a node is accepted iff every non-sentinel lane has
  * radius <= (d.d + eps2)*theta2.  Writes acc[2*src] for non-sentinel lanes

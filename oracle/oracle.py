@@ -52,6 +52,13 @@ def lib() -> C.CDLL:
         L.oracle_bh_build.argtypes = [C.c_size_t, fp, fp, fp, fp, C.c_void_p,
                                       C.POINTER(C.c_size_t)]
         L.oracle_bh_build.restype = C.c_long
+        L.oracle_cell_bits.argtypes = [C.c_float, C.c_float, fp, C.c_uint]
+        L.oracle_cell_bits.restype = C.c_uint32
+        L.oracle_bh_capacity_sparse.argtypes = [C.c_size_t, C.c_uint]
+        L.oracle_bh_capacity_sparse.restype = C.c_size_t
+        L.oracle_bh_build_sparse.argtypes = [C.c_size_t, fp, fp, fp, fp, fp, C.c_uint,
+                                             C.c_void_p, C.POINTER(C.c_size_t)]
+        L.oracle_bh_build_sparse.restype = C.c_long
         L.oracle_bh_traverse.argtypes = [C.c_size_t, C.c_uint32, fp, fp, C.c_void_p,
                                          C.c_size_t, C.c_float, C.c_float, fp, dp, u64, u64,
                                          u64, C.POINTER(C.c_uint8), C.c_int]
@@ -142,13 +149,25 @@ def cell20_box(x: float, y: float, box) -> int:
     return int(lib().oracle_cell20_box(x, y, _f(box)))
 
 
-def bh_build(pos, mass, box=None):
+def cell_bits(x: float, y: float, box=None, bits: int = 15) -> int:
+    """Cell on a 2^bits grid per axis (fixed domain, or the box {cx, cy, h, r0})."""
+    if box is None:
+        return int(lib().oracle_cell_bits(x, y, None, bits))
+    bx = _f32(box)
+    return int(lib().oracle_cell_bits(x, y, _f(bx), bits))
+
+
+def bh_build(pos, mass, box=None, bits=None):
     """Returns (tree dict, n_nodes, n_inside, pos_after_kill, mass_after_kill);
     with box ({cx, cy, h, r0}) the build is over that box instead of the fixed
-    [-500, 500) domain."""
+    [-500, 500) domain.  bits=None is the dense pyramid restatement of the
+    reference; bits=10 or 15 the sparse build with leaves at that depth."""
     pos, mass = _f32(pos).copy(), _f32(mass).copy()
     na = mass.shape[0]
-    cap = int(lib().oracle_bh_capacity(na))
+    if bits is None:
+        cap = int(lib().oracle_bh_capacity(na))
+    else:
+        cap = int(lib().oracle_bh_capacity_sparse(na, bits))
     counts = np.zeros(cap, np.uint32)
     radii = np.zeros(cap, np.float32)
     npos = np.zeros((cap, 2), np.float32)
@@ -157,7 +176,12 @@ def bh_build(pos, mass, box=None):
     t = _Tree(counts.ctypes.data_as(C.POINTER(C.c_uint32)), _f(radii), _f(npos), _f(nmass),
               src.ctypes.data_as(C.POINTER(C.c_uint32)), cap)
     inside = C.c_size_t(0)
-    if box is None:
+    if bits is not None:
+        bx = None if box is None else _f32(box)
+        nn = lib().oracle_bh_build_sparse(na, _f(pos), _f(mass), None, None,
+                                          None if bx is None else _f(bx), bits, C.byref(t),
+                                          C.byref(inside))
+    elif box is None:
         nn = lib().oracle_bh_build(na, _f(pos), _f(mass), None, None, C.byref(t), C.byref(inside))
     else:
         bx = _f32(box)

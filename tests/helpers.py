@@ -69,18 +69,21 @@ def box_numpy(pos: np.ndarray) -> np.ndarray:
     return np.array([c[0], c[1], h, np.float32(np.float32(h * h) * np.float32(2.0))], np.float32)
 
 
-def brute_tree(pos: np.ndarray, mass: np.ndarray, box=None):
+def brute_tree(pos: np.ndarray, mass: np.ndarray, box=None, bits: int = 10):
     """Builds the reference topology by recursive subdivision of the domain
     (explicit geometry, no Morton codes): quadrant by comparing the normalised
     coordinate against the cell's midpoint.  Returns node arrays in pre-order
-    and the sorted source list, with FP32 sums in the reference association."""
+    and the sorted source list, with FP32 sums in the reference association.
+    bits = 15 puts the leaves on a 32768^2 grid (depth 15) instead of 1024^2."""
     na = len(mass)
     p = pos.astype(np.float32)
+    TREE = 1 << bits
     if box is None:
-        cells = cells_numpy(pos)
-        inside = [i for i in range(na) if cells[i] != 0xFFFFFFFF]
-        # integer leaf coordinates from the same normalisation
         q = (((p / SIM_SIZE) + np.float32(1.0)) * np.float32(0.5)) * np.float32(TREE)
+        with np.errstate(invalid="ignore"):
+            ok = (q[:, 0] >= 0) & (q[:, 0] < TREE) & (q[:, 1] >= 0) & (q[:, 1] < TREE)
+        inside = [i for i in range(na) if ok[i]]
+        # integer leaf coordinates from the same normalisation
         ij = {i: (int(q[i, 0]), int(q[i, 1])) for i in inside}
         r0 = np.float32(500000.0)
     else:
@@ -113,7 +116,7 @@ def brute_tree(pos: np.ndarray, mass: np.ndarray, box=None):
         return sx, sy, sm
 
     def sums(members, depth, x0, y0):
-        if depth == 10:
+        if depth == bits:
             return leaf_sums(members)
         quads, origins = split(members, depth, x0, y0)
         acc = []
@@ -142,7 +145,7 @@ def brute_tree(pos: np.ndarray, mass: np.ndarray, box=None):
         radii.append(np.float32(np.float32(r0) / np.float32(4 ** depth)))
         npos.append((np.float32(cx), np.float32(cy))); nmass.append(sm)
         size = 1
-        if depth == 10:
+        if depth == bits:
             for i in members:
                 order.append(i)
                 counts.append(1); radii.append(np.float32(0))

@@ -275,6 +275,91 @@ def test_bh_build_box_matches_recursive_quadtree(seed):
     assert tree["radii"][0] == box[3]
 
 
+# ---- sparse build, 15-bit leaves (opt-in, beyond the reference) ------------------
+
+def _sparse_inputs():
+    rng = np.random.default_rng(31)
+    pos = rng.normal(0, 200, size=(2000, 2)).astype(np.float32)
+    pos[:600] = rng.normal(30, 0.1, size=(600, 2))         # one dense knot
+    pos[600:610] = [[700, 0], [0, -900], [np.nan, 1], [500, 500], [-500, -500],
+                    [3, 3], [3, 3], [3, 3], [-0.0, 0.0], [0.0, -0.0]]
+    yield "knot-2000", pos, rng.uniform(1, 10, 2000).astype(np.float32)
+    pos, _, mass = H.dyadic_particles(700, seed=32)
+    yield "dyadic-700", pos, mass
+    yield "one-point-64", np.full((64, 2), 0.1, np.float32), np.ones(64, np.float32)
+    yield "single", np.array([[5.0, 6.0]], np.float32), np.array([2.0], np.float32)
+    yield "empty", np.full((64, 2), 900, np.float32), np.ones(64, np.float32)
+
+
+@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("boxed", [False, True])
+def test_sparse_build_at_10_bits_is_the_dense_build(case, boxed):
+    """The sparse recursion evaluates the dense pyramid's rule only where
+    particles are, so at 10 bits it must reproduce the reference restatement
+    byte for byte: node arrays, sorted sources, kills."""
+    name, pos, mass = list(_sparse_inputs())[case]
+    box = O.bh_box(pos) if boxed else None
+    a = O.bh_build(pos, mass, box=box)
+    b = O.bh_build(pos, mass, box=box, bits=10)
+    assert a[1:3] == b[1:3], name
+    for k in ("counts", "radii", "node_pos", "node_mass"):
+        assert a[0][k].tobytes() == b[0][k].tobytes(), k
+    assert np.array_equal(a[0]["sorted_src"], b[0]["sorted_src"])
+    assert a[3].tobytes() == b[3].tobytes() and a[4].tobytes() == b[4].tobytes()
+
+
+@pytest.mark.parametrize("case", range(5))
+@pytest.mark.parametrize("boxed", [False, True])
+def test_sparse_build_15_bits_matches_recursive_quadtree(case, boxed):
+    name, pos, mass = list(_sparse_inputs())[case]
+    box = O.bh_box(pos) if boxed else None
+    tree, nn, nin, pk, mk = O.bh_build(pos, mass, box=box, bits=15)
+    _compare_tree(tree, H.brute_tree(pos, mass, box=box, bits=15))
+    if nn:
+        r0 = box[3] if boxed else np.float32(500000.0)
+        assert tree["radii"][0] == (r0 if nn > 1 else 0)
+
+
+def test_cell_bits_prefix_and_bitloop():
+    """15-bit codes: the bit-loop Morton of the truncated coordinate, and
+    code >> 10 is the 20-bit cell (fixed domain and adaptive box)."""
+    rng = np.random.default_rng(33)
+    pos = rng.uniform(-520, 520, size=(4000, 2)).astype(np.float32)
+    pos[:6] = [[-500, -500], [499.99997, 0], [500, 0], [0, -500.00003], [1e-30, -1e-30], [0, 0]]
+    q = (((pos / H.SIM_SIZE) + np.float32(1)) * np.float32(0.5)) * np.float32(32768)
+    ok = (q >= 0).all(1) & (q < 32768).all(1)
+    want = np.where(ok, H.morton_bitloop(np.where(ok, q[:, 0], 0).astype(np.uint32),
+                                         np.where(ok, q[:, 1], 0).astype(np.uint32), bits=15),
+                    np.uint32(0xFFFFFFFF))
+    got = np.array([O.cell_bits(float(x), float(y)) for x, y in pos], np.uint32)
+    assert np.array_equal(got, want)
+    c20 = np.array([O.cell20(float(x), float(y)) for x, y in pos], np.uint32)
+    assert np.array_equal(np.where(got == 0xFFFFFFFF, got, got >> 10), c20)
+    box = O.bh_box(pos)
+    gb = np.array([O.cell_bits(float(x), float(y), box) for x, y in pos], np.uint32)
+    cb = np.array([O.cell20_box(float(x), float(y), box) for x, y in pos], np.uint32)
+    assert np.array_equal(gb >> 10, cb)
+
+
+def test_sparse_15_bit_invariants():
+    """counts[0] = N_in + #cells of >= 2 particles over depths 0..15 (from the
+    codes alone); the knot that is one depth-10 bucket of 600 particles is
+    split into far smaller leaves."""
+    name, pos, mass = next(_sparse_inputs())
+    tree, nn, nin, pk, _ = O.bh_build(pos, mass, bits=15)
+    codes = np.array([O.cell_bits(float(x), float(y)) for x, y in pos], np.uint32)
+    codes = codes[codes != 0xFFFFFFFF]
+    ncell = 0
+    for d in range(16):
+        _, cnt = np.unique(codes >> np.uint32(2 * (15 - d)), return_counts=True)
+        ncell += int((cnt >= 2).sum())
+    assert nin == codes.size and tree["counts"][0] == nin + ncell == nn
+    _, b10 = np.unique(codes >> np.uint32(10), return_counts=True)
+    _, b15 = np.unique(codes, return_counts=True)
+    assert b10.max() >= 300 and b15.max() <= b10.max() // 4
+    assert tree["radii"][tree["radii"] > 0].min() == np.float32(500000.0) / np.float32(4.0**15)
+
+
 @pytest.mark.parametrize("W", [1, 32, 64])
 @pytest.mark.parametrize("theta", [0.5, 1.0])
 def test_bh_traverse_matches_recursive_group_vote(W, theta):
