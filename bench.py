@@ -102,9 +102,16 @@ def bench_direct_sum(steps: int, warmup: int) -> dict:
     }
 
 
-def bench_barnes_hut(steps: int, warmup: int) -> dict:
+def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = False) -> dict:
+    """leaf_bits = 15 / box: the opt-in tree options beyond the reference
+    (their trees differ from the reference's, so vs_baseline is null)."""
     p = gsim.generate("two-exponential-disks", BH_N, 33.0, 1.0, 10.0, 1.0, seed=3)
     sim = gsim.Simulation(p, "barnes-hut", accuracy_parameter=1.0, group_width=64)
+    if leaf_bits != 10:
+        sim.set_tuning(gsim.TUNE_BH_LEAF_BITS, leaf_bits)
+    if box:
+        sim.set_tuning(gsim.TUNE_BH_ADAPTIVE_BOX, 1)
+    reference_tree = leaf_bits == 10 and not box
     sim.step(max(warmup, 1))
     sim.synchronize()
     sim.set_timing(True)
@@ -131,7 +138,7 @@ def bench_barnes_hut(steps: int, warmup: int) -> dict:
         "value": ms_step,
         "unit": "ms/step",
         "higher_is_better": False,
-        "vs_baseline": REF_BH_MS_500K / ms_step,
+        "vs_baseline": REF_BH_MS_500K / ms_step if reference_tree else None,
         "particle_steps_per_s": BH_N / (ms_step * 1e-3),
         "build_ms": build_ms,
         "traverse_ms": trav_ms,
@@ -140,7 +147,8 @@ def bench_barnes_hut(steps: int, warmup: int) -> dict:
         "config": {"workload": "barnes-hut, two exponential disks of 250,000 (radius 33, "
                                "scale 7, centres +-40, bulk vx -+2), theta=1, W=64, "
                                "G=1, softening 0.2, dt=1e-3", "particles": BH_N,
-                   "theta": 1.0, "group_width": 64},
+                   "theta": 1.0, "group_width": 64, "leaf_bits": leaf_bits,
+                   "adaptive_box": box},
         "roofline": {
             "bound": "valu",
             "kernel": "bh_traverse_bal_kernel",
@@ -343,6 +351,10 @@ def main() -> int:
     ap.add_argument("--bh-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--bh-leaf-bits", type=int, default=10, choices=(10, 15),
+                    help="Barnes-Hut leaf grid bits per axis (15: beyond the reference)")
+    ap.add_argument("--bh-adaptive-box", action="store_true",
+                    help="Barnes-Hut tree over the bounding square (beyond the reference)")
     ap.add_argument("--batch-steps", type=int, default=100)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=1.0,
@@ -364,8 +376,10 @@ def main() -> int:
     log(f"  {ds['value']:.1f} Ginteractions/s, {ds['ms_per_step']:.4f} ms/step")
     bh = None
     if not args.no_bh:
-        log(f"barnes-hut 500k: steps {args.bh_steps}")
-        bh = bench_barnes_hut(args.bh_steps, max(args.warmup, 1))
+        log(f"barnes-hut 500k: steps {args.bh_steps}, {args.bh_leaf_bits}-bit leaves"
+            + (", adaptive box" if args.bh_adaptive_box else ""))
+        bh = bench_barnes_hut(args.bh_steps, max(args.warmup, 1), args.bh_leaf_bits,
+                              args.bh_adaptive_box)
         log(f"  {bh['value']:.4f} ms/step (build {bh['build_ms']:.4f}, traverse "
             f"{bh['traverse_ms']:.4f})")
     bt = None

@@ -283,7 +283,7 @@ gsim_status gsim_create(const gsim_params *params, const float *pos_xy, const fl
         gsim_status st = ds_configure(s, gsim::ds_sym_default(na));
         if (st != GSIM_OK) return bail(st);
     } else {
-        e = gsim::bh_alloc(&s->bh, na);
+        e = gsim::bh_alloc(&s->bh, na, gsim::kTreeDepth);
         if (e != hipSuccess)
             return bail(fail(GSIM_ERROR_OUT_OF_MEMORY, "Barnes-Hut buffers: %s", hipGetErrorString(e)));
     }
@@ -380,6 +380,26 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
             GSIM_HIP(hipStreamSynchronize(s->stream));
             s->bh_box = value ? 1u : 0u;
             return GSIM_OK;
+        case GSIM_TUNE_BH_LEAF_BITS: {
+            if (s->params.algorithm != GSIM_BARNES_HUT)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "not a Barnes-Hut handle");
+            if (value != gsim::kTreeDepth && value != gsim::kDeepBits)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "leaf bits must be 10 or 15");
+            if (value == s->bh.bits) return GSIM_OK;
+            GSIM_HIP(hipSetDevice(s->device));
+            GSIM_HIP(hipStreamSynchronize(s->stream));
+            // the node arrays are sized for the depth: reallocate them
+            gsim::bh_free(&s->bh);
+            s->has_tree = false;
+            const hipError_t e = gsim::bh_alloc(&s->bh, s->na, value);
+            if (e != hipSuccess) {
+                gsim::bh_free(&s->bh);
+                (void)gsim::bh_alloc(&s->bh, s->na, gsim::kTreeDepth);
+                return fail(GSIM_ERROR_OUT_OF_MEMORY, "Barnes-Hut buffers for %u-bit leaves: %s",
+                            value, hipGetErrorString(e));
+            }
+            return GSIM_OK;
+        }
         case GSIM_TUNE_BH_SPLIT:
             if (value > 1000) return fail(GSIM_ERROR_INVALID_ARGUMENT, "split threshold must be in [0, 1000]");
             s->trav_split = value;
@@ -449,7 +469,7 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
             GSIM_HIP(hipMemcpy(buf, s->bh.keys[0], *written, hipMemcpyDeviceToHost));
             uint32_t *u = (uint32_t *)buf;
             for (size_t i = 0; i < na; ++i)
-                if (u[i] >= gsim::kKilledKey) u[i] = 0xffffffffu;
+                if (u[i] >= (1u << (2 * s->bh.bits))) u[i] = 0xffffffffu;
             return GSIM_OK;
         }
         case GSIM_DUMP_BOX:

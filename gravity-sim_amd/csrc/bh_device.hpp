@@ -33,6 +33,23 @@ __device__ __forceinline__ int shared_depth(uint32_t a, uint32_t b) {
     return 9 - ((31 - __clz(x)) >> 1);
 }
 
+// The same for keys of `bits` bits per axis (0..bits).
+__device__ __forceinline__ int shared_depth_bits(uint32_t a, uint32_t b, int bits) {
+    const uint32_t x = a ^ b;
+    if (x == 0) return bits;
+    return bits - 1 - ((31 - __clz(x)) >> 1);
+}
+
+// 15-bit cells (30-bit keys): the normalised coordinate of cell20 scaled by
+// 32768 instead of 1024.  Scaling by a power of two is exact, so key >> 10 is
+// the 20-bit cell.
+__device__ __forceinline__ uint32_t cell30(float x, float y) {
+    const float qx = unit_coord(x) * 32768.0f, qy = unit_coord(y) * 32768.0f;
+    if (qx >= 0.0f && qx < 32768.0f && qy >= 0.0f && qy < 32768.0f)
+        return part1by1((uint32_t)qx) | (part1by1((uint32_t)qy) << 1);
+    return 0xffffffffu;
+}
+
 // Radius base of the fixed domain: S*S*2 (TreeMoveShader.comp:70-71).
 constexpr float kFixedR0 = kSimSize * kSimSize * 2.0f;
 
@@ -50,6 +67,16 @@ __device__ __forceinline__ uint32_t cell20_box(float x, float y, float cx, float
     const float qy = ((((y - cy) / h) + 1.0f) * 0.5f) * (float)kTreeSize;
     const uint32_t ix = min(qx > 0.0f ? (uint32_t)qx : 0u, kTreeSize - 1);
     const uint32_t iy = min(qy > 0.0f ? (uint32_t)qy : 0u, kTreeSize - 1);
+    return part1by1(ix) | (part1by1(iy) << 1);
+}
+
+// cell20_box on the 32768^2 grid (clamped to [0, 32767]).
+__device__ __forceinline__ uint32_t cell30_box(float x, float y, float cx, float cy, float h) {
+    if (!__builtin_isfinite(x) || !__builtin_isfinite(y)) return 0xffffffffu;
+    const float qx = ((((x - cx) / h) + 1.0f) * 0.5f) * 32768.0f;
+    const float qy = ((((y - cy) / h) + 1.0f) * 0.5f) * 32768.0f;
+    const uint32_t ix = min(qx > 0.0f ? (uint32_t)qx : 0u, 32767u);
+    const uint32_t iy = min(qy > 0.0f ? (uint32_t)qy : 0u, 32767u);
     return part1by1(ix) | (part1by1(iy) << 1);
 }
 

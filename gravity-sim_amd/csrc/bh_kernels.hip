@@ -115,7 +115,8 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
     box[3] = (h * h) * 2.0f;
 }
 
-template <bool kBox>
+// kDeep: 30-bit keys of the 32768^2 grid (15-bit leaves) instead of 20-bit cells.
+template <bool kBox, bool kDeep>
 __global__ __launch_bounds__(256) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     const float *__restrict__ box) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < 2 * kTiles) tile_range[i] = 0;
-    if (i < 3 * 256) digit_total[i] = 0;
+    if (i < 4 * 256) digit_total[i] = 0;
     if (i == 0) {
         ctr->visits = 0;
         ctr->accepts = 0;
@@ -132,7 +133,9 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     }
     if (i >= na) return;
     const float2 p = pos[i];
-    uint32_t c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
+    uint32_t c;
+    if (kDeep) c = kBox ? cell30_box(p.x, p.y, box[0], box[1], box[2]) : cell30(p.x, p.y);
+    else c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
     if (c == 0xffffffffu) {
         // InitShader.comp:110-117; the reference leaves vel_out stale, here it
         // keeps the input velocity.
@@ -141,7 +144,7 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
         pos_out[i] = make_float2(k, k);
         mass[i] = 0.0f;
         vel_out[i] = vel[i];
-        c = kKilledKey;
+        c = kDeep ? kKilledKeyDeep : kKilledKey;
     }
     keys[i] = c;
     idx[i] = i;
@@ -276,29 +279,32 @@ struct PosInfo {
     uint32_t key;
 };
 
+// bits: 10 (the reference's grid) or 15 (15-bit leaves); keys at or above
+// 1 << 2*bits are killed particles.
 __device__ __forceinline__ PosInfo pos_info(const uint32_t *__restrict__ keys, uint32_t na,
-                                            uint32_t k) {
+                                            uint32_t k, int bits) {
     PosInfo q{-1, -1, 0u, 0u, false, 0u};
     if (k >= na) return q;
+    const uint32_t killed = 1u << (2 * bits);
     q.key = keys[k];
-    if (q.key >= kKilledKey) return q;
+    if (q.key >= killed) return q;
     q.inside = true;
-    q.sk = k == 0 ? -1 : shared_depth(keys[k - 1], q.key);
+    q.sk = k == 0 ? -1 : shared_depth_bits(keys[k - 1], q.key, bits);
     if (k + 1 < na) {
         const uint32_t nk = keys[k + 1];
-        if (nk < kKilledKey) q.sk1 = shared_depth(q.key, nk);
+        if (nk < killed) q.sk1 = shared_depth_bits(q.key, nk, bits);
     }
     q.ns = q.sk1 > q.sk ? (uint32_t)(q.sk1 - q.sk) : 0u;
-    q.lf = q.sk < 10 ? 1u : 0u;
+    q.lf = q.sk < bits ? 1u : 0u;
     return q;
 }
 
 __global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict__ keys,
-                                                      uint32_t na,
+                                                      uint32_t na, int bits,
                                                       uint32_t *__restrict__ block_cells,
                                                       uint32_t *__restrict__ block_leaves) {
     __shared__ uint32_t wa[4], wb[4];
-    const PosInfo q = pos_info(keys, na, blockIdx.x * 256 + threadIdx.x);
+    const PosInfo q = pos_info(keys, na, blockIdx.x * 256 + threadIdx.x, bits);
     uint32_t a = q.ns, b = q.lf;
     wave_scan2(a, b);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict
 
 __global__ __launch_bounds__(256) void bh_emit_kernel(
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ sidx,
-    const float2 *__restrict__ pos, const float *__restrict__ mass, uint32_t na,
+    const float2 *__restrict__ pos, const float *__restrict__ mass, uint32_t na, int bits,
     const uint32_t *__restrict__ block_cells, const uint32_t *__restrict__ block_leaves,
     int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
     uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
@@ -324,7 +330,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     BhCounters *__restrict__ ctr) {
     __shared__ uint32_t wa[4], wb[4];
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    const PosInfo q = pos_info(keys, na, k);
+    const PosInfo q = pos_info(keys, na, k, bits);
     uint32_t a = q.ns, b = q.lf;
     wave_scan2(a, b);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -388,14 +394,34 @@ constexpr uint32_t kBoxParts = 256;  // blocks of the box reduction
 // coalesced load and adds them in order.  Phase B, one thread per sorted position: the
 // number of consecutive particle nodes from that particle's slot, i.e. up to
 // the next position where a cell node starts (breaker[C(k+1)]).
+// With 15-bit leaves (nsum != nullptr) a leaf of two or more particles is
+// written as its depth-15 cell node at once, with its sums in nsum for the
+// level reduction; there are no tiles.
+__device__ __forceinline__ void emit_deep_leaf(const float4 &r, uint32_t f, uint32_t n,
+                                               const int8_t *__restrict__ sdep,
+                                               const uint32_t *__restrict__ cpos,
+                                               float4 *__restrict__ node4,
+                                               uint32_t *__restrict__ ncount,
+                                               float4 *__restrict__ nsum, float r0) {
+    if (n < 2u) return;
+    const TileCell c{n, n + 1u, f, 0u, r.x, r.y, r.z, 0.0f};
+    emit_cell(c, (int)kDeepBits, sdep, cpos, node4, ncount, r0);
+    const uint32_t slot = f + cpos[f] + (uint32_t)((int)kDeepBits - (int)sdep[f] - 1);
+    nsum[slot] = make_float4(r.x, r.y, r.z, __uint_as_float(n));
+}
+
 __global__ __launch_bounds__(256) void bh_leaf_kernel(
     const uint32_t *__restrict__ leaf_first, const uint32_t *__restrict__ leaf_key,
-    const float4 *__restrict__ prod, const uint32_t *__restrict__ keys, uint32_t na,
+    const float4 *__restrict__ prod, const uint32_t *__restrict__ keys, uint32_t na, int bits,
     const uint32_t *__restrict__ cpos, const uint32_t *__restrict__ breaker,
     const BhCounters *__restrict__ ctr, float4 *__restrict__ leaf_sum,
-    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ prun) {
+    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ prun,
+    const int8_t *__restrict__ sdep, float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
+    float4 *__restrict__ nsum, const float *__restrict__ box) {
     __shared__ float4 stage[4][64];
     const uint32_t n_in = ctr->n_in, nl = ctr->n_leaves, ncells = ctr->n_cells;
+    const bool deep = nsum != nullptr;
+    const float r0 = box ? box[3] : kFixedR0;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
@@ -411,9 +437,10 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
         const bool small = valid && e - f <= kSmallLeaf;
         if (small) {
             const float4 r = lane_leaf_sum(prod, f, e);
-            leaf_sum[L] = make_float4(r.x, r.y, r.z, __uint_as_float(e - f));
+            if (deep) emit_deep_leaf(r, f, e - f, sdep, cpos, node4, ncount, nsum, r0);
+            else leaf_sum[L] = make_float4(r.x, r.y, r.z, __uint_as_float(e - f));
         }
-        if (valid) {
+        if (valid && !deep) {
             const uint32_t T = leaf_key[L] >> 10;
             if (L == 0 || (leaf_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
             if (L + 1 == nl || (leaf_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
@@ -435,15 +462,81 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
             const uint32_t bf = __builtin_amdgcn_readfirstlane(__shfl(f, j, 64));
             const uint32_t be = __builtin_amdgcn_readfirstlane(__shfl(e, j, 64));
             const float4 r = wave_leaf_sum(stage[w], prod, bf, be, lane);
-            if (lane == 0)
-                leaf_sum[wave_id + (i0 + j) * waves] = make_float4(r.x, r.y, r.z, __uint_as_float(be - bf));
+            if (lane == 0) {
+                if (deep) emit_deep_leaf(r, bf, be - bf, sdep, cpos, node4, ncount, nsum, r0);
+                else leaf_sum[wave_id + (i0 + j) * waves] = make_float4(r.x, r.y, r.z, __uint_as_float(be - bf));
+            }
         }
     }
     for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n_in; k += gridDim.x * 256) {
-        const PosInfo q = pos_info(keys, na, k);
+        const PosInfo q = pos_info(keys, na, k, bits);
         const uint32_t c1 = cpos[k] + q.ns;  // C(k+1)
         const uint32_t run_end = c1 < ncells ? breaker[c1] : n_in;
         prun[k + c1] = run_end > k ? run_end - k : 1u;
+    }
+}
+
+// ---- 6'. sparse level reduction (15-bit leaves) ---------------------------------
+
+// One launch per depth d = 14..0.  The cell node of depth d that starts at
+// sorted position k (s(k) < d <= s(k+1)) reads its children in pre-order from
+// slot + 1 on: a particle node (count 1) contributes 0 + m*p, a cell node its
+// sums in nsum; each child's quadrant comes from the key bits of its first
+// particle.  Sums are ((c0 + c1) + c2) + c3 with an empty quadrant +0: the
+// dense tile reduction's rule, evaluated only where particles are.
+__global__ __launch_bounds__(256) void bh_level_kernel(
+    const uint32_t *__restrict__ keys, const int8_t *__restrict__ sdep,
+    const uint32_t *__restrict__ cpos, const float4 *__restrict__ prod,
+    const BhCounters *__restrict__ ctr, float4 *__restrict__ node4,
+    uint32_t *__restrict__ ncount, float4 *__restrict__ nsum, int d,
+    const float *__restrict__ box) {
+    const uint32_t n_in = ctr->n_in, n_nodes = ctr->n_nodes;
+    const float r0 = box ? box[3] : kFixedR0;
+    const uint32_t shift = 2u * (kDeepBits - 1u - (uint32_t)d);  // child quadrant bits
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n_in; k += gridDim.x * 256) {
+        const int sk = sdep[k];
+        const int sk1 = k + 1 < n_in ? (int)sdep[k + 1] : -1;
+        if (!(sk < d && d <= sk1)) continue;
+        const uint32_t slot = k + cpos[k] + (uint32_t)(d - sk - 1);
+        const uint32_t prefix = keys[k] >> (shift + 2u);
+        float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0, v2 = v0, v3 = v0;
+        uint32_t c = slot + 1u, f = k, n = 0u, y = 1u;
+        // at most four children (distinct quadrants)
+        for (int it = 0; it < 4 && f < n_in && c < n_nodes; ++it) {
+            const uint32_t key = keys[f];
+            if ((key >> (shift + 2u)) != prefix) break;
+            const uint32_t cy = ncount[c];
+            float4 v;
+            uint32_t cn;
+            if (cy == 1u) {  // a particle node
+                const float4 p = prod[f];
+                v = make_float4(0.0f + p.x, 0.0f + p.y, 0.0f + p.z, 0.0f);
+                cn = 1u;
+            } else {
+                v = nsum[c];
+                cn = __float_as_uint(v.w);
+            }
+            const uint32_t q = (key >> shift) & 3u;
+            if (q == 0u) v0 = v;
+            else if (q == 1u) v1 = v;
+            else if (q == 2u) v2 = v;
+            else v3 = v;
+            n += cn;
+            y += cy;
+            f += cn;
+            c += cy;
+        }
+        const float sx = ((v0.x + v1.x) + v2.x) + v3.x;
+        const float sy = ((v0.y + v1.y) + v2.y) + v3.y;
+        const float sm = ((v0.z + v1.z) + v2.z) + v3.z;
+        float px = sx, py = sy;
+        if (sm != 0.0f) {
+            px = px / sm;
+            py = py / sm;
+        }
+        ncount[slot] = y;
+        node4[slot] = make_float4(px, py, sm, cell_radius(d, r0));
+        nsum[slot] = make_float4(sx, sy, sm, __uint_as_float(n));
     }
 }
 
@@ -876,19 +969,20 @@ __global__ __launch_bounds__(256) void bh_morton30_kernel(const float2 *__restri
 
 }  // namespace
 
-size_t bh_capacity(size_t na) {
+size_t bh_capacity(size_t na, uint32_t bits) {
     // every particle plus at most min(4^d, na/2) cells with >= 2 particles per level
     size_t cells = 0;
-    for (int d = 0; d <= (int)kTreeDepth; ++d) {
+    for (int d = 0; d <= (int)bits; ++d) {
         const size_t lvl = (size_t)1 << (2 * d);
         cells += lvl < na / 2 ? lvl : na / 2;
     }
     return na + cells + 64;
 }
 
-hipError_t bh_alloc(BhBuffers *b, size_t na) {
+hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     b->na = na;
-    b->cap = bh_capacity(na);
+    b->bits = bits;
+    b->cap = bh_capacity(na, bits);
     b->sort_blocks = (na + kSortTile - 1) / kSortTile;
     b->prep_blocks = (na + 255) / 256;
     hipError_t e = hipSuccess;
@@ -901,7 +995,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na) {
         GSIM_ALLOC(b->idx[k], na);
     }
     GSIM_ALLOC(b->hist, 256 * b->sort_blocks);
-    GSIM_ALLOC(b->digit_total, 3 * 256);
+    GSIM_ALLOC(b->digit_total, 4 * 256);
     GSIM_ALLOC(b->breaker, b->cap);
     GSIM_ALLOC(b->prun, b->cap);
     GSIM_ALLOC(b->block_cells, b->prep_blocks);
@@ -916,6 +1010,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na) {
     GSIM_ALLOC(b->top, kTiles);
     GSIM_ALLOC(b->node4, b->cap);
     GSIM_ALLOC(b->ncount, b->cap);
+    if (bits == kDeepBits) GSIM_ALLOC(b->nsum, b->cap);
     GSIM_ALLOC(b->sorted_src, na);
     GSIM_ALLOC(b->ctr, 1);
     GSIM_ALLOC(b->box, 4);
@@ -937,7 +1032,8 @@ void bh_free(BhBuffers *b) {
     }
     void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->prun, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
-                    b->node4, b->ncount, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box, b->box_part};
+                    b->node4, b->ncount, b->nsum, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box,
+                    b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
 }
@@ -948,24 +1044,28 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     const uint32_t na = (uint32_t)s->na;
     const uint32_t g256 = (na + 255) / 256;
     const uint32_t gkeys = g256 > (2 * kTiles + 255) / 256 ? g256 : (2 * kTiles + 255) / 256;
+    const bool deep = b.bits == kDeepBits;
+    const int bits = (int)b.bits;
     const float *box = nullptr;
     if (s->bh_box) {
         const uint32_t nparts = std::min<uint32_t>(g256, kBoxParts);
         bh_box_partial_kernel<<<nparts, 256, 0, s->stream>>>(pos, na, b.box_part);
         bh_box_final_kernel<<<1, 256, 0, s->stream>>>(b.box_part, nparts, b.box);
         box = b.box;
-        bh_keys_kernel<true><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
-                                                           b.keys[0], b.idx[0], b.tile_range,
-                                                           b.digit_total, b.ctr, box);
-    } else {
-        bh_keys_kernel<false><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na,
-                                                            b.keys[0], b.idx[0], b.tile_range,
-                                                            b.digit_total, b.ctr, nullptr);
     }
-    // keys[0] -> keys[1] -> keys[2] -> keys[1]; keys[0] keeps the per-particle cells
-    const int in_of[3] = {0, 1, 2}, out_of[3] = {1, 2, 1};
+#define GSIM_KEYS(B, D)                                                                    \
+    bh_keys_kernel<B, D><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na, \
+                                                       b.keys[0], b.idx[0], b.tile_range,   \
+                                                       b.digit_total, b.ctr, box)
+    if (box) { if (deep) GSIM_KEYS(true, true); else GSIM_KEYS(true, false); }
+    else { if (deep) GSIM_KEYS(false, true); else GSIM_KEYS(false, false); }
+#undef GSIM_KEYS
+    // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
+    // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
+    const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
+    const int passes = deep ? 4 : 3;
     const uint32_t nblk = (uint32_t)b.sort_blocks;
-    for (int pass = 0; pass < 3; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         const uint32_t shift = 8u * pass;
         const int i = in_of[pass], o = out_of[pass];
         uint32_t *totals = b.digit_total + 256 * pass;
@@ -975,18 +1075,26 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
         radix_downsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
             b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, shift, b.hist, nblk);
     }
-    b.sorted_slot = 1;
-    const uint32_t *sk = b.keys[1], *si = b.idx[1];
+    b.sorted_slot = deep ? 2 : 1;
+    const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
     const uint32_t pb = (uint32_t)b.prep_blocks;
-    bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, b.block_cells, b.block_leaves);
+    bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, bits, b.block_cells, b.block_leaves);
     scan_exclusive_kernel<<<2, 1024, 0, s->stream>>>(b.block_cells, b.block_leaves, pb);
-    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, na, b.block_cells,
+    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, na, bits, b.block_cells,
                                               b.block_leaves, b.sdep, b.cpos, b.prod,
                                               b.leaf_first, b.leaf_key, b.node4, b.ncount,
                                               b.sorted_src, b.breaker, b.ctr);
     const uint32_t lb = pb < kLeafBlocks ? pb : kLeafBlocks;
-    bh_leaf_kernel<<<lb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, sk, na, b.cpos,
-                                              b.breaker, b.ctr, b.leaf_sum, b.tile_range, b.prun);
+    bh_leaf_kernel<<<lb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, sk, na, bits,
+                                              b.cpos, b.breaker, b.ctr, b.leaf_sum, b.tile_range,
+                                              b.prun, b.sdep, b.node4, b.ncount,
+                                              deep ? b.nsum : nullptr, box);
+    if (deep) {
+        for (int d = (int)kDeepBits - 1; d >= 0; --d)
+            bh_level_kernel<<<pb, 256, 0, s->stream>>>(sk, b.sdep, b.cpos, b.prod, b.ctr, b.node4,
+                                                       b.ncount, b.nsum, d, box);
+        return hipGetLastError();
+    }
     bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
                                                   b.leaf_sum, b.sdep, b.cpos, b.node4,
                                                   b.ncount, b.top, box);

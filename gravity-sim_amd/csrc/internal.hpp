@@ -28,6 +28,10 @@ constexpr float kSimSize = 500.0f;
 constexpr uint32_t kTreeSize = 1024;
 constexpr uint32_t kTreeDepth = 10;
 constexpr uint32_t kKilledKey = 1u << 20;  // sorts after every 20-bit cell
+// 15-bit leaves (opt-in, beyond the reference): a 32768^2 leaf grid, depth 15,
+// 30-bit keys sorted in four 8-bit passes.
+constexpr uint32_t kDeepBits = 15;
+constexpr uint32_t kKilledKeyDeep = 1u << 30;  // sorts after every 30-bit cell
 constexpr uint32_t kTileDepth = 5;          // tile kernel: one depth-5 cell per workgroup
 constexpr uint32_t kTiles = 1u << (2 * kTileDepth);
 
@@ -35,7 +39,7 @@ struct BhCounters {
     uint32_t n_in;      // particles inside the domain (padding included)
     uint32_t n_cells;   // cell nodes (cells holding >= 2 particles)
     uint32_t n_nodes;   // n_in + n_cells = counts[0]
-    uint32_t n_leaves;  // occupied depth-10 cells
+    uint32_t n_leaves;  // occupied leaf cells (depth 10, or 15 with 15-bit leaves)
     unsigned long long visits;   // real-lane node visits of the traversal
     unsigned long long accepts;  // real-lane acceptances
     uint32_t next_group;         // work counter of the persistent traversal
@@ -49,9 +53,10 @@ struct TileCell {  // one cell of the dense tile/top reduction
 
 struct BhBuffers {
     size_t na = 0, cap = 0, sort_blocks = 0, prep_blocks = 0;
+    uint32_t bits = 10;              // leaf grid bits per axis the buffers are sized for
     uint32_t *keys[3] = {}, *idx[3] = {};
     uint32_t *hist = nullptr;        // [256 * sort_blocks] per-block digit counts
-    uint32_t *digit_total = nullptr; // [3 * 256] per-pass digit totals
+    uint32_t *digit_total = nullptr; // [4 * 256] per-pass digit totals
     uint32_t *breaker = nullptr;     // [cap] sorted position where cell node #o starts
     uint32_t *prun = nullptr;        // [cap] particle nodes left in the run at a slot
     uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
@@ -66,6 +71,8 @@ struct BhBuffers {
     TileCell *top = nullptr;         // [kTiles]
     float4 *node4 = nullptr;         // [cap]
     uint32_t *ncount = nullptr;      // [cap]
+    float4 *nsum = nullptr;          // [cap] 15-bit leaves only: a cell node's
+                                     // (sx, sy, sm, particle count as bits)
     uint32_t *sorted_src = nullptr;  // [Na]
     BhCounters *ctr = nullptr;
     uint32_t sorted_slot = 0;        // which keys/idx buffer holds the sorted result
@@ -165,8 +172,8 @@ hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
                              float2 *pos_out, float2 *vel_out);
 
 // Barnes-Hut (bh_kernels.hip)
-size_t bh_capacity(size_t na);
-hipError_t bh_alloc(BhBuffers *b, size_t na);
+size_t bh_capacity(size_t na, uint32_t bits);
+hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits);
 void bh_free(BhBuffers *b);
 hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_out,
                            const float2 *vel, float2 *vel_out);

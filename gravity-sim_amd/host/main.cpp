@@ -11,7 +11,9 @@
 // Additions: --seed= (generation seed; the reference seeds from the clock),
 // --group-width= (Barnes-Hut vote width W, 64 or 32), --device=, and
 // --adaptive-box (Barnes-Hut tree over the particles' bounding square instead
-// of the fixed [-500, 500) domain; beyond the reference, off by default).
+// of the fixed [-500, 500) domain; beyond the reference, off by default) and
+// --leaf-bits=15 (Barnes-Hut leaves on a 32768^2 grid at depth 15; beyond the
+// reference, 10 by default).
 #include <chrono>
 #include <cstdarg>
 #include <cstdint>
@@ -49,6 +51,8 @@ const char *const kHelp =
     "\t--device: The GPU to run on. Defaulted to 0.\n"
     "\t--adaptive-box: Barnes-Hut only: build the tree over the particles' bounding square\n"
     "\t\tinstead of the fixed [-500, 500) domain (no particle leaves it). Off by default.\n"
+    "\t--leaf-bits: Barnes-Hut only: 10 (a 1024x1024 leaf grid) or 15 (32768x32768, depth 15).\n"
+    "\t\tDefaulted to 10.\n"
     "Available options:\n"
     "\t--help: Displays the current message and exits the program.\n"
     "\t--log-detailed: Outputs non-crucial logs that might be useful for debugging or additional information.\n"
@@ -119,6 +123,7 @@ struct ProgramInfo {
     uint64_t maxSimulationCount = UINT64_MAX;
     uint64_t seed = (uint64_t)std::time(nullptr);
     uint32_t groupWidth = 64;
+    uint32_t leafBits = 10;
     int32_t device = 0;
     bool logDetailed = false, noGraphics = false, benchmark = false, adaptiveBox = false;
 };
@@ -165,6 +170,7 @@ int main(int argc, char **argv) {
         else if (starts(a, "--seed=", &v)) pi.seed = std::strtoull(v, nullptr, 10);
         else if (starts(a, "--group-width=", &v)) pi.groupWidth = (uint32_t)std::strtoul(v, nullptr, 10);
         else if (starts(a, "--device=", &v)) pi.device = (int32_t)std::strtol(v, nullptr, 10);
+        else if (starts(a, "--leaf-bits=", &v)) pi.leafBits = (uint32_t)std::strtoul(v, nullptr, 10);
         else if (!std::strcmp(a, "--log-detailed")) pi.logDetailed = true;
         else if (!std::strcmp(a, "--no-graphics")) pi.noGraphics = true;
         else if (!std::strcmp(a, "--benchmark")) pi.benchmark = true;
@@ -215,6 +221,8 @@ int main(int argc, char **argv) {
             bh.reset(new gsim::BarnesHutSimulation(ps.get(), pi.groupWidth, pi.device));
             if (pi.adaptiveBox)
                 GSIM_CHECK(gsim_set_tuning(bh->GetHandle(), GSIM_TUNE_BH_ADAPTIVE_BOX, 1));
+            if (pi.leafBits != 10)
+                GSIM_CHECK(gsim_set_tuning(bh->GetHandle(), GSIM_TUNE_BH_LEAF_BITS, pi.leafBits));
         }
         logger.Log(INFO, "Device library: %s", gsim_version());
 

@@ -83,7 +83,8 @@ typedef struct gsim_sim gsim_sim;
 
 /* Debug dump selectors for gsim_debug_dump (state of the most recent step). */
 typedef enum gsim_dump {
-    GSIM_DUMP_CELLS = 1,       /* u32[Na]  20-bit cell per particle, 0xffffffff = killed */
+    GSIM_DUMP_CELLS = 1,       /* u32[Na]  20-bit cell per particle (30-bit with 15-bit
+                                  leaves), 0xffffffff = killed                          */
     GSIM_DUMP_MORTON30 = 2,    /* u32[Na]  30-bit Morton code, 0xffffffff = killed     */
     GSIM_DUMP_SORTED_SRC = 3,  /* u32[Na]  particle index per sorted slot (Na = empty)  */
     GSIM_DUMP_COUNTS = 4,      /* u32[nodes] subtree sizes                              */
@@ -159,8 +160,9 @@ gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
                               uint32_t slices);
 
 /* Launch tuning that never changes results beyond the documented tolerance
- * (visit/accept counts and trees are unaffected by every knob), and one
- * opt-in option beyond the reference, GSIM_TUNE_BH_ADAPTIVE_BOX. */
+ * (visit/accept counts and trees are unaffected by every knob), and two
+ * opt-in options beyond the reference, GSIM_TUNE_BH_ADAPTIVE_BOX and
+ * GSIM_TUNE_BH_LEAF_BITS. */
 typedef enum gsim_tuning {
     GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
     GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
@@ -176,13 +178,23 @@ typedef enum gsim_tuning {
     GSIM_TUNE_BH_SPLIT = 7,           /* balanced traversal: groups whose last walk
                                          cost exceeds this multiple of the mean are
                                          split across 4 waves (0 = never; default 5)    */
-    GSIM_TUNE_BH_ADAPTIVE_BOX = 8     /* 0/1, beyond the reference and off by default:
+    GSIM_TUNE_BH_ADAPTIVE_BOX = 8,    /* 0/1, beyond the reference and off by default:
                                          each step's tree spans the bounding square of
                                          the particles (centre c, half-size h = extent
                                          x (0.5 + 2^-9)) instead of [-500, 500), so
                                          only non-finite positions are killed; cell
                                          radius (h*h)*2 / 4^d.  Changes the topology,
                                          so results differ from the reference's.        */
+    GSIM_TUNE_BH_LEAF_BITS = 9        /* 10 (the reference's 1024^2 leaf grid, the
+                                         default) or 15, beyond the reference: leaves
+                                         on a 32768^2 grid at depth 15, so a dense knot
+                                         is no longer one bucket that every group
+                                         streams whole.  The sums follow the same
+                                         quadrant rule; cells and leaves below depth 10
+                                         change the topology, so results differ from
+                                         the reference's.  Combines with the adaptive
+                                         box.  Reallocates the node arrays (synchronous;
+                                         they grow by up to 2.5 x Na entries).          */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 
