@@ -40,9 +40,10 @@ DS_SYM_ISSUED_PER_PAIR = 13.3125
 # x 2 + WRITE_SIZE x 1 (factors from a 256 MiB copy), 2 x 2,417 + 48,950 KiB;
 # profiles/ds50k_bh500k_counters.md
 DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
-# HBM bytes per bh_traverse_bal_kernel launch at 500,032 particles (same
-# counters): 2 x 67,720 + 34,130 KiB
-BH_TRAFFIC_BYTES = (2 * 67720 + 34130) * 1024
+# HBM bytes per bh_traverse64_kernel launch (the default traversal: one wave
+# per group, staged runs) at 500,032 particles (same counters):
+# 2 x 72,190 + 34,023 KiB
+BH_TRAFFIC_BYTES = (2 * 72190 + 34023) * 1024
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
@@ -151,7 +152,7 @@ def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = F
                    "adaptive_box": box},
         "roofline": {
             "bound": "valu",
-            "kernel": "bh_traverse_bal_kernel",
+            "kernel": "bh_traverse64_kernel",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",

@@ -522,7 +522,7 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
             GSIM_HIP(hipMemcpy(tmp.data(), s->bh.node4, sizeof(float4) * nodes, hipMemcpyDeviceToHost));
             float *f = (float *)buf;
             for (size_t k = 0; k < nodes; ++k) {
-                if (what == GSIM_DUMP_RADII) f[k] = tmp[k].w;
+                if (what == GSIM_DUMP_RADII) f[k] = tmp[k].w > 0.0f ? tmp[k].w : 0.0f;  // run length -> 0
                 else if (what == GSIM_DUMP_NODE_MASS) f[k] = tmp[k].z;
                 else { f[2 * k] = tmp[k].x; f[2 * k + 1] = tmp[k].y; }
             }

@@ -79,7 +79,7 @@ struct BatchDevice {
     TileCell *top = nullptr;
     // nodes of all trees
     float4 *node4 = nullptr;
-    uint32_t *ncount = nullptr, *prun = nullptr, *breaker = nullptr;
+    uint32_t *ncount = nullptr, *breaker = nullptr;
 };
 
 struct BatchSystem {

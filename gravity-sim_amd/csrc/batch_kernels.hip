@@ -342,7 +342,7 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
     BatchCounts *counts, const uint32_t *__restrict__ bh_prefix,
     const RankStats *__restrict__ stats, float4 *__restrict__ leaf_sum,
     uint32_t *__restrict__ tile_range, uint32_t *__restrict__ live_tiles,
-    uint32_t *__restrict__ prun) {
+    float4 *__restrict__ node4) {
     __shared__ float4 stage[4][64];
     const uint32_t n = counts->n_bh_particles, nl = counts->n_leaves, ncells = counts->n_cells;
     const uint32_t waves = gridDim.x * 4;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
         const uint32_t c1 = cpos[k] + q.ns;  // C(k+1)
         uint32_t run_end = c1 < ncells ? breaker[c1] : n;
         run_end = min(run_end, bh_prefix[q.rank] + stats[q.rank].n_in);
-        prun[k + c1] = run_end > k ? run_end - k : 1u;
+        set_run_length(node4, k + c1, run_end > k ? run_end - k : 1u);
     }
 }
 
@@ -496,11 +496,10 @@ __global__ __launch_bounds__(1024) void bb_top_kernel(const TileCell *__restrict
 // Barnes-Hut group g of one system: the walk of bh_kernels.hip over the
 // system's node range [root, end_slot) with its own constants.
 __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st, float4 *stage,
-                              const uint32_t *__restrict__ sorted_src,
+                              uint2 *queue, const uint32_t *__restrict__ sorted_src,
                               const float2 *__restrict__ pos, const float2 *__restrict__ vel,
                               const float4 *__restrict__ node4,
-                              const uint32_t *__restrict__ ncount,
-                              const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
+                              const uint32_t *__restrict__ ncount, float2 *__restrict__ pos_out,
                               float2 *__restrict__ vel_out, uint32_t node_cap,
                               unsigned long long &acc) {
     const uint32_t lane = threadIdx.x & 63;
@@ -517,7 +516,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     const uint32_t root = __builtin_amdgcn_readfirstlane(st.root);
     Partials P;
     unsigned long long vis = 0;
-    (void)walk_partials<true>(root, end, stage, node4, ncount, prun, px, py, real, nreal, d.eps2,
+    (void)walk_partials<true>(root, end, stage, queue, node4, ncount, px, py, real, nreal, d.eps2,
                               d.theta2, 0xfu, P, vis, acc);
     if (real) {
         float ax, ay;
@@ -577,9 +576,10 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
     const float2 *__restrict__ vel, const float *__restrict__ mass,
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
-    const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
-    float2 *__restrict__ vel_out, float2 *__restrict__ partial, uint32_t node_cap) {
+    float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ partial,
+    uint32_t node_cap) {
     __shared__ float4 stage[4][64];
+    __shared__ uint2 runq[4][kRunQueue];
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t nitems = counts->n_items;
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             // no particle inside the domain, or a spent budget (the keys
             // kernel has already written the system's output)
             if (!live || st.n_in == 0) continue;
-            bb_walk_group(item.y, d, st, stage[w], sorted_src, pos, vel, node4, ncount, prun,
+            bb_walk_group(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel, node4, ncount,
                           pos_out, vel_out, node_cap, acc);
         } else if (live) {
             bb_ds_tile(item.y, d, pos, mass, partial);
@@ -676,7 +676,7 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
                                           d.sorted_src, d.breaker, d.stats);
     bb_leaf_kernel<<<1024, 256, 0, st>>>(d.leaf_first, d.leaf_key, d.prod, sk, d.cpos, d.breaker,
                                          d.counts, d.bh_prefix, d.stats, d.leaf_sum,
-                                         d.tile_range, d.live_tiles, d.prun);
+                                         d.tile_range, d.live_tiles, d.node4);
     const uint32_t g_tile = (uint32_t)std::min<size_t>((size_t)b->max_systems * kTiles, 4096);
     bb_tile_kernel<<<g_tile, 256, 0, st>>>(d.tile_range, d.leaf_first, d.leaf_key, d.leaf_sum,
                                            d.sdep, d.cpos, d.counts, d.live_tiles, d.node4,
@@ -686,7 +686,7 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     const uint32_t waves = 1024u * b->waves_per_simd;
     bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.remaining, d.stats, d.counts, d.sorted_src,
                                                d.pos[in], d.vel[in], d.mass, d.node4, d.ncount,
-                                               d.prun, d.pos[out], d.vel[out], d.partial,
+                                               d.pos[out], d.vel[out], d.partial,
                                                (uint32_t)b->node_capacity);
     bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.remaining, d.ds_sys, d.ds_prefix,
                                                  d.counts, d.partial, d.pos[in], d.vel[in],

@@ -123,6 +123,15 @@ __device__ __forceinline__ void emit_cell(const TileCell &c, int d,
     node4[slot] = make_float4(px, py, c.sm, cell_radius(d, r0));
 }
 
+// A particle node's fourth word (the reference's radius 0): minus the number
+// of particle nodes from it to the next cell node, capped at 2^24 so it is
+// exact in FP32 (a longer run continues from the next slot, which holds its
+// own count).  Every reader takes w <= 0 as a particle node.
+__device__ __forceinline__ void set_run_length(float4 *__restrict__ node4, uint32_t slot,
+                                               uint32_t n) {
+    reinterpret_cast<float *>(node4)[4 * (size_t)slot + 3] = -(float)min(n, 1u << 24);
+}
+
 // Inclusive scan of one value across a 256-thread workgroup; returns the
 // inclusive prefix and writes the workgroup total.
 __device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t *total,
@@ -216,11 +225,13 @@ __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__r
 
 // ---- the walk of one W = 64 group --------------------------------------------
 // Summation order, the same in every W = 64 launch, single-system or batched:
-// the particle runs (leaf buckets) are cut into 64-node chunks numbered in
-// walk order across runs; chunk c is accumulated node by node into partial
-// c mod 4, every accepted cell into partial 0, and the acceleration is
-// ((p0 + p1) + p2) + p3.  One wave can compute all four partials, or four
-// waves one each (a dense-core group split across waves), with the same bits.
+// accepted cells are added in walk order into c; the particle runs (leaf
+// buckets) are cut into 64-node chunks numbered in walk order across runs;
+// chunk j is summed node by node from zero and its sum added to partial
+// j mod 4; the acceleration is (((c + p0) + p1) + p2) + p3.  One wave can
+// compute all of them, or four waves one partial each (a dense-core group
+// split across waves; the wave of p0 also adds the cells and hands over
+// c + p0), with the same bits.
 // Native 4-float vector: conditional copies stay in registers (HIP's float4
 // is a union-based struct whose selects the compiler may route through
 // private memory).
@@ -231,127 +242,180 @@ __device__ __forceinline__ f4v load_node(const float4 *__restrict__ node4, uint3
 }
 
 struct Partials {
+    float xc = 0.0f, yc = 0.0f;  // accepted cells
     float x0 = 0.0f, y0 = 0.0f, x1 = 0.0f, y1 = 0.0f, x2 = 0.0f, y2 = 0.0f, x3 = 0.0f, y3 = 0.0f;
 };
 
-// Runs f(ax, ay) on partial k (wave-uniform), each case on named registers so
-// the partials never leave them.
-template <class F>
-__device__ __forceinline__ void with_partial(Partials &P, uint32_t k, F &&f) {
-    switch (k) {
-        case 0: f(P.x0, P.y0); break;
-        case 1: f(P.x1, P.y1); break;
-        case 2: f(P.x2, P.y2); break;
-        default: f(P.x3, P.y3); break;
-    }
+// Adds a chunk's sum to partial k (wave-uniform).  Selects rather than a
+// switch: a switch over the fields lets the compiler index Partials as an
+// array in private memory.
+__device__ __forceinline__ void add_partial(Partials &P, uint32_t k, float cx, float cy) {
+    const float ax0 = P.x0 + cx, ay0 = P.y0 + cy, ax1 = P.x1 + cx, ay1 = P.y1 + cy;
+    const float ax2 = P.x2 + cx, ay2 = P.y2 + cy, ax3 = P.x3 + cx, ay3 = P.y3 + cy;
+    P.x0 = k == 0u ? ax0 : P.x0;
+    P.y0 = k == 0u ? ay0 : P.y0;
+    P.x1 = k == 1u ? ax1 : P.x1;
+    P.y1 = k == 1u ? ay1 : P.y1;
+    P.x2 = k == 2u ? ax2 : P.x2;
+    P.y2 = k == 2u ? ay2 : P.y2;
+    P.x3 = k == 3u ? ax3 : P.x3;
+    P.y3 = k == 3u ? ay3 : P.y3;
 }
 
 __device__ __forceinline__ void partial_sum(const Partials &P, float &ax, float &ay) {
-    ax = ((P.x0 + P.x1) + P.x2) + P.x3;
-    ay = ((P.y0 + P.y1) + P.y2) + P.y3;
+    ax = (((P.xc + P.x0) + P.x1) + P.x2) + P.x3;
+    ay = (((P.yc + P.y0) + P.y1) + P.y2) + P.y3;
+}
+
+// What the wave computing partial k of a split group hands over: c + p0 for
+// k = 0, else pk; added in order they give partial_sum's bits.
+__device__ __forceinline__ void split_share(const Partials &P, uint32_t k, float &ax, float &ay) {
+    switch (k) {
+        case 0: ax = P.xc + P.x0; ay = P.yc + P.y0; break;
+        case 1: ax = P.x1; ay = P.y1; break;
+        case 2: ax = P.x2; ay = P.y2; break;
+        default: ax = P.x3; ay = P.y3; break;
+    }
+}
+
+// Particle runs a wave queues in LDS before summing them (kStaged walk).
+constexpr uint32_t kRunQueue = 64;
+
+// The run chunks [queue[0], queue[qn]) of one queue batch, chunk numbers from
+// ch0: each chunk this wave owns ((parts >> (chunk & 3)) & 1) is loaded with
+// one coalesced load while the previous one is summed (across run
+// boundaries), staged in the wave's LDS as packed 12-byte (x, y, m) records,
+// summed node by node from zero and added to its partial.
+__device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, uint32_t ch0,
+                                              float4 *stage, const float4 *__restrict__ node4,
+                                              float px, float py, float eps2, uint32_t parts,
+                                              Partials &P) {
+    const uint32_t lane = threadIdx.x & 63;
+    float *st = reinterpret_cast<float *>(stage);
+    const f4v *st4 = reinterpret_cast<const f4v *>(stage);
+    // cursor: run e, chunk start base, run end rend, chunk number ch
+    uint32_t e = 0, base = 0, rend = 0, ch = ch0;
+    auto run_at = [&](uint32_t i, uint32_t &b, uint32_t &r) {
+        const uint2 q = queue[i];
+        b = __builtin_amdgcn_readfirstlane(q.x);
+        r = b + __builtin_amdgcn_readfirstlane(q.y);
+    };
+    auto owned = [&](uint32_t c) { return ((parts >> (c & 3u)) & 1u) != 0u; };
+    auto step = [&](uint32_t &ee, uint32_t &bb, uint32_t &rr, uint32_t &cc) {
+        bb += 64u;
+        ++cc;
+        if (bb >= rr) {
+            ++ee;
+            if (ee < qn) run_at(ee, bb, rr);
+        }
+    };
+    if (qn == 0) return;
+    run_at(0, base, rend);
+    while (e < qn && !owned(ch)) step(e, base, rend, ch);
+    f4v cur = load_node(node4, base + lane, e < qn && lane < min(64u, rend - base));
+    while (e < qn) {
+        const uint32_t cnt = min(64u, rend - base), k = ch & 3u;
+        uint32_t ne = e, nb = base, nr = rend, nch = ch;
+        do {
+            step(ne, nb, nr, nch);
+        } while (ne < qn && !owned(nch));
+        const f4v next = load_node(node4, nb + lane, ne < qn && lane < min(64u, nr - nb));
+        st[3 * lane] = cur.x;
+        st[3 * lane + 1] = cur.y;
+        st[3 * lane + 2] = cur.z;
+        __builtin_amdgcn_wave_barrier();
+        float cx = 0.0f, cy = 0.0f;
+        uint32_t j = 0;
+        for (; j + 8 <= cnt; j += 8) {
+            f4v r[6];
+#pragma unroll
+            for (int u = 0; u < 6; ++u) r[u] = st4[3 * (j / 4) + u];
+            const float *q = reinterpret_cast<const float *>(r);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                accumulate(q[3 * u], q[3 * u + 1], q[3 * u + 2], px, py, eps2, cx, cy);
+        }
+        for (; j < cnt; ++j)
+            accumulate(st[3 * j], st[3 * j + 1], st[3 * j + 2], px, py, eps2, cx, cy);
+        add_partial(P, k, cx, cy);
+        __builtin_amdgcn_wave_barrier();
+        cur = next;
+        e = ne;
+        base = nb;
+        rend = nr;
+        ch = nch;
+    }
 }
 
 // Walks the pre-order nodes [ind, end) for the group's lanes (ForceShader.comp
 // :111-141): a cell is accepted iff every real lane has radius <= (dx*dx +
 // dy*dy + eps2) * theta2, rounded as the shader writes it, then ind += count,
 // else ind++.  Particle nodes are always accepted, so a run of them is a plain
-// direct sum.  `parts` (bit k) selects the partials this wave computes; the
+// direct sum; a particle node's fourth word (the reference's radius 0) holds
+// minus the number of particle nodes from it to the next cell node, so one
+// 16-byte record tells a cell from a run and gives the run's length.  `parts` (bit k) selects the partials this wave computes; the
 // wave computing partial 0 adds the cells and counts the real-lane visits
-// and acceptances.  kStaged: run chunks come through the wave's LDS stage (one
-// coalesced load per chunk, broadcast reads; walking all four partials, the
-// next chunk of a run is loaded while the current one is summed); otherwise
-// through scalar loads.  Returns
-// the walk's VALU weight (a cell visit ~12 instructions, a particle ~10) in
-// units of 4, capped at 65535.
+// and acceptances.  kStaged: the walk decides first and queues the particle
+// runs (up to kRunQueue, `queue` is the wave's LDS) -- no decision depends on
+// a force -- then sums the queued runs with the next chunk always in flight
+// (sum_run_queue); the node record, its count and its run length are loaded
+// together, so each step of the walk is one round trip.  Otherwise runs are
+// summed as they come, through scalar loads.  Returns the walk's VALU weight
+// (a cell visit ~12 instructions, a particle ~10) in units of 4, capped at
+// 65535.
 template <bool kStaged>
 __device__ __forceinline__ uint32_t walk_partials(
-    uint32_t ind, uint32_t end, float4 *stage, const float4 *__restrict__ node4,
-    const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun, float px, float py,
-    bool real, uint32_t nreal, float eps2, float theta2, uint32_t parts, Partials &P,
-    unsigned long long &vis, unsigned long long &acc) {
+    uint32_t ind, uint32_t end, float4 *stage, uint2 *queue, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, float px, float py, bool real, uint32_t nreal,
+    float eps2, float theta2, uint32_t parts, Partials &P, unsigned long long &vis,
+    unsigned long long &acc) {
     const uint32_t lane = threadIdx.x & 63;
     const bool first = (parts & 1u) != 0;
-    uint32_t chunk = 0, cells = 0, runs = 0;
-    while (ind < end) {
+    uint32_t chunk = 0, cells = 0, runs = 0, qn = 0, qch = 0;
+    while (true) {
+      while (ind < end && qn < kRunQueue) {
         const float4 nd = node4[ind];
-        if (nd.w == 0.0f) {
-            uint32_t n = prun[ind];
-            n = __builtin_amdgcn_readfirstlane(min(max(n, 1u), end - ind));
-            const uint32_t rend = ind + n;
-            if (kStaged) {
-                // all partials: every chunk of the run is this wave's, so the
-                // next one is loaded while the current one is summed
-                // the stage holds (x, y, m) records packed at 12 bytes, so a
-                // broadcast read of 48 bytes brings four particles
-                const bool all = parts == 0xfu;
-                f4v next = load_node(node4, ind + lane, all && lane < min(64u, n));
-                float *st = reinterpret_cast<float *>(stage);
-                const f4v *st4 = reinterpret_cast<const f4v *>(stage);
-                for (uint32_t base = ind; base < rend; base += 64, ++chunk) {
-                    const uint32_t k = chunk & 3u;
-                    if (!((parts >> k) & 1u)) continue;
-                    const uint32_t cnt = min(64u, rend - base);
-                    f4v cur;
-                    if (all) {
-                        cur = next;
-                        const uint32_t nb = base + 64;
-                        const uint32_t ncnt = nb < rend ? min(64u, rend - nb) : 0u;
-                        next = load_node(node4, nb + lane, lane < ncnt);
-                    } else {
-                        cur = load_node(node4, base + lane, lane < cnt);
-                    }
-                    st[3 * lane] = cur.x;
-                    st[3 * lane + 1] = cur.y;
-                    st[3 * lane + 2] = cur.z;
-                    __builtin_amdgcn_wave_barrier();
-                    with_partial(P, k, [&](float &ax, float &ay) {
-                        uint32_t j = 0;
-                        for (; j + 8 <= cnt; j += 8) {
-                            f4v r[6];
-#pragma unroll
-                            for (int u = 0; u < 6; ++u) r[u] = st4[3 * (j / 4) + u];
-                            const float *q = reinterpret_cast<const float *>(r);
-#pragma unroll
-                            for (int u = 0; u < 8; ++u)
-                                accumulate(q[3 * u], q[3 * u + 1], q[3 * u + 2], px, py, eps2, ax, ay);
-                        }
-                        for (; j < cnt; ++j)
-                            accumulate(st[3 * j], st[3 * j + 1], st[3 * j + 2], px, py, eps2, ax, ay);
-                    });
-                    __builtin_amdgcn_wave_barrier();
-                }
-            } else {
-                for (uint32_t base = ind; base < rend; base += 64, ++chunk) {
-                    const uint32_t k = chunk & 3u;
-                    if (!((parts >> k) & 1u)) continue;
-                    const uint32_t ce = base + min(64u, rend - base);
-                    with_partial(P, k, [&](float &ax, float &ay) {
-                        uint32_t j = base;
-                        for (; j + 4 <= ce; j += 4) {
-                            float4 q[4];
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) q[u] = node4[j + u];
-#pragma unroll
-                            for (int u = 0; u < 4; ++u)
-                                accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, ax, ay);
-                        }
-                        for (; j < ce; ++j) {
-                            const float4 q = node4[j];
-                            accumulate(q.x, q.y, q.z, px, py, eps2, ax, ay);
-                        }
-                    });
-                }
-            }
+        const uint32_t cnt = ncount[ind];
+        if (nd.w <= 0.0f) {  // a particle node: w = -(particle nodes left in its run)
+            const uint32_t pr = (uint32_t)(-nd.w);
+            const uint32_t n = __builtin_amdgcn_readfirstlane(min(max(pr, 1u), end - ind));
             if (first) {
                 vis += (unsigned long long)n * nreal;
                 acc += (unsigned long long)n * nreal;
             }
             runs += n;
+            if (kStaged) {
+                if (lane == 0) queue[qn] = make_uint2(ind, n);
+                ++qn;
+                chunk += (n + 63u) >> 6;
+                ind += n;
+                continue;
+            }
+            const uint32_t rend = ind + n;
+            for (uint32_t base = ind; base < rend; base += 64, ++chunk) {
+                const uint32_t k = chunk & 3u;
+                if (!((parts >> k) & 1u)) continue;
+                const uint32_t ce = base + min(64u, rend - base);
+                float cx = 0.0f, cy = 0.0f;
+                uint32_t j = base;
+                for (; j + 4 <= ce; j += 4) {
+                    float4 q[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[u] = node4[j + u];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+                        accumulate(q[u].x, q[u].y, q[u].z, px, py, eps2, cx, cy);
+                }
+                for (; j < ce; ++j) {
+                    const float4 q = node4[j];
+                    accumulate(q.x, q.y, q.z, px, py, eps2, cx, cy);
+                }
+                add_partial(P, k, cx, cy);
+            }
             ind = rend;
             continue;
         }
         ++cells;
-        const uint32_t cnt = ncount[ind];
         const float dx = nd.x - px;
         const float dy = nd.y - py;
         const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
@@ -361,14 +425,22 @@ __device__ __forceinline__ uint32_t walk_partials(
             if (first) {
                 const float inv = __builtin_amdgcn_rsqf(r2);
                 const float sc = (nd.z * inv) * (inv * inv);
-                P.x0 = __builtin_fmaf(dx, sc, P.x0);
-                P.y0 = __builtin_fmaf(dy, sc, P.y0);
+                P.xc = __builtin_fmaf(dx, sc, P.xc);
+                P.yc = __builtin_fmaf(dy, sc, P.yc);
                 acc += nreal;
             }
             ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
         } else {
             ind = __builtin_amdgcn_readfirstlane(ind + 1u);
         }
+      }
+      if (kStaged && qn) {  // the one place queued runs are summed
+          __builtin_amdgcn_wave_barrier();
+          sum_run_queue(queue, qn, qch, stage, node4, px, py, eps2, parts, P);
+          qn = 0;
+          qch = chunk;
+      }
+      if (ind >= end) break;
     }
     return min((cells * 12u + runs * 10u) >> 2, 65535u);
 }

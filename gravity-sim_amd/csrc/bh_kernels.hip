@@ -415,8 +415,8 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
     const float4 *__restrict__ prod, const uint32_t *__restrict__ keys, uint32_t na, int bits,
     const uint32_t *__restrict__ cpos, const uint32_t *__restrict__ breaker,
     const BhCounters *__restrict__ ctr, float4 *__restrict__ leaf_sum,
-    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ prun,
-    const int8_t *__restrict__ sdep, float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
+    uint32_t *__restrict__ tile_range, const int8_t *__restrict__ sdep,
+    float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
     float4 *__restrict__ nsum, const float *__restrict__ box) {
     __shared__ float4 stage[4][64];
     const uint32_t n_in = ctr->n_in, nl = ctr->n_leaves, ncells = ctr->n_cells;
@@ -472,7 +472,7 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
         const PosInfo q = pos_info(keys, na, k, bits);
         const uint32_t c1 = cpos[k] + q.ns;  // C(k+1)
         const uint32_t run_end = c1 < ncells ? breaker[c1] : n_in;
-        prun[k + c1] = run_end > k ? run_end - k : 1u;
+        set_run_length(node4, k + c1, run_end > k ? run_end - k : 1u);
     }
 }
 
@@ -638,10 +638,10 @@ __global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict
 // One group's walk (all four partials) and integrate.
 template <bool kDebugAccel, bool kStaged>
 __device__ __forceinline__ void walk_group(
-    uint32_t g, uint32_t total, float4 *stage, const uint32_t *__restrict__ sorted_src,
+    uint32_t g, uint32_t total, float4 *stage, uint2 *queue, const uint32_t *__restrict__ sorted_src,
     const float2 *__restrict__ pos, const float2 *__restrict__ vel,
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
-    const uint32_t *__restrict__ prun, float2 *__restrict__ pos_out,
+    float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
     float theta2, float kv, float kp, unsigned long long &vis, unsigned long long &acc,
     uint32_t *__restrict__ gcost = nullptr) {
@@ -656,8 +656,8 @@ __device__ __forceinline__ void walk_group(
     }
     const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
     Partials P;
-    const uint32_t cost = walk_partials<kStaged>(0u, total, stage, node4, ncount, prun, px, py, real,
-                                                 nreal, eps2, theta2, 0xfu, P, vis, acc);
+    const uint32_t cost = walk_partials<kStaged>(0u, total, stage, queue, node4, ncount, px, py,
+                                                 real, nreal, eps2, theta2, 0xfu, P, vis, acc);
     if (real) {
         float ax, ay;
         partial_sum(P, ax, ay);
@@ -716,16 +716,19 @@ __global__ __launch_bounds__(1024) void bh_cost_scan_kernel(const uint32_t *__re
 constexpr uint32_t kSplitSlots = 16;
 constexpr uint32_t kSegList = 1024;
 
-template <bool kDebugAccel>
+// kSplit = false: every unit is a whole group (one specialised walk in the
+// kernel, all four partials on one wave).
+template <bool kDebugAccel, bool kSplit>
 __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
     const float2 *__restrict__ vel, const float4 *__restrict__ node4,
-    const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun,
+    const uint32_t *__restrict__ ncount,
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, const uint32_t *__restrict__ gpre, uint32_t *__restrict__ gcost,
     uint32_t split_x) {
     __shared__ float4 stage[16][64];
+    __shared__ uint2 runq[16][kRunQueue];
     __shared__ uint32_t seg[4];            // first group, end, next unit, list fill
     __shared__ uint32_t list[kSegList];    // split groups first, then the others
     __shared__ float2 hpart[kSplitSlots][4][64];
@@ -756,7 +759,7 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     const uint32_t count = hi - lo, listed = min(count, kSegList);
     const uint32_t i = threadIdx.x;  // one listed group per thread (kSegList == blockDim)
     bool split = false;
-    if (split_x && i < listed &&
+    if (kSplit && split_x && i < listed &&
         (unsigned long long)(gcost[lo + i] + 1u) * groups > split_x * total_cost) {
         const uint32_t slot = atomicAdd(&seg[3], 1u);
         if (slot < kSplitSlots) {
@@ -779,19 +782,18 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         if (lane == 0) u = atomicAdd(&seg[2], 1u);
         u = __builtin_amdgcn_readfirstlane(__shfl(u, 0, 64));
         if (u >= units) break;
-        if (u >= 4 * nsplit) {  // a whole group
-            const uint32_t v = u - 4 * nsplit + nsplit;
-            const uint32_t g = __builtin_amdgcn_readfirstlane(v < listed ? list[v] : lo + v);
-            walk_group<kDebugAccel, true>(g, total_nodes, stage[w], sorted_src, pos, vel, node4,
-                                          ncount, prun, pos_out, vel_out, acc_dbg, na, eps2,
-                                          theta2, kv, kp, vis, acc, gcost);
-            continue;
-        }
-        // one partial of a split group
+        // a whole group (all partials), or one partial of a split group; one
+        // walk call for both keeps a single copy of the walk in the kernel
+        const bool whole = !kSplit || u >= 4 * nsplit;
+        const uint32_t v = u - 4 * nsplit + nsplit;
         const uint32_t h = u >> 2, part = u & 3u;
-        const uint32_t g = __builtin_amdgcn_readfirstlane(list[h]);
+        uint32_t g;
+        if (whole) g = v < listed ? list[v] : lo + v;
+        else g = list[h];  // h < nsplit <= kSplitSlots
+        g = __builtin_amdgcn_readfirstlane(g);
+        const uint32_t parts = !kSplit ? 0xfu : whole ? 0xfu : 1u << part;
         const uint32_t src = sorted_src[g * 64 + lane];
-        const bool real = src < na;
+        const bool real = src < na;  // ForceShader.comp:80-92
         float px = 0.0f, py = 0.0f;
         if (real) {
             const float2 p = pos[src];
@@ -800,36 +802,32 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         }
         const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
         Partials P;
-        const uint32_t cost = walk_partials<true>(0u, total_nodes, stage[w], node4, ncount, prun,
-                                                  px, py, real, nreal, eps2, theta2, 1u << part,
-                                                  P, vis, acc);
+        const uint32_t cost = walk_partials<true>(0u, total_nodes, stage[w], runq[w], node4, ncount,
+                                                  px, py, real, nreal, eps2, theta2, parts, P, vis,
+                                                  acc);
         float ax = 0.0f, ay = 0.0f;
-        with_partial(P, part, [&](float &px_, float &py_) {
-            ax = px_;
-            ay = py_;
-        });
-        hpart[h][part][lane] = make_float2(ax, ay);
-        if (part == 0 && lane == 0) gcost[g] = cost;
-        __threadfence_block();
-        uint32_t done = 0;
-        if (lane == 0) done = atomicAdd(&hdone[h], 1u);
-        done = __builtin_amdgcn_readfirstlane(__shfl(done, 0, 64));
-        if (done != 3u) continue;
-        __threadfence_block();  // the last of the four adds the partials in order
-        Partials Q;
-        Q.x0 = hpart[h][0][lane].x;
-        Q.y0 = hpart[h][0][lane].y;
-        Q.x1 = hpart[h][1][lane].x;
-        Q.y1 = hpart[h][1][lane].y;
-        Q.x2 = hpart[h][2][lane].x;
-        Q.y2 = hpart[h][2][lane].y;
-        Q.x3 = hpart[h][3][lane].x;
-        Q.y3 = hpart[h][3][lane].y;
+        if (whole) {
+            partial_sum(P, ax, ay);
+            if (lane == 0) gcost[g] = cost;
+        } else {
+            split_share(P, part, ax, ay);
+            hpart[h][part][lane] = make_float2(ax, ay);
+            if (part == 0 && lane == 0) gcost[g] = cost;
+            __threadfence_block();
+            uint32_t done = 0;
+            if (lane == 0) done = atomicAdd(&hdone[h], 1u);
+            done = __builtin_amdgcn_readfirstlane(__shfl(done, 0, 64));
+            if (done != 3u) continue;
+            __threadfence_block();  // the last of the four adds the shares in order
+            ax = ((hpart[h][0][lane].x + hpart[h][1][lane].x) + hpart[h][2][lane].x) +
+                 hpart[h][3][lane].x;
+            ay = ((hpart[h][0][lane].y + hpart[h][1][lane].y) + hpart[h][2][lane].y) +
+                 hpart[h][3][lane].y;
+        }
         if (real) {
-            partial_sum(Q, ax, ay);
-            const float2 v = vel[src];
-            const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
-            pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+            const float2 vv = vel[src];
+            const float nvx = vv.x + ax * kv, nvy = vv.y + ay * kv;
+            pos_out[src] = make_float2(px + (vv.x + nvx) * kp, py + (vv.y + nvy) * kp);
             vel_out[src] = make_float2(nvx, nvy);
             if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
         }
@@ -847,11 +845,12 @@ template <bool kDebugAccel, bool kStaged, bool kPersistent>
 __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
     const float2 *__restrict__ vel, const float4 *__restrict__ node4,
-    const uint32_t *__restrict__ ncount, const uint32_t *__restrict__ prun,
+    const uint32_t *__restrict__ ncount,
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
+    __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t groups = na / 64;
     // node count is bounded by the node arrays; every iteration advances ind by
@@ -859,22 +858,23 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
     unsigned long long vis = 0, acc = 0;
     float4 *st = stage[kStaged ? w : 0];
+    uint2 *rq = runq[kStaged ? w : 0];
     if (kPersistent) {
         while (true) {
             uint32_t g = 0;
             if (lane == 0) g = atomicAdd(&ctr->next_group, 1u);
             g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, 64));
             if (g >= groups) break;
-            walk_group<kDebugAccel, kStaged>(g, total, st, sorted_src, pos, vel, node4, ncount,
-                                             prun, pos_out, vel_out, acc_dbg, na, eps2, theta2,
-                                             kv, kp, vis, acc);
+            walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4,
+                                             ncount, pos_out, vel_out, acc_dbg, na, eps2,
+                                             theta2, kv, kp, vis, acc);
         }
     } else {
         const uint32_t g = blockIdx.x * 4 + w;
         if (g >= groups) return;  // whole wave beyond the last group
-        walk_group<kDebugAccel, kStaged>(g, total, st, sorted_src, pos, vel, node4, ncount, prun,
-                                         pos_out, vel_out, acc_dbg, na, eps2, theta2, kv, kp,
-                                         vis, acc);
+        walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
+                                         pos_out, vel_out, acc_dbg, na, eps2, theta2, kv,
+                                         kp, vis, acc);
     }
     if (lane == 0 && vis) {
         atomicAdd(&ctr->visits, vis);
@@ -997,7 +997,6 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->hist, 256 * b->sort_blocks);
     GSIM_ALLOC(b->digit_total, 4 * 256);
     GSIM_ALLOC(b->breaker, b->cap);
-    GSIM_ALLOC(b->prun, b->cap);
     GSIM_ALLOC(b->block_cells, b->prep_blocks);
     GSIM_ALLOC(b->block_leaves, b->prep_blocks);
     GSIM_ALLOC(b->sdep, na + 1);
@@ -1030,7 +1029,7 @@ void bh_free(BhBuffers *b) {
         (void)hipFree(b->keys[k]);
         (void)hipFree(b->idx[k]);
     }
-    void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->prun, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
+    void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box,
                     b->box_part};
@@ -1087,7 +1086,7 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     const uint32_t lb = pb < kLeafBlocks ? pb : kLeafBlocks;
     bh_leaf_kernel<<<lb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, sk, na, bits,
                                               b.cpos, b.breaker, b.ctr, b.leaf_sum, b.tile_range,
-                                              b.prun, b.sdep, b.node4, b.ncount,
+                                              b.sdep, b.node4, b.ncount,
                                               deep ? b.nsum : nullptr, box);
     if (deep) {
         for (int d = (int)kDeepBits - 1; d >= 0; --d)
@@ -1114,16 +1113,13 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         if (s->trav_mode & 4u) {
             bh_cost_scan_kernel<<<1, 1024, 0, s->stream>>>(b.gcost, groups, b.gpre);
             const uint32_t P = s->trav_segments;
-            if (dbg)
-                bh_traverse_bal_kernel<true><<<P, 1024, 0, s->stream>>>(
-                    b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc,
-                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost,
-                    s->trav_split);
-            else
-                bh_traverse_bal_kernel<false><<<P, 1024, 0, s->stream>>>(
-                    b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc,
-                    na, (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost,
-                    s->trav_split);
+#define GSIM_BAL(D, S)                                                                          \
+    bh_traverse_bal_kernel<D, S><<<P, 1024, 0, s->stream>>>(                                   \
+        b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,          \
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost, s->trav_split)
+            if (dbg) { if (s->trav_split) GSIM_BAL(true, true); else GSIM_BAL(true, false); }
+            else { if (s->trav_split) GSIM_BAL(false, true); else GSIM_BAL(false, false); }
+#undef GSIM_BAL
             return hipGetLastError();
         }
         const bool persistent = (s->trav_mode & 1u) != 0, staged = (s->trav_mode & 2u) != 0;
@@ -1136,7 +1132,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         }
 #define GSIM_TRAV(D, S, P)                                                                 \
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
-        b.sorted_src, pos, vel, b.node4, b.ncount, b.prun, b.ctr, pos_out, vel_out, acc, na, \
+        b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
         (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }

@@ -7,8 +7,9 @@
 //                   partials float2[(sum over groups of B - 16g)*64]
 //   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
 //                   cpos u32[Na], prod float4[Na], leaf records, tile results,
-//                   breaker u32[cap], prun u32[cap] (particle-run lengths),
-//                   node4 float4[cap] {x, y, mass, radius}, ncount u32[cap],
+//                   breaker u32[cap], node4 float4[cap] {x, y, mass, radius}
+//                   (a particle node's radius word holds minus its run length),
+//                   ncount u32[cap],
 //                   sorted_src u32[Na], device counters
 #pragma once
 
@@ -58,7 +59,6 @@ struct BhBuffers {
     uint32_t *hist = nullptr;        // [256 * sort_blocks] per-block digit counts
     uint32_t *digit_total = nullptr; // [4 * 256] per-pass digit totals
     uint32_t *breaker = nullptr;     // [cap] sorted position where cell node #o starts
-    uint32_t *prun = nullptr;        // [cap] particle nodes left in the run at a slot
     uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
     uint32_t *block_leaves = nullptr;
     int8_t *sdep = nullptr;          // [Na] shared depth with the previous sorted key
@@ -132,8 +132,9 @@ struct gsim_sim {
     float eps2 = 0, theta2 = 0, dt = 0, kv = 0, kp = 0;
     uint32_t W = 64;
     uint32_t trav_waves_per_simd = 8;  // persistent traversal width
-    uint32_t trav_mode = 6;            // bit 0 persistent, bit 1 LDS-staged runs,
-                                       // bit 2 cost-balanced segments (staged)
+    uint32_t trav_mode = 2;            // bit 0 persistent, bit 1 LDS-staged runs,
+                                       // bit 2 cost-balanced segments (staged);
+                                       // default 2: one wave per group, staged
     uint32_t trav_segments = 512;      // balanced traversal: workgroups (2 per CU)
     uint32_t trav_split = 5;           // balanced traversal: split groups above this x mean cost
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain

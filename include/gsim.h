@@ -166,9 +166,10 @@ gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
 typedef enum gsim_tuning {
     GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
     GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
-    GSIM_TUNE_BH_TRAVERSAL = 3,       /* bit 0 persistent groups, bit 1 LDS-staged runs;
-                                         6 = cost-balanced contiguous segments (the
-                                         default)                                       */
+    GSIM_TUNE_BH_TRAVERSAL = 3,       /* bit 0 persistent groups, bit 1 LDS-staged runs
+                                         (2, one wave per group with staged runs, is the
+                                         default); 6 = cost-balanced contiguous
+                                         segments, with dense-core groups split          */
     GSIM_TUNE_DS_SYMMETRIC = 4,       /* 0/1: each pair once (Newton's third law); the
                                          default for 16,384 <= Na <= 262,144           */
     GSIM_TUNE_DS_SYM_COLS = 5,        /* symmetric kernel: 1 or 2 column blocks per

@@ -53,9 +53,11 @@ def test_log_file_and_generation_warning(tmp_path):
     assert "[WARNING]     A particle input file was provided" in log.read_text()
 
 
-def _final_state(algo, pin, steps, extra=()):
+def _final_state(algo, pin, steps, extra=(), tunings=()):
     p = gsim.load_particles(str(pin))
     with gsim.Simulation(p, algo, **dict(extra)) as s:
+        for knob, value in tunings:
+            s.set_tuning(knob, value)
         s.step(steps)
         return s.read()
 
@@ -83,3 +85,27 @@ def test_cli_run_matches_library(tmp_path, algo):
     assert np.allclose(got.pos, ref.pos, atol=1e-6, rtol=0)
     assert np.allclose(got.vel, ref.vel, atol=1e-6, rtol=0)
     assert np.allclose(got.mass, ref.mass, atol=1e-6, rtol=0)
+
+
+@pytest.mark.gpu
+def test_cli_tree_options_match_library(tmp_path):
+    """--leaf-bits=15 --adaptive-box (beyond the reference): the CLI's final
+    state equals the library's with the same two options; a leaf-bit count
+    other than 10 or 15 is refused."""
+    pin = tmp_path / "in.txt"
+    pout = tmp_path / "out.txt"
+    gsim.save_particles(str(pin), gsim.generate("galaxy-collision", 3000, 700.0, 1.0, 10.0,
+                                                seed=8))
+    r = run(f"--particles-in={pin}", f"--particles-out={pout}", "--simulation-algorithm=barnes-hut",
+            "--simulation-count=120", "--no-graphics", "--leaf-bits=15", "--adaptive-box")
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    bad = run(f"--particles-in={pin}", "--simulation-algorithm=barnes-hut",
+              "--simulation-count=1", "--no-graphics", "--leaf-bits=12")
+    assert bad.returncode != 0 and "leaf bits must be 10 or 15" in bad.stdout + bad.stderr
+    got = gsim.load_particles(str(pout))
+    ref = _final_state("barnes-hut", pin, 120, tunings=((gsim.TUNE_BH_LEAF_BITS, 15),
+                                                       (gsim.TUNE_BH_ADAPTIVE_BOX, 1)))
+    assert np.array_equal(ref.mass, gsim.load_particles(str(pin)).mass)  # nothing killed
+    assert np.allclose(got.pos, ref.pos, atol=1e-6, rtol=0)
+    assert np.allclose(got.vel, ref.vel, atol=1e-6, rtol=0)
