@@ -674,7 +674,7 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
                                           d.block_cells, d.block_leaves, d.sdep, d.cpos, d.prod,
                                           d.leaf_first, d.leaf_key, d.node4, d.ncount,
                                           d.sorted_src, d.breaker, d.stats);
-    bb_leaf_kernel<<<1024, 256, 0, st>>>(d.leaf_first, d.leaf_key, d.prod, sk, d.cpos, d.breaker,
+    bb_leaf_kernel<<<4096, 256, 0, st>>>(d.leaf_first, d.leaf_key, d.prod, sk, d.cpos, d.breaker,
                                          d.counts, d.bh_prefix, d.stats, d.leaf_sum,
                                          d.tile_range, d.live_tiles, d.node4);
     const uint32_t g_tile = (uint32_t)std::min<size_t>((size_t)b->max_systems * kTiles, 4096);

@@ -185,38 +185,48 @@ __device__ __forceinline__ float4 lane_leaf_sum(const float4 *__restrict__ prod,
 }
 
 // A larger leaf [f, e) (wave-uniform bounds) by the whole wave: 64 products
-// per coalesced load into the wave's LDS stage, the next chunk in flight while
-// the current one is summed (every lane computes the same running sum).
+// per coalesced load (the next chunk in flight while the current one is
+// summed) staged in the wave's LDS as packed 12-byte (m*x, m*y, m) records,
+// so three 16-byte broadcast reads bring four; a full chunk is unrolled so the
+// reads run ahead of the three dependent add chains, which set the pace
+// (every lane computes the same running sum).
 __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__restrict__ prod,
                                                 uint32_t f, uint32_t e, uint32_t lane) {
     typedef float v4 __attribute__((ext_vector_type(4)));  // stays in registers
     const v4 *vp = reinterpret_cast<const v4 *>(prod);
     const v4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    float *st = reinterpret_cast<float *>(stage);
+    const v4 *st4 = reinterpret_cast<const v4 *>(stage);
     float sx = 0.0f, sy = 0.0f, sm = 0.0f;
     v4 next = f + lane < e ? vp[f + lane] : zero;
     for (uint32_t base = f; base < e; base += 64) {
         const uint32_t cnt = min(64u, e - base);
-        reinterpret_cast<v4 *>(stage)[lane] = next;
+        st[3 * lane] = next.x;
+        st[3 * lane + 1] = next.y;
+        st[3 * lane + 2] = next.z;
         __builtin_amdgcn_wave_barrier();
         const uint32_t nb = base + 64 + lane;
         next = nb < e ? vp[nb] : zero;
-        uint32_t k = 0;
-        for (; k + 8 <= cnt; k += 8) {
-            float4 q[8];
+        if (cnt == 64u) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) q[u] = stage[k + u];
+            for (int k = 0; k < 64; k += 16) {
+                v4 r[12];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                sx = sx + q[u].x;
-                sy = sy + q[u].y;
-                sm = sm + q[u].z;
+                for (int u = 0; u < 12; ++u) r[u] = st4[3 * (k / 4) + u];
+                const float *q = reinterpret_cast<const float *>(r);
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    sx = sx + q[3 * u];
+                    sy = sy + q[3 * u + 1];
+                    sm = sm + q[3 * u + 2];
+                }
             }
-        }
-        for (; k < cnt; ++k) {
-            const float4 q = stage[k];
-            sx = sx + q.x;
-            sy = sy + q.y;
-            sm = sm + q.z;
+        } else {
+            for (uint32_t k = 0; k < cnt; ++k) {
+                sx = sx + st[3 * k];
+                sy = sy + st[3 * k + 1];
+                sm = sm + st[3 * k + 2];
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }

@@ -385,7 +385,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
 
 // ---- 5. leaf sums and particle runs ------------------------------------------
 
-constexpr uint32_t kLeafBlocks = 1024;
+constexpr uint32_t kLeafBlocks = 4096;
 constexpr uint32_t kBoxParts = 256;  // blocks of the box reduction
 
 // Phase A, one wave per leaf: the sum runs over the leaf's particles in

@@ -275,6 +275,36 @@ def test_bh_tree_bit_exact(case):
     assert np.array_equal(t["node_pos"], ot["node_pos"])
 
 
+def test_bh_leaf_sizes_at_chunk_boundaries():
+    """Leaves of 24 (the largest summed by one lane), 25, 63, 64, 65, 127,
+    128, 129, 200 and 1,000 particles: the wave-summed leaves mix full 64-node
+    chunks and a ragged tail, and every sum must be the oracle's, bit for
+    bit."""
+    rng = np.random.default_rng(209)
+    sizes = [24, 25, 63, 64, 65, 127, 128, 129, 200, 1000]
+    pos, mass = [], []
+    w = 1000.0 / 1024.0  # one depth-10 cell per leaf, well inside its edges
+    for i, k in enumerate(sizes):
+        corner = np.array([-500.0 + (100 + 80 * i) * w, -500.0 + 600 * w])
+        pos.append((corner + rng.uniform(0.1, 0.85, size=(k, 2)) * w).astype(np.float32))
+        mass.append(rng.uniform(1, 10, k).astype(np.float32))
+    pos = np.vstack(pos).astype(np.float32)
+    mass = np.concatenate(mass)
+    p = gsim.Particles(pos, np.zeros_like(pos), mass)
+    s = _sim(p, "barnes-hut")
+    s.step(1)
+    t = s.tree()
+    s.close()
+    pp = p.padded()
+    ot, nn, *_ = O.bh_build(pp.pos, pp.mass)
+    assert t["total"] == nn
+    for k in ("sorted_src", "counts", "radii", "node_mass", "node_pos"):
+        assert np.array_equal(t[k], ot[k]), k
+    # every leaf is one bucket: its cell node sits right before its particles
+    leaves = [i for i in range(nn) if t["radii"][i] == np.float32(500000.0 / 4.0**10)]
+    assert sorted(int(t["counts"][i]) - 1 for i in leaves) == sorted(sizes + [len(pp.mass) - len(mass)])
+
+
 def test_bh_degenerate_trees():
     cases = {
         "single": (np.array([[1.0, 2.0]], np.float32), np.array([3.0], np.float32)),
