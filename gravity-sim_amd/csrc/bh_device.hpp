@@ -172,14 +172,25 @@ __device__ __forceinline__ void wave_scan2(uint32_t &a, uint32_t &b) {
 // Leaves of at most kSmallLeaf particles are summed by one lane.
 constexpr uint32_t kSmallLeaf = 24;
 
+// Loads go out eight at a time (one round trip per eight products); the
+// additions keep the sequential order.
 __device__ __forceinline__ float4 lane_leaf_sum(const float4 *__restrict__ prod, uint32_t f,
                                                 uint32_t e) {
+    typedef float v4 __attribute__((ext_vector_type(4)));  // stays in registers
+    const v4 *vp = reinterpret_cast<const v4 *>(prod);
+    const v4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     float sx = 0.0f, sy = 0.0f, sm = 0.0f;
-    for (uint32_t k = f; k < e; ++k) {
-        const float4 q = prod[k];
-        sx = sx + q.x;
-        sy = sy + q.y;
-        sm = sm + q.z;
+    for (uint32_t k = f; k < e; k += 8) {
+        v4 q[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) q[u] = k + u < e ? vp[k + u] : zero;
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const bool in = k + u < e;
+            sx = in ? sx + q[u].x : sx;
+            sy = in ? sy + q[u].y : sy;
+            sm = in ? sm + q[u].z : sm;
+        }
     }
     return make_float4(sx, sy, sm, 0.0f);
 }
