@@ -305,10 +305,22 @@ def bench_serve(seconds: float, budget_gb: float = 200.0, max_active: int = 0,
     }
 
 
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(budget_s: float = 12.0) -> dict:
     """The oracle's FP32 direct sum (scalar C, one thread per core slice) on
     the same 50k workload, timed on this host: a bounded sample of whole
-    force passes over all 50,000 x 50,000 pairs."""
+    force passes over all 50,000 x 50,000 pairs on up to 16 threads, and one
+    pass on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, used here only as the CPU baseline
 
@@ -326,10 +338,36 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
         oracle.ds_accel_f32(p.pos, p.mass, eps2, nthreads=threads)
     el = time.perf_counter() - t0
     pairs = float(DS_N) * DS_N * reps
+    t0 = time.perf_counter()
+    oracle.ds_accel_f32(p.pos, p.mass, eps2, nthreads=1)
+    el1 = time.perf_counter() - t0
     return {"value": pairs / el / 1e9, "unit": "Ginteractions/s", "cores": threads,
             "kind": "port",
+            "one_core_value": float(DS_N) * DS_N / el1 / 1e9,
+            "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
             "sample": f"{reps + 1} full 50,000x50,000 FP32 force passes of the oracle "
-                      f"({threads} threads), integrate excluded"}
+                      f"({threads} threads) and one on 1 thread, integrate excluded"}
+
+
+def cpu_baseline_bh(steps: int = 3) -> dict:
+    """The oracle's whole Barnes-Hut step (dense-pyramid build, group walk,
+    integrate) on the same 500k workload, on up to 16 threads: ms per step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker, used here only as the CPU baseline
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    p = gsim.generate("two-exponential-disks", BH_N, 33.0, 1.0, 10.0, 1.0, seed=3).padded()
+    eps2 = oracle.square_f32(0.2)
+    theta2 = oracle.square_f32(1.0)
+    pos, vel, mass = p.pos, p.vel, p.mass
+    pos, vel, mass, _, _ = oracle.bh_step(pos, vel, mass, 1.0, 1e-3, eps2, theta2, 64, threads)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pos, vel, mass, _, _ = oracle.bh_step(pos, vel, mass, 1.0, 1e-3, eps2, theta2, 64, threads)
+    el = time.perf_counter() - t0
+    return {"value": el * 1e3 / steps, "unit": "ms/step", "cores": threads, "kind": "port",
+            "sample": f"{steps + 1} whole Barnes-Hut steps of the oracle at 500,032 "
+                      f"particles ({threads} threads)"}
 
 
 def copy_calibration(mib: int = 256, reps: int = 3) -> None:
@@ -399,7 +437,12 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         cpu = cpu_baseline()
-        log(f"  {cpu['value']:.2f} Ginteractions/s on {cpu['cores']} threads")
+        log(f"  {cpu['value']:.2f} Ginteractions/s on {cpu['cores']} threads, "
+            f"{cpu['one_core_value']:.3f} on one")
+        if bh is not None:
+            bh["cpu_baseline"] = cpu_baseline_bh()
+            log(f"  Barnes-Hut 500k: {bh['cpu_baseline']['value']:.1f} ms/step on "
+                f"{bh['cpu_baseline']['cores']} threads")
     ref_ginter = DS_N * DS_N / (REF_DS_MS_50K * 1e-3) / 1e9
     line = {
         "metric": "direct-sum Ginteractions/s @50k",
