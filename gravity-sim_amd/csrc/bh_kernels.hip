@@ -191,13 +191,14 @@ __global__ __launch_bounds__(256) void radix_scan_kernel(uint32_t *__restrict__ 
     }
 }
 
-// Exclusive scan of one array per workgroup (blockIdx.x selects a or b), in
-// place; each thread scans a contiguous chunk.
+// Exclusive scan of one array per workgroup (blockIdx.x selects a, b or c),
+// in place; each thread scans a contiguous chunk.
 __global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t *__restrict__ a,
                                                                uint32_t *__restrict__ b,
+                                                               uint32_t *__restrict__ c,
                                                                uint32_t count) {
     __shared__ uint32_t part[1024];
-    uint32_t *data = blockIdx.x == 0 ? a : b;
+    uint32_t *data = blockIdx.x == 0 ? a : blockIdx.x == 1 ? b : c;
     if (!data) return;
     const uint32_t t = threadIdx.x;
     const uint32_t per = (count + 1023) / 1024;
@@ -277,13 +278,14 @@ struct PosInfo {
     uint32_t ns, lf;
     bool inside;
     uint32_t key;
+    bool g10;  // first particle of an occupied depth-10 cell (15-bit leaves)
 };
 
 // bits: 10 (the reference's grid) or 15 (15-bit leaves); keys at or above
 // 1 << 2*bits are killed particles.
 __device__ __forceinline__ PosInfo pos_info(const uint32_t *__restrict__ keys, uint32_t na,
                                             uint32_t k, int bits) {
-    PosInfo q{-1, -1, 0u, 0u, false, 0u};
+    PosInfo q{-1, -1, 0u, 0u, false, 0u, false};
     if (k >= na) return q;
     const uint32_t killed = 1u << (2 * bits);
     q.key = keys[k];
@@ -296,26 +298,33 @@ __device__ __forceinline__ PosInfo pos_info(const uint32_t *__restrict__ keys, u
     }
     q.ns = q.sk1 > q.sk ? (uint32_t)(q.sk1 - q.sk) : 0u;
     q.lf = q.sk < bits ? 1u : 0u;
+    q.g10 = q.sk < 10;
     return q;
 }
 
+// block_gran (15-bit leaves only, else nullptr): per-block counts of the
+// occupied depth-10 cells as well.
 __global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict__ keys,
                                                       uint32_t na, int bits,
                                                       uint32_t *__restrict__ block_cells,
-                                                      uint32_t *__restrict__ block_leaves) {
-    __shared__ uint32_t wa[4], wb[4];
+                                                      uint32_t *__restrict__ block_leaves,
+                                                      uint32_t *__restrict__ block_gran) {
+    __shared__ uint32_t wa[4], wb[4], wc[4];
     const PosInfo q = pos_info(keys, na, blockIdx.x * 256 + threadIdx.x, bits);
     uint32_t a = q.ns, b = q.lf;
     wave_scan2(a, b);
+    const uint32_t c = (uint32_t)__popcll(__ballot(q.g10));
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 63) {
         wa[wave] = a;
         wb[wave] = b;
+        wc[wave] = c;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         block_cells[blockIdx.x] = wa[0] + wa[1] + wa[2] + wa[3];
         block_leaves[blockIdx.x] = wb[0] + wb[1] + wb[2] + wb[3];
+        if (block_gran) block_gran[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
     }
 }
 
@@ -327,23 +336,29 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
     float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
     uint32_t *__restrict__ sorted_src, uint32_t *__restrict__ breaker,
-    BhCounters *__restrict__ ctr) {
-    __shared__ uint32_t wa[4], wb[4];
+    BhCounters *__restrict__ ctr, const uint32_t *__restrict__ block_gran,
+    uint32_t *__restrict__ gran_first, uint32_t *__restrict__ gran_key) {
+    __shared__ uint32_t wa[4], wb[4], wc[4];
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
     const PosInfo q = pos_info(keys, na, k, bits);
     uint32_t a = q.ns, b = q.lf;
     wave_scan2(a, b);
+    const unsigned long long gb = __ballot(q.g10);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 63) {
         wa[wave] = a;
         wb[wave] = b;
+        wc[wave] = (uint32_t)__popcll(gb);
     }
     __syncthreads();
     uint32_t ea = a - q.ns + block_cells[blockIdx.x];
     uint32_t eb = b - q.lf + block_leaves[blockIdx.x];
+    uint32_t ec = (uint32_t)__popcll(gb & ((1ull << lane) - 1ull)) +
+                  (block_gran ? block_gran[blockIdx.x] : 0u);
     for (uint32_t w = 0; w < wave; ++w) {
         ea += wa[w];
         eb += wb[w];
+        ec += wc[w];
     }
     if (k >= na) return;
     if (!q.inside) {
@@ -353,7 +368,9 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
             ctr->n_cells = 0;
             ctr->n_nodes = 0;
             ctr->n_leaves = 0;
+            ctr->n_gran = 0;
             leaf_first[0] = 0;
+            if (gran_first) gran_first[0] = 0;
         }
         return;
     }
@@ -374,12 +391,18 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         leaf_key[L] = q.key;
     }
     for (uint32_t j = 0; j < q.ns; ++j) breaker[C + j] = k;
+    if (gran_first && q.g10) {
+        gran_first[ec] = k;
+        gran_key[ec] = q.key >> 10;  // the 20-bit cell
+    }
     if (q.sk1 < 0) {  // last in-domain position
         ctr->n_in = k + 1;
         ctr->n_cells = C + q.ns;
         ctr->n_nodes = k + 1 + C + q.ns;
         ctr->n_leaves = L + q.lf;
         leaf_first[L + q.lf] = k + 1;
+        ctr->n_gran = ec + (q.g10 ? 1u : 0u);
+        if (gran_first) gran_first[ec + (q.g10 ? 1u : 0u)] = k + 1;
     }
 }
 
@@ -478,12 +501,59 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
 
 // ---- 6'. sparse level reduction (15-bit leaves) ---------------------------------
 
-// One launch per depth d = 14..0.  The cell node of depth d that starts at
-// sorted position k (s(k) < d <= s(k+1)) reads its children in pre-order from
-// slot + 1 on: a particle node (count 1) contributes 0 + m*p, a cell node its
-// sums in nsum; each child's quadrant comes from the key bits of its first
-// particle.  Sums are ((c0 + c1) + c2) + c3 with an empty quadrant +0: the
-// dense tile reduction's rule, evaluated only where particles are.
+// The sums of the depth-d cell (>= 2 particles) that starts at sorted position
+// k, from its children in pre-order at slot + 1 on: a particle node (count 1)
+// contributes 0 + m*p, a cell node its sums in nsum; each child's quadrant
+// comes from the key bits of its first particle.  Sums are ((c0 + c1) + c2) + c3
+// with an empty quadrant +0: the dense tile reduction's rule, evaluated only
+// where particles are.  Returns (sx, sy, sm) and the particle count n and
+// subtree size y.
+struct CellSums {
+    float sx, sy, sm;
+    uint32_t n, y;
+};
+
+__device__ __forceinline__ CellSums child_sums(const uint32_t *__restrict__ keys,
+                                               const float4 *__restrict__ prod,
+                                               const uint32_t *__restrict__ ncount,
+                                               const float4 *__restrict__ nsum, uint32_t k,
+                                               uint32_t slot, int d, uint32_t n_in,
+                                               uint32_t n_nodes) {
+    const uint32_t shift = 2u * (kDeepBits - 1u - (uint32_t)d);  // child quadrant bits
+    const uint32_t prefix = keys[k] >> (shift + 2u);
+    float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0, v2 = v0, v3 = v0;
+    uint32_t c = slot + 1u, f = k, n = 0u, y = 1u;
+    // at most four children (distinct quadrants)
+    for (int it = 0; it < 4 && f < n_in && c < n_nodes; ++it) {
+        const uint32_t key = keys[f];
+        if ((key >> (shift + 2u)) != prefix) break;
+        const uint32_t cy = ncount[c];
+        float4 v;
+        uint32_t cn;
+        if (cy == 1u) {  // a particle node
+            const float4 p = prod[f];
+            v = make_float4(0.0f + p.x, 0.0f + p.y, 0.0f + p.z, 0.0f);
+            cn = 1u;
+        } else {
+            v = nsum[c];
+            cn = __float_as_uint(v.w);
+        }
+        const uint32_t q = (key >> shift) & 3u;
+        if (q == 0u) v0 = v;
+        else if (q == 1u) v1 = v;
+        else if (q == 2u) v2 = v;
+        else v3 = v;
+        n += cn;
+        y += cy;
+        f += cn;
+        c += cy;
+    }
+    return CellSums{((v0.x + v1.x) + v2.x) + v3.x, ((v0.y + v1.y) + v2.y) + v3.y,
+                    ((v0.z + v1.z) + v2.z) + v3.z, n, y};
+}
+
+// One launch per depth d = 14..11: every cell node of depth d (a sorted
+// position k with s(k) < d <= s(k+1) starts one) gets its node and its sums.
 __global__ __launch_bounds__(256) void bh_level_kernel(
     const uint32_t *__restrict__ keys, const int8_t *__restrict__ sdep,
     const uint32_t *__restrict__ cpos, const float4 *__restrict__ prod,
@@ -492,51 +562,54 @@ __global__ __launch_bounds__(256) void bh_level_kernel(
     const float *__restrict__ box) {
     const uint32_t n_in = ctr->n_in, n_nodes = ctr->n_nodes;
     const float r0 = box ? box[3] : kFixedR0;
-    const uint32_t shift = 2u * (kDeepBits - 1u - (uint32_t)d);  // child quadrant bits
     for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n_in; k += gridDim.x * 256) {
         const int sk = sdep[k];
         const int sk1 = k + 1 < n_in ? (int)sdep[k + 1] : -1;
         if (!(sk < d && d <= sk1)) continue;
         const uint32_t slot = k + cpos[k] + (uint32_t)(d - sk - 1);
-        const uint32_t prefix = keys[k] >> (shift + 2u);
-        float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), v1 = v0, v2 = v0, v3 = v0;
-        uint32_t c = slot + 1u, f = k, n = 0u, y = 1u;
-        // at most four children (distinct quadrants)
-        for (int it = 0; it < 4 && f < n_in && c < n_nodes; ++it) {
-            const uint32_t key = keys[f];
-            if ((key >> (shift + 2u)) != prefix) break;
-            const uint32_t cy = ncount[c];
-            float4 v;
-            uint32_t cn;
-            if (cy == 1u) {  // a particle node
-                const float4 p = prod[f];
-                v = make_float4(0.0f + p.x, 0.0f + p.y, 0.0f + p.z, 0.0f);
-                cn = 1u;
-            } else {
-                v = nsum[c];
-                cn = __float_as_uint(v.w);
-            }
-            const uint32_t q = (key >> shift) & 3u;
-            if (q == 0u) v0 = v;
-            else if (q == 1u) v1 = v;
-            else if (q == 2u) v2 = v;
-            else v3 = v;
-            n += cn;
-            y += cy;
-            f += cn;
-            c += cy;
+        const CellSums c = child_sums(keys, prod, ncount, nsum, k, slot, d, n_in, n_nodes);
+        float px = c.sx, py = c.sy;
+        if (c.sm != 0.0f) {
+            px = px / c.sm;
+            py = py / c.sm;
         }
-        const float sx = ((v0.x + v1.x) + v2.x) + v3.x;
-        const float sy = ((v0.y + v1.y) + v2.y) + v3.y;
-        const float sm = ((v0.z + v1.z) + v2.z) + v3.z;
-        float px = sx, py = sy;
-        if (sm != 0.0f) {
-            px = px / sm;
-            py = py / sm;
+        ncount[slot] = c.y;
+        node4[slot] = make_float4(px, py, c.sm, cell_radius(d, r0));
+        nsum[slot] = make_float4(c.sx, c.sy, c.sm, __uint_as_float(c.n));
+    }
+}
+
+// Depth 10 and up with 15-bit leaves: the cells of depth <= 10 are those of a
+// 10-bit tree over the same particles, so the dense tile and top kernels
+// reduce them once each occupied depth-10 cell (granule) has its record: the
+// sums of its depth-10 cell (child_sums) when it holds >= 2 particles, else
+// 0 + m*p of its one particle; particle count; subtree size.  Also marks each
+// depth-5 tile's granule range for the tile kernel.
+__global__ __launch_bounds__(256) void bh_gran_kernel(
+    const uint32_t *__restrict__ keys, const int8_t *__restrict__ sdep,
+    const uint32_t *__restrict__ cpos, const float4 *__restrict__ prod,
+    const BhCounters *__restrict__ ctr, const uint32_t *__restrict__ ncount,
+    const float4 *__restrict__ nsum, const uint32_t *__restrict__ gran_first,
+    const uint32_t *__restrict__ gran_key, float4 *__restrict__ gran_sum,
+    uint32_t *__restrict__ gran_y, uint32_t *__restrict__ tile_range) {
+    const uint32_t n_in = ctr->n_in, n_nodes = ctr->n_nodes, ng = ctr->n_gran;
+    for (uint32_t L = blockIdx.x * 256 + threadIdx.x; L < ng; L += gridDim.x * 256) {
+        const uint32_t k = gran_first[L];
+        const int sk = sdep[k];
+        const int sk1 = k + 1 < n_in ? (int)sdep[k + 1] : -1;
+        if (sk1 >= 10) {
+            const uint32_t slot = k + cpos[k] + (uint32_t)(10 - sk - 1);
+            const CellSums c = child_sums(keys, prod, ncount, nsum, k, slot, 10, n_in, n_nodes);
+            gran_sum[L] = make_float4(c.sx, c.sy, c.sm, __uint_as_float(c.n));
+            gran_y[L] = c.y;
+        } else {
+            const float4 p = prod[k];
+            gran_sum[L] = make_float4(0.0f + p.x, 0.0f + p.y, 0.0f + p.z, __uint_as_float(1u));
+            gran_y[L] = 1u;
         }
-        ncount[slot] = y;
-        node4[slot] = make_float4(px, py, sm, cell_radius(d, r0));
-        nsum[slot] = make_float4(sx, sy, sm, __uint_as_float(n));
+        const uint32_t T = gran_key[L] >> 10;
+        if (L == 0 || (gran_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
+        if (L + 1 == ng || (gran_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
     }
 }
 
@@ -553,7 +626,9 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
     const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
     const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
     float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top,
-    const float *__restrict__ box) {
+    const float *__restrict__ box, const uint32_t *__restrict__ leaf_y) {
+    // leaf_y (15-bit leaves): the depth-10 records are granules whose subtree
+    // size is given; otherwise leaves of n particles, y = n + [n >= 2]
     __shared__ TileCell l10[1024];
     __shared__ TileCell up[341];
     const uint32_t T = blockIdx.x, t = threadIdx.x;
@@ -570,7 +645,8 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
         const uint32_t c = leaf_key[L] & 1023u;
         const float4 s = leaf_sum[L];
         const uint32_t n = __float_as_uint(s.w);
-        TileCell cell{n, n + (n >= 2u ? 1u : 0u), leaf_first[L], 0u, s.x, s.y, s.z, 0.0f};
+        const uint32_t y = leaf_y ? leaf_y[L] : n + (n >= 2u ? 1u : 0u);
+        TileCell cell{n, y, leaf_first[L], 0u, s.x, s.y, s.z, 0.0f};
         l10[c] = cell;
         emit_cell(cell, 10, sdep, cpos, node4, ncount, r0);
     }
@@ -1009,7 +1085,13 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->top, kTiles);
     GSIM_ALLOC(b->node4, b->cap);
     GSIM_ALLOC(b->ncount, b->cap);
-    if (bits == kDeepBits) GSIM_ALLOC(b->nsum, b->cap);
+    if (bits == kDeepBits) {
+        GSIM_ALLOC(b->nsum, b->cap);
+        GSIM_ALLOC(b->gran_first, na + 1);
+        GSIM_ALLOC(b->gran_key, na);
+        GSIM_ALLOC(b->gran_y, na);
+        GSIM_ALLOC(b->block_gran, b->prep_blocks);
+    }
     GSIM_ALLOC(b->sorted_src, na);
     GSIM_ALLOC(b->ctr, 1);
     GSIM_ALLOC(b->box, 4);
@@ -1031,7 +1113,8 @@ void bh_free(BhBuffers *b) {
     }
     void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
-                    b->node4, b->ncount, b->nsum, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box,
+                    b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
+                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box,
                     b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
@@ -1077,26 +1160,37 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     b.sorted_slot = deep ? 2 : 1;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
     const uint32_t pb = (uint32_t)b.prep_blocks;
-    bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, bits, b.block_cells, b.block_leaves);
-    scan_exclusive_kernel<<<2, 1024, 0, s->stream>>>(b.block_cells, b.block_leaves, pb);
+    bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, bits, b.block_cells, b.block_leaves,
+                                              deep ? b.block_gran : nullptr);
+    scan_exclusive_kernel<<<deep ? 3 : 2, 1024, 0, s->stream>>>(b.block_cells, b.block_leaves,
+                                                                b.block_gran, pb);
     bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, na, bits, b.block_cells,
                                               b.block_leaves, b.sdep, b.cpos, b.prod,
                                               b.leaf_first, b.leaf_key, b.node4, b.ncount,
-                                              b.sorted_src, b.breaker, b.ctr);
+                                              b.sorted_src, b.breaker, b.ctr,
+                                              deep ? b.block_gran : nullptr,
+                                              deep ? b.gran_first : nullptr, b.gran_key);
     const uint32_t lb = pb < kLeafBlocks ? pb : kLeafBlocks;
     bh_leaf_kernel<<<lb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, sk, na, bits,
                                               b.cpos, b.breaker, b.ctr, b.leaf_sum, b.tile_range,
                                               b.sdep, b.node4, b.ncount,
                                               deep ? b.nsum : nullptr, box);
     if (deep) {
-        for (int d = (int)kDeepBits - 1; d >= 0; --d)
+        // sparse levels below depth 10, then depth-10 records for the dense path
+        for (int d = (int)kDeepBits - 1; d > (int)kTreeDepth; --d)
             bh_level_kernel<<<pb, 256, 0, s->stream>>>(sk, b.sdep, b.cpos, b.prod, b.ctr, b.node4,
                                                        b.ncount, b.nsum, d, box);
-        return hipGetLastError();
+        bh_gran_kernel<<<pb, 256, 0, s->stream>>>(sk, b.sdep, b.cpos, b.prod, b.ctr, b.ncount,
+                                                  b.nsum, b.gran_first, b.gran_key, b.leaf_sum,
+                                                  b.gran_y, b.tile_range);
+        bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.gran_first, b.gran_key,
+                                                      b.leaf_sum, b.sdep, b.cpos, b.node4,
+                                                      b.ncount, b.top, box, b.gran_y);
+    } else {
+        bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
+                                                      b.leaf_sum, b.sdep, b.cpos, b.node4,
+                                                      b.ncount, b.top, box, nullptr);
     }
-    bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
-                                                  b.leaf_sum, b.sdep, b.cpos, b.node4,
-                                                  b.ncount, b.top, box);
     bh_top_kernel<<<1, 1024, 0, s->stream>>>(b.top, b.sdep, b.cpos, b.node4, b.ncount, box);
     return hipGetLastError();
 }

@@ -44,7 +44,8 @@ struct BhCounters {
     unsigned long long visits;   // real-lane node visits of the traversal
     unsigned long long accepts;  // real-lane acceptances
     uint32_t next_group;         // work counter of the persistent traversal
-    uint32_t pad[3];
+    uint32_t n_gran;             // 15-bit leaves: occupied depth-10 cells
+    uint32_t pad[2];
 };
 
 struct TileCell {  // one cell of the dense tile/top reduction
@@ -73,6 +74,13 @@ struct BhBuffers {
     uint32_t *ncount = nullptr;      // [cap]
     float4 *nsum = nullptr;          // [cap] 15-bit leaves only: a cell node's
                                      // (sx, sy, sm, particle count as bits)
+    // 15-bit leaves only: the occupied depth-10 cells ("granules") that feed
+    // the dense tile reduction (first sorted position, 20-bit cell, subtree
+    // size; their sums go to leaf_sum), and the per-block granule counts
+    uint32_t *gran_first = nullptr;  // [Na+1]
+    uint32_t *gran_key = nullptr;    // [Na]
+    uint32_t *gran_y = nullptr;      // [Na]
+    uint32_t *block_gran = nullptr;  // [prep_blocks]
     uint32_t *sorted_src = nullptr;  // [Na]
     BhCounters *ctr = nullptr;
     uint32_t sorted_slot = 0;        // which keys/idx buffer holds the sorted result
