@@ -503,8 +503,21 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
         case GSIM_DUMP_COUNTERS:
             *written = 2 * sizeof(uint64_t);
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
-            ((uint64_t *)buf)[0] = ctr.visits;
-            ((uint64_t *)buf)[1] = ctr.accepts;
+            {
+                const size_t groups = na / 64;
+                std::vector<unsigned long long> v(groups), a(groups);
+                if (groups) {
+                    GSIM_HIP(hipMemcpy(v.data(), s->bh.gvis, groups * 8, hipMemcpyDeviceToHost));
+                    GSIM_HIP(hipMemcpy(a.data(), s->bh.gacc, groups * 8, hipMemcpyDeviceToHost));
+                }
+                uint64_t sv = 0, sa = 0;
+                for (size_t g = 0; g < groups; ++g) {
+                    sv += v[g];
+                    sa += a[g];
+                }
+                ((uint64_t *)buf)[0] = sv;
+                ((uint64_t *)buf)[1] = sa;
+            }
             return GSIM_OK;
         case GSIM_DUMP_COUNTS:
             *written = sizeof(uint32_t) * nodes;

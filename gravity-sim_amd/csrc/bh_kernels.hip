@@ -127,8 +127,6 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     if (i < 2 * kTiles) tile_range[i] = 0;
     if (i < 4 * 256) digit_total[i] = 0;
     if (i == 0) {
-        ctr->visits = 0;
-        ctr->accepts = 0;
         ctr->next_group = 0;
     }
     if (i >= na) return;
@@ -719,8 +717,8 @@ __device__ __forceinline__ void walk_group(
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
     float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
-    float theta2, float kv, float kp, unsigned long long &vis, unsigned long long &acc,
-    uint32_t *__restrict__ gcost = nullptr) {
+    float theta2, float kv, float kp, unsigned long long *__restrict__ gvis,
+    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gcost = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[g * 64 + lane];
     const bool real = src < na;  // ForceShader.comp:80-92
@@ -732,8 +730,13 @@ __device__ __forceinline__ void walk_group(
     }
     const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
     Partials P;
+    unsigned long long vis = 0, acc = 0;
     const uint32_t cost = walk_partials<kStaged>(0u, total, stage, queue, node4, ncount, px, py,
                                                  real, nreal, eps2, theta2, 0xfu, P, vis, acc);
+    if (lane == 0) {  // this group's visits and acceptances, summed on a dump
+        gvis[g] = vis;
+        gacc[g] = acc;
+    }
     if (real) {
         float ax, ay;
         partial_sum(P, ax, ay);
@@ -802,7 +805,8 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, const uint32_t *__restrict__ gpre, uint32_t *__restrict__ gcost,
-    uint32_t split_x) {
+    uint32_t split_x, unsigned long long *__restrict__ gvis,
+    unsigned long long *__restrict__ gacc) {
     __shared__ float4 stage[16][64];
     __shared__ uint2 runq[16][kRunQueue];
     __shared__ uint32_t seg[4];            // first group, end, next unit, list fill
@@ -852,7 +856,6 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     __syncthreads();
     const uint32_t total_nodes = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
     const uint32_t units = 4 * nsplit + (count - nsplit);
-    unsigned long long vis = 0, acc = 0;
     while (true) {
         uint32_t u = 0;
         if (lane == 0) u = atomicAdd(&seg[2], 1u);
@@ -878,9 +881,14 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         }
         const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
         Partials P;
+        unsigned long long vis = 0, acc = 0;
         const uint32_t cost = walk_partials<true>(0u, total_nodes, stage[w], runq[w], node4, ncount,
                                                   px, py, real, nreal, eps2, theta2, parts, P, vis,
                                                   acc);
+        if (lane == 0 && (parts & 1u)) {  // the counting wave of the group
+            gvis[g] = vis;
+            gacc[g] = acc;
+        }
         float ax = 0.0f, ay = 0.0f;
         if (whole) {
             partial_sum(P, ax, ay);
@@ -908,10 +916,6 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
             if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
         }
     }
-    if (lane == 0 && vis) {
-        atomicAdd(&ctr->visits, vis);
-        atomicAdd(&ctr->accepts, acc);
-    }
 }
 
 // kPersistent: waves take groups from a device counter until all na/64 are
@@ -924,7 +928,8 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     const uint32_t *__restrict__ ncount,
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
-    float kv, float kp) {
+    float kv, float kp, unsigned long long *__restrict__ gvis,
+    unsigned long long *__restrict__ gacc) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
     __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -932,7 +937,6 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     // node count is bounded by the node arrays; every iteration advances ind by
     // at least one, so each walk ends
     const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
-    unsigned long long vis = 0, acc = 0;
     float4 *st = stage[kStaged ? w : 0];
     uint2 *rq = runq[kStaged ? w : 0];
     if (kPersistent) {
@@ -943,18 +947,14 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
             if (g >= groups) break;
             walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4,
                                              ncount, pos_out, vel_out, acc_dbg, na, eps2,
-                                             theta2, kv, kp, vis, acc);
+                                             theta2, kv, kp, gvis, gacc);
         }
     } else {
         const uint32_t g = blockIdx.x * 4 + w;
         if (g >= groups) return;  // whole wave beyond the last group
         walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
                                          pos_out, vel_out, acc_dbg, na, eps2, theta2, kv,
-                                         kp, vis, acc);
-    }
-    if (lane == 0 && vis) {
-        atomicAdd(&ctr->visits, vis);
-        atomicAdd(&ctr->accepts, acc);
+                                         kp, gvis, gacc);
     }
 }
 
@@ -966,7 +966,8 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     const float2 *__restrict__ vel, const float4 *__restrict__ node4,
     const uint32_t *__restrict__ ncount, BhCounters *__restrict__ ctr,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
-    uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp) {
+    uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp,
+    unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc) {
     const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
     if ((slot & ~63u) >= na) return;
     const uint32_t lane = threadIdx.x & 63;
@@ -1023,9 +1024,9 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
         vis += __shfl_xor(vis, off, 64);
         acc += __shfl_xor(acc, off, 64);
     }
-    if (lane == 0 && vis) {
-        atomicAdd(&ctr->visits, vis);
-        atomicAdd(&ctr->accepts, acc);
+    if (lane == 0) {  // the wave's two half-wave groups, summed on a dump
+        gvis[slot >> 6] = vis;
+        gacc[slot >> 6] = acc;
     }
 }
 
@@ -1097,10 +1098,14 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->box, 4);
     GSIM_ALLOC(b->box_part, kBoxParts);
     GSIM_ALLOC(b->gcost, na / 64);
+    GSIM_ALLOC(b->gvis, na / 64 + 1);
+    GSIM_ALLOC(b->gacc, na / 64 + 1);
     GSIM_ALLOC(b->gpre, na / 64 + 1);
 #undef GSIM_ALLOC
     if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
     if (e == hipSuccess) e = hipMemset(b->gcost, 0, sizeof(uint32_t) * (na / 64));
+    if (e == hipSuccess) e = hipMemset(b->gvis, 0, sizeof(unsigned long long) * (na / 64 + 1));
+    if (e == hipSuccess) e = hipMemset(b->gacc, 0, sizeof(unsigned long long) * (na / 64 + 1));
     if (e == hipSuccess) e = hipMemset(b->ncount, 0, sizeof(uint32_t) * b->cap);
     if (e == hipSuccess) e = hipMemset(b->node4, 0, sizeof(float4) * b->cap);
     return e;
@@ -1114,7 +1119,7 @@ void bh_free(BhBuffers *b) {
     void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
-                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gpre, b->box,
+                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gpre, b->gvis, b->gacc, b->box,
                     b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
@@ -1210,7 +1215,8 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #define GSIM_BAL(D, S)                                                                          \
     bh_traverse_bal_kernel<D, S><<<P, 1024, 0, s->stream>>>(                                   \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,          \
-        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost, s->trav_split)
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost, s->trav_split,   \
+        b.gvis, b.gacc)
             if (dbg) { if (s->trav_split) GSIM_BAL(true, true); else GSIM_BAL(true, false); }
             else { if (s->trav_split) GSIM_BAL(false, true); else GSIM_BAL(false, false); }
 #undef GSIM_BAL
@@ -1227,7 +1233,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #define GSIM_TRAV(D, S, P)                                                                 \
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
-        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp)
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
             else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }
@@ -1240,11 +1246,11 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         if (dbg)
             bh_traverse32_kernel<true><<<grid, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc);
         else
             bh_traverse32_kernel<false><<<grid, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp);
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc);
     }
     return hipGetLastError();
 }

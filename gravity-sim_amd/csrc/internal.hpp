@@ -41,8 +41,6 @@ struct BhCounters {
     uint32_t n_cells;   // cell nodes (cells holding >= 2 particles)
     uint32_t n_nodes;   // n_in + n_cells = counts[0]
     uint32_t n_leaves;  // occupied leaf cells (depth 10, or 15 with 15-bit leaves)
-    unsigned long long visits;   // real-lane node visits of the traversal
-    unsigned long long accepts;  // real-lane acceptances
     uint32_t next_group;         // work counter of the persistent traversal
     uint32_t n_gran;             // 15-bit leaves: occupied depth-10 cells
     uint32_t pad[2];
@@ -88,6 +86,11 @@ struct BhBuffers {
     // the exclusive prefix of (cost + 1) that cuts the groups into segments
     uint32_t *gcost = nullptr;       // [Na/64], zero before the first walk
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
+    // each group's real-lane node visits and acceptances in the last walk,
+    // written by the group's wave and summed on a dump (one same-address
+    // atomic per wave cost ~5% of the walk)
+    unsigned long long *gvis = nullptr;  // [Na/64 + 1]
+    unsigned long long *gacc = nullptr;  // [Na/64 + 1]
     // adaptive bounding box (opt-in): {cx, cy, h, r0} and per-block extremes
     float *box = nullptr;            // [4]
     float4 *box_part = nullptr;      // [256] (xmin, xmax, ymin, ymax)
