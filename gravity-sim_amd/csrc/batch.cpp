@@ -49,7 +49,7 @@ void free_batch(gsim_batch *b) {
                     d.keys[0], d.keys[1], d.idx[0], d.idx[1], d.hist, d.digit_total,
                     d.block_cells, d.block_leaves, d.sdep, d.cpos, d.sorted_src, d.prod,
                     d.leaf_sum, d.leaf_first, d.leaf_key, d.tile_range, d.live_tiles, d.top, d.node4,
-                    d.ncount, d.breaker};
+                    d.ncount, d.breaker, d.wave_acc};
     for (void *p : ptrs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -268,8 +268,10 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     GSIM_BALLOC(d.node4, b->node_capacity);
     GSIM_BALLOC(d.ncount, b->node_capacity);
     GSIM_BALLOC(d.breaker, b->node_capacity);
+    GSIM_BALLOC(d.wave_acc, gsim::kMaxForceWaves);
 #undef GSIM_BALLOC
     if (e == hipSuccess) e = hipMemset(d.counts, 0, sizeof(gsim::BatchCounts));
+    if (e == hipSuccess) e = hipMemset(d.wave_acc, 0, sizeof(unsigned long long) * gsim::kMaxForceWaves);
     if (e == hipSuccess) e = hipMemset(d.ncount, 0, sizeof(uint32_t) * b->node_capacity);
     if (e == hipSuccess) e = hipMemset(d.node4, 0, sizeof(float4) * b->node_capacity);
     if (e != hipSuccess) {

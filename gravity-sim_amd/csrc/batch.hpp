@@ -49,6 +49,7 @@ constexpr uint32_t kUnlimitedSteps = 0xffffffffu;
 constexpr uint32_t kBatchRankShift = 21;                   // key = rank << 21 | cell (bit 20 = killed)
 constexpr uint32_t kBatchMaxSystems = 1u << (32 - kBatchRankShift);  // 2048
 constexpr uint32_t kBatchSentinel = 0xffffffffu;
+constexpr uint32_t kMaxForceWaves = 1024u * 8u;  // persistent force launch: 8 waves per SIMD
 // Largest arena: node slots (3 per particle) and partial sums stay within
 // 32-bit indices.
 constexpr size_t kBatchMaxParticles = (0x7fffffffull / 8) / 64 * 64;
@@ -80,6 +81,10 @@ struct BatchDevice {
     // nodes of all trees
     float4 *node4 = nullptr;
     uint32_t *ncount = nullptr, *breaker = nullptr;
+    // each force wave's Barnes-Hut acceptances in the last step, summed into
+    // counts->accepts by the step's last kernel (a same-address atomic per
+    // wave costs more than the sum)
+    unsigned long long *wave_acc = nullptr;  // [kMaxForceWaves]
 };
 
 struct BatchSystem {

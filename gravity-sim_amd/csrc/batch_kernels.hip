@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     const float2 *__restrict__ vel, const float *__restrict__ mass,
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ partial,
-    uint32_t node_cap) {
+    uint32_t node_cap, unsigned long long *__restrict__ wave_acc) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t w = threadIdx.x >> 6;
@@ -600,15 +600,30 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             bb_ds_tile(item.y, d, pos, mass, partial);
         }
     }
-    if ((threadIdx.x & 63) == 0 && acc) atomicAdd(&counts->accepts, acc);
+    if ((threadIdx.x & 63) == 0) wave_acc[blockIdx.x * 4 + w] = acc;
 }
 
-// End of a step: spend one step of every limited budget.
+// End of a step: spend one step of every limited budget, and add the force
+// waves' acceptances to the running total (one atomic per block).
 __global__ __launch_bounds__(256) void bb_finish_kernel(uint32_t *__restrict__ remaining,
-                                                        const BatchCounts *__restrict__ counts) {
+                                                        BatchCounts *__restrict__ counts,
+                                                        const unsigned long long *__restrict__ wave_acc,
+                                                        uint32_t nwaves) {
+    __shared__ unsigned long long part[4];
     for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < counts->n_sys; s += gridDim.x * 256) {
         const uint32_t r = remaining[s];
         if (r != 0u && r != kUnlimitedSteps) remaining[s] = r - 1u;
+    }
+    unsigned long long a = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nwaves; i += gridDim.x * 256)
+        a += wave_acc[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(&counts->accepts, t);
     }
 }
 
@@ -687,11 +702,11 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.remaining, d.stats, d.counts, d.sorted_src,
                                                d.pos[in], d.vel[in], d.mass, d.node4, d.ncount,
                                                d.pos[out], d.vel[out], d.partial,
-                                               (uint32_t)b->node_capacity);
+                                               (uint32_t)b->node_capacity, d.wave_acc);
     bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.remaining, d.ds_sys, d.ds_prefix,
                                                  d.counts, d.partial, d.pos[in], d.vel[in],
                                                  d.pos[out], d.vel[out]);
-    bb_finish_kernel<<<8, 256, 0, st>>>(d.remaining, d.counts);
+    bb_finish_kernel<<<8, 256, 0, st>>>(d.remaining, d.counts, d.wave_acc, waves);
     return hipGetLastError();
 }
 
