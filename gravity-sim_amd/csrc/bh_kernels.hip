@@ -9,7 +9,8 @@
 //   1. keys      20-bit Morton cell per particle (InitShader.comp:80-98); the
 //                out-of-domain kill of InitShader.comp:110-117.
 //   2. sort      stable LSD radix sort of (cell, index), 3 passes of 8 bits,
-//                so a leaf's particles are in ascending index order.
+//                so a leaf's particles are in ascending index order: one
+//                histogram launch, then one launch per pass (look-back).
 //   3. prep/scan s(k) = depth shared by sorted keys k-1 and k; the cell nodes
 //                that start at k are the depths in (s(k), s(k+1)], so an
 //                exclusive scan C(k) of their number gives every pre-order
@@ -121,14 +122,17 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
-    uint32_t *__restrict__ digit_total, BhCounters *__restrict__ ctr,
-    const float *__restrict__ box) {
+    uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status, uint32_t status_words,
+    BhCounters *__restrict__ ctr, const float *__restrict__ box) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i < 2 * kTiles) tile_range[i] = 0;
     if (i < 4 * 256) digit_total[i] = 0;
+    for (uint32_t j = i; j < status_words; j += gridDim.x * 256) status[j] = 0;
     if (i == 0) {
         ctr->next_group = 0;
+        ctr->sort_stall = 0;
     }
+    if (i < 4) ctr->sort_tile[i] = 0;
     if (i >= na) return;
     const float2 p = pos[i];
     uint32_t c;
@@ -148,44 +152,159 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     idx[i] = i;
 }
 
-// ---- 2. stable LSD radix sort ------------------------------------------------
+// ---- 2. stable LSD radix sort, one sweep per pass ------------------------------
+// One launch counts the digits of every pass; then each pass is one launch over
+// tiles of kSortTile keys.  A workgroup takes the next tile from a counter (so
+// every lower tile belongs to a workgroup that is already running), counts the
+// tile's digits, publishes them, and sums the counts of the tiles before it by
+// decoupled look-back over per-(tile, digit) status words: 0 (not yet counted),
+// kStA | the tile's count, kStP | the inclusive prefix over tiles 0..tile.  A
+// tile publishes kStA before it reads any predecessor, so every wait ends.
+// Keys are then ranked within the tile in (round, wave, lane) order, which is
+// input order, so the sort is stable.
 
-__global__ __launch_bounds__(kSortThreads) void radix_upsweep_kernel(
-    const uint32_t *__restrict__ keys, uint32_t n, uint32_t shift,
-    uint32_t *__restrict__ hist, uint32_t nblk, uint32_t *__restrict__ digit_total) {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-#pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
-    }
-    __syncthreads();
-    const uint32_t c = h[threadIdx.x];
-    hist[threadIdx.x * nblk + blockIdx.x] = c;
-    if (c) atomicAdd(&digit_total[threadIdx.x], c);
+constexpr uint32_t kStA = 1u << 30, kStP = 2u << 30, kStVal = kStA - 1u;
+constexpr int kLookWindow = 4;          // predecessors read at once per digit
+constexpr uint32_t kSpinLimit = 1u << 20;  // bound on re-reads of one status word
+constexpr int kHistTiles = 1;           // tiles counted per histogram workgroup
+
+__device__ __forceinline__ uint32_t status_load(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One workgroup per digit: exclusive offsets of that digit for every sort
-// block, starting at the digit's global base (sum of smaller digits' totals).
-__global__ __launch_bounds__(256) void radix_scan_kernel(uint32_t *__restrict__ hist,
-                                                         uint32_t nblk,
-                                                         const uint32_t *__restrict__ digit_total) {
+__device__ __forceinline__ void status_store(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// digit_total[pass * 256 + d] += keys of digit d in pass `pass` (zeroed by the
+// keys kernel).
+template <int kPasses>
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
+    const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ digit_total) {
+    __shared__ uint32_t h[kPasses][256];
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * (kHistTiles * kSortTile);
+#pragma unroll 4
+    for (int r = 0; r < kHistTiles * kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + t;
+        if (i < n) {
+            const uint32_t k = keys[i];
+#pragma unroll
+            for (int p = 0; p < kPasses; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < kPasses; ++p) {
+        const uint32_t c = h[p][t];
+        if (c) atomicAdd(&digit_total[p * 256 + t], c);
+    }
+}
+
+// One pass: digit (key >> shift) & 255.  digit_total: this pass's 256 totals;
+// status: this pass's [tiles][256] words, zeroed; tile_ctr: zeroed.
+// Within the tile, key r * kSortThreads + t is ranked among the equal digits
+// of its wave by ballots; slot[r][wave][d] holds each (round, wave) count,
+// which thread d turns into exclusive offsets in (round, wave) order, so
+// three barriers order the whole tile.
+__global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
+    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
+    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
+    uint32_t shift, const uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status,
+    uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ stall) {
+    constexpr int kRows = kSortRounds * (kSortThreads / 64);
+    __shared__ uint32_t slot[kRows * 256];
+    __shared__ uint32_t start[256];
     __shared__ uint32_t wsum[4];
-    const uint32_t d = blockIdx.x, t = threadIdx.x;
-    uint32_t total = 0;
-    (void)block_scan256(t < d ? digit_total[t] : 0u, &total, wsum);
-    uint32_t carry = total;
-    uint32_t *row = hist + (size_t)d * nblk;
-    for (uint32_t c0 = 0; c0 < nblk; c0 += 256) {
-        const uint32_t i = c0 + t;
-        const uint32_t x = i < nblk ? row[i] : 0u;
-        uint32_t tot = 0;
-        const uint32_t incl = block_scan256(x, &tot, wsum);
-        if (i < nblk) row[i] = carry + incl - x;
-        carry += tot;
+    __shared__ uint32_t tile_sh;
+    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    if (t == 0) tile_sh = atomicAdd(tile_ctr, 1u);
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) slot[k * 256 + t] = 0;
+    const uint32_t total = digit_total[t];
+    uint32_t all = 0;
+    const uint32_t gbase = block_scan256(total, &all, wsum) - total;  // synchronises
+    const uint32_t tile = tile_sh;
+    const uint32_t base = tile * kSortTile;
+    uint32_t key[kSortRounds], val[kSortRounds], rank[kSortRounds];
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + t;
+        key[r] = i < n ? keys_in[i] : 0u;
+        val[r] = i < n ? idx_in[i] : 0u;
+    }
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        const bool valid = base + r * kSortThreads + t < n;
+        const uint32_t d = (key[r] >> shift) & 255u;
+        uint64_t m = __ballot(valid);  // lanes of this wave holding the same digit
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool set = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(set);
+            m &= set ? bb : ~bb;
+        }
+        rank[r] = (uint32_t)__popcll(m & below);
+        if (valid && rank[r] == 0) slot[(r * (kSortThreads / 64) + wave) * 256 + d] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+    uint32_t c = 0;  // keys of digit t before row k, then the tile's count
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+        const uint32_t x = slot[k * 256 + t];
+        slot[k * 256 + t] = c;
+        c += x;
+    }
+    uint32_t *const st = status + t;  // this digit's word of tile j: st[j * 256]
+    status_store(st + (size_t)tile * 256, (tile ? kStA : kStP) | c);
+    uint32_t excl = 0;
+    if (tile) {
+        int j = (int)tile - 1;
+        bool done = false;
+        while (!done) {
+            uint32_t v[kLookWindow];
+#pragma unroll
+            for (int q = 0; q < kLookWindow; ++q)
+                v[q] = j - q >= 0 ? status_load(st + (size_t)(j - q) * 256) : kStP;
+#pragma unroll
+            for (int q = 0; q < kLookWindow; ++q) {
+                if (!done) {
+                    uint32_t x = v[q];
+#pragma unroll 1
+                    for (uint32_t spin = 0; (x >> 30) == 0; ++spin) {
+                        if (spin >= kSpinLimit) {
+                            atomicAdd(stall, 1u);
+                            x = kStP;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        x = status_load(st + (size_t)(j - q) * 256);
+                    }
+                    excl += x & kStVal;
+                    done = (x & kStP) != 0;
+                }
+            }
+            j -= kLookWindow;
+        }
+        status_store(st + (size_t)tile * 256, kStP | (excl + c));
+    }
+    start[t] = gbase + excl;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) {
+        if (base + r * kSortThreads + t < n) {
+            const uint32_t d = (key[r] >> shift) & 255u;
+            const uint32_t off =
+                start[d] + slot[(r * (kSortThreads / 64) + wave) * 256 + d] + rank[r];
+            if (off < n) {
+                keys_out[off] = key[r];
+                idx_out[off] = val[r];
+            }
+        }
     }
 }
 
@@ -216,55 +335,6 @@ __global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t *__restri
         const uint32_t v = data[i];
         data[i] = run;
         run += v;
-    }
-}
-
-__global__ __launch_bounds__(kSortThreads) void radix_downsweep_kernel(
-    const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
-    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
-    uint32_t shift, const uint32_t *__restrict__ hist, uint32_t nblk) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[kSortThreads / 64][256];
-    __shared__ uint32_t base_off[256];
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    run[t] = 0;
-#pragma unroll
-    for (int w = 0; w < kSortThreads / 64; ++w) wcnt[w][t] = 0;
-    base_off[t] = hist[t * nblk + blockIdx.x];
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint32_t i = blockIdx.x * kSortTile + r * kSortThreads + t;
-        const bool valid = i < n;
-        const uint32_t key = valid ? keys_in[i] : 0u;
-        const uint32_t val = valid ? idx_in[i] : 0u;
-        const uint32_t d = (key >> shift) & 255u;
-        // lanes of this wave holding the same digit
-        uint64_t m = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            const bool set = (d >> bit) & 1u;
-            const uint64_t bb = __ballot(set);
-            m &= set ? bb : ~bb;
-        }
-        const uint32_t rank = (uint32_t)__popcll(m & below);
-        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
-        __syncthreads();
-        if (valid) {
-            uint32_t off = base_off[d] + run[d] + rank;
-            for (uint32_t w = 0; w < wave; ++w) off += wcnt[w][d];
-            keys_out[off] = key;
-            idx_out[off] = val;
-        }
-        __syncthreads();
-        uint32_t tot = 0;
-#pragma unroll
-        for (int w = 0; w < kSortThreads / 64; ++w) {
-            tot += wcnt[w][t];
-            wcnt[w][t] = 0;
-        }
-        run[t] += tot;
-        __syncthreads();
     }
 }
 
@@ -1071,7 +1141,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
         GSIM_ALLOC(b->keys[k], na);
         GSIM_ALLOC(b->idx[k], na);
     }
-    GSIM_ALLOC(b->hist, 256 * b->sort_blocks);
+    GSIM_ALLOC(b->status, 4 * 256 * b->sort_blocks);
     GSIM_ALLOC(b->digit_total, 4 * 256);
     GSIM_ALLOC(b->breaker, b->cap);
     GSIM_ALLOC(b->block_cells, b->prep_blocks);
@@ -1116,7 +1186,7 @@ void bh_free(BhBuffers *b) {
         (void)hipFree(b->keys[k]);
         (void)hipFree(b->idx[k]);
     }
-    void *ptrs[] = {b->hist, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
+    void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
                     b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gpre, b->gvis, b->gacc, b->box,
@@ -1140,27 +1210,28 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
         bh_box_final_kernel<<<1, 256, 0, s->stream>>>(b.box_part, nparts, b.box);
         box = b.box;
     }
+    const int passes = deep ? 4 : 3;
+    const uint32_t nblk = (uint32_t)b.sort_blocks;
+    const uint32_t status_words = (uint32_t)passes * nblk * 256u;
 #define GSIM_KEYS(B, D)                                                                    \
     bh_keys_kernel<B, D><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na, \
                                                        b.keys[0], b.idx[0], b.tile_range,   \
-                                                       b.digit_total, b.ctr, box)
+                                                       b.digit_total, b.status, status_words, \
+                                                       b.ctr, box)
     if (box) { if (deep) GSIM_KEYS(true, true); else GSIM_KEYS(true, false); }
     else { if (deep) GSIM_KEYS(false, true); else GSIM_KEYS(false, false); }
 #undef GSIM_KEYS
     // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
     // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
     const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
-    const int passes = deep ? 4 : 3;
-    const uint32_t nblk = (uint32_t)b.sort_blocks;
+    const uint32_t hblk = (nblk + kHistTiles - 1) / kHistTiles;
+    if (deep) radix_hist_kernel<4><<<hblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
+    else radix_hist_kernel<3><<<hblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
     for (int pass = 0; pass < passes; ++pass) {
-        const uint32_t shift = 8u * pass;
         const int i = in_of[pass], o = out_of[pass];
-        uint32_t *totals = b.digit_total + 256 * pass;
-        radix_upsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(b.keys[i], na, shift, b.hist,
-                                                                   nblk, totals);
-        radix_scan_kernel<<<256, 256, 0, s->stream>>>(b.hist, nblk, totals);
-        radix_downsweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
-            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, shift, b.hist, nblk);
+        radix_onesweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
+            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, b.digit_total + 256 * pass,
+            b.status + (size_t)pass * nblk * 256, &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
     }
     b.sorted_slot = deep ? 2 : 1;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];

@@ -43,7 +43,9 @@ struct BhCounters {
     uint32_t n_leaves;  // occupied leaf cells (depth 10, or 15 with 15-bit leaves)
     uint32_t next_group;         // work counter of the persistent traversal
     uint32_t n_gran;             // 15-bit leaves: occupied depth-10 cells
-    uint32_t pad[2];
+    uint32_t sort_stall;         // look-back waits that hit their bound (never expected)
+    uint32_t pad;
+    uint32_t sort_tile[4];       // tile counter of each radix pass
 };
 
 struct TileCell {  // one cell of the dense tile/top reduction
@@ -55,7 +57,7 @@ struct BhBuffers {
     size_t na = 0, cap = 0, sort_blocks = 0, prep_blocks = 0;
     uint32_t bits = 10;              // leaf grid bits per axis the buffers are sized for
     uint32_t *keys[3] = {}, *idx[3] = {};
-    uint32_t *hist = nullptr;        // [256 * sort_blocks] per-block digit counts
+    uint32_t *status = nullptr;      // [4 * sort_blocks * 256] look-back words per pass
     uint32_t *digit_total = nullptr; // [4 * 256] per-pass digit totals
     uint32_t *breaker = nullptr;     // [cap] sorted position where cell node #o starts
     uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
