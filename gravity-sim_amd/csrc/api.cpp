@@ -461,6 +461,9 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
         return fail(GSIM_ERROR_INVALID_ARGUMENT, "dump %d needs a Barnes-Hut handle", what);
     gsim::BhCounters ctr{};
     GSIM_HIP(hipMemcpy(&ctr, s->bh.ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    if (ctr.sort_stall)
+        return fail(GSIM_ERROR_DEVICE, "radix sort look-back gave up on %u status words: the tree is invalid",
+                    ctr.sort_stall);
     const size_t nodes = s->has_tree ? ctr.n_nodes : 0;
     switch (what) {
         case GSIM_DUMP_CELLS: {

@@ -46,7 +46,7 @@ void free_batch(gsim_batch *b) {
     gsim::BatchDevice &d = b->d;
     void *ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.mass, d.partial, d.desc, d.remaining,
                     d.bh_sys, d.bh_prefix, d.ds_sys, d.ds_prefix, d.items, d.counts, d.stats,
-                    d.keys[0], d.keys[1], d.idx[0], d.idx[1], d.hist, d.digit_total,
+                    d.keys[0], d.keys[1], d.idx[0], d.idx[1], d.status, d.digit_total,
                     d.block_cells, d.block_leaves, d.sdep, d.cpos, d.sorted_src, d.prod,
                     d.leaf_sum, d.leaf_first, d.leaf_key, d.tile_range, d.live_tiles, d.top, d.node4,
                     d.ncount, d.breaker, d.wave_acc};
@@ -251,7 +251,7 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     GSIM_BALLOC(d.remaining, max_systems);
     GSIM_BALLOC(d.counts, 1);
     GSIM_BALLOC(d.stats, max_systems);
-    GSIM_BALLOC(d.hist, 256 * b->tiles_cap);
+    GSIM_BALLOC(d.status, 2 * 256 * b->tiles_cap);
     GSIM_BALLOC(d.digit_total, 4 * 256);
     GSIM_BALLOC(d.block_cells, b->blocks_cap);
     GSIM_BALLOC(d.block_leaves, b->blocks_cap);
@@ -441,6 +441,11 @@ gsim_status gsim_batch_read(gsim_batch *b, uint32_t id, float *pos_xy, float *ve
     if (it == b->systems.end()) return fail(GSIM_ERROR_INVALID_ARGUMENT, "no system %u in the batch", id);
     GSIM_HIP(hipSetDevice(b->device));
     GSIM_HIP(hipStreamSynchronize(b->stream));
+    uint32_t stall = 0;
+    GSIM_HIP(hipMemcpy(&stall, &b->d.counts->sort_stall, sizeof(stall), hipMemcpyDeviceToHost));
+    if (stall)
+        return fail(GSIM_ERROR_DEVICE, "radix sort look-back gave up on %u status words: the batch state is invalid",
+                    stall);
     const gsim::BatchSystem &s = it->second;
     if (pos_xy) GSIM_HIP(hipMemcpy(pos_xy, b->d.pos[b->cur] + s.off, sizeof(float2) * s.n, hipMemcpyDeviceToHost));
     if (vel_xy) GSIM_HIP(hipMemcpy(vel_xy, b->d.vel[b->cur] + s.off, sizeof(float2) * s.n, hipMemcpyDeviceToHost));
@@ -467,8 +472,8 @@ gsim_status gsim_batch_query(const gsim_batch *b, gsim_batch_info *out) {
 }
 
 size_t gsim_batch_bytes_per_particle(void) {
-    // state 36 B, Barnes-Hut scratch 65 B, 3 node slots x 28 B, partials 32 B
-    return 36 + 65 + 3 * 28 + 32;
+    // state 36 B, Barnes-Hut scratch 66 B, 3 node slots x 28 B, partials 32 B
+    return 36 + 66 + 3 * 28 + 32;
 }
 
 gsim_status gsim_batch_set_waves(gsim_batch *b, uint32_t waves_per_simd) {

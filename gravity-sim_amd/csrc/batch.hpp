@@ -40,7 +40,8 @@ struct BatchCounts {
     uint32_t n_leaves;        // occupied leaves of all trees (emit)
     uint32_t n_sys;           // all systems (direct sum and Barnes-Hut)
     uint32_t n_live_tiles;    // (system, depth-5 tile) pairs holding leaves (leaf kernel)
-    uint32_t pad;
+    uint32_t sort_stall;      // radix look-back waits that hit their bound (never expected)
+    uint32_t sort_tile[4];    // tile counter of each radix pass
     unsigned long long accepts;  // Barnes-Hut real-lane acceptances, all steps
 };
 
@@ -69,7 +70,8 @@ struct BatchDevice {
     RankStats *stats = nullptr;
     // Barnes-Hut scratch over all sorted positions
     uint32_t *keys[2] = {}, *idx[2] = {};
-    uint32_t *hist = nullptr, *digit_total = nullptr;
+    uint32_t *status = nullptr;       // [2][tiles_cap * 256] radix look-back words, by pass parity
+    uint32_t *digit_total = nullptr;  // [4 * 256] per-pass digit totals
     uint32_t *block_cells = nullptr, *block_leaves = nullptr;
     int8_t *sdep = nullptr;
     uint32_t *cpos = nullptr, *sorted_src = nullptr;

@@ -14,13 +14,13 @@
 // as well.
 #include "batch.hpp"
 #include "bh_device.hpp"
+#include "radix_device.hpp"
 
 #include <algorithm>
 
 namespace gsim {
 namespace {
 
-constexpr int kSortTile = 2048;
 constexpr uint32_t kCellMask = (1u << 20) - 1u;
 constexpr uint32_t kKilledBit = 1u << 20;
 
@@ -45,17 +45,23 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t *__restrict__ keys,
     uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
-    RankStats *__restrict__ stats, uint32_t *__restrict__ digit_total) {
+    RankStats *__restrict__ stats, uint32_t *__restrict__ digit_total,
+    uint32_t *__restrict__ status) {
     const uint32_t nb = counts->n_bh, n = counts->n_bh_particles;
     const uint32_t tid = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
     for (uint32_t t = tid; t < nb * 2 * kTiles; t += stride) tile_range[t] = 0;
     for (uint32_t t = tid; t < nb; t += stride) stats[t] = RankStats{0u, 0u, 0u, 0u};
     for (uint32_t t = tid; t < 4 * 256; t += stride) digit_total[t] = 0;
+    // the first radix pass's look-back words (each pass clears the next one's)
+    const uint32_t status_words = (n + kSortTile - 1) / kSortTile * 256;
+    for (uint32_t t = tid; t < status_words; t += stride) status[t] = 0;
     if (tid == 0) {
         counts->n_cells = 0;
         counts->n_leaves = 0;
         counts->n_live_tiles = 0;
+        counts->sort_stall = 0;
     }
+    if (tid < 4) counts->sort_tile[tid] = 0;
     for (uint32_t f = tid; f < n; f += stride) {
         const uint32_t r = rank_of(bh_prefix, nb, f);
         const uint32_t di = bh_sys[r];
@@ -87,96 +93,62 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
 }
 
 // ---- stable LSD radix sort over all sorted positions (4 x 8 bits) -----------
+// One-sweep passes (radix_device.hpp) with persistent workgroups: the live
+// tile count comes from device memory.
 
-__global__ __launch_bounds__(256) void bb_upsweep_kernel(const uint32_t *__restrict__ keys,
-                                                         const BatchCounts *__restrict__ counts,
-                                                         uint32_t shift, uint32_t *__restrict__ hist,
-                                                         uint32_t *__restrict__ digit_total) {
-    __shared__ uint32_t h[256];
+// digit_total[pass * 256 + d] += keys of digit d in pass `pass`; the per-wave
+// aggregation matters here: a wave's keys usually share one system, so the
+// upper digits repeat across the wave.
+__global__ __launch_bounds__(kSortThreads) void bb_hist_kernel(
+    const uint32_t *__restrict__ keys, const BatchCounts *__restrict__ counts,
+    uint32_t *__restrict__ digit_total) {
+    __shared__ uint32_t h[4][256];
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) h[p][t] = 0;
+    __syncthreads();
     const uint32_t n = counts->n_bh_particles;
-    const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        h[threadIdx.x] = 0;
-        __syncthreads();
-        for (int r = 0; r < kSortTile / 256; ++r) {
-            const uint32_t i = tile * kSortTile + r * 256 + threadIdx.x;
-            if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    for (uint32_t base = blockIdx.x * kSortTile; base < n; base += gridDim.x * kSortTile) {
+        uint32_t k[kSortRounds];
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) {
+            const uint32_t i = base + r * kSortThreads + t;
+            k[r] = i < n ? keys[i] : 0u;
         }
-        __syncthreads();
-        const uint32_t c = h[threadIdx.x];
-        hist[threadIdx.x * ntiles + tile] = c;
-        if (c) atomicAdd(&digit_total[threadIdx.x], c);
-        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSortRounds; ++r) hist_add<4, true>(h, k[r], base + r * kSortThreads + t < n);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const uint32_t c = h[p][t];
+        if (c) atomicAdd(&digit_total[p * 256 + t], c);
     }
 }
 
-__global__ __launch_bounds__(256) void bb_digit_scan_kernel(uint32_t *__restrict__ hist,
-                                                            const BatchCounts *__restrict__ counts,
-                                                            const uint32_t *__restrict__ digit_total) {
-    __shared__ uint32_t wsum[4];
-    const uint32_t ntiles = (counts->n_bh_particles + kSortTile - 1) / kSortTile;
-    const uint32_t d = blockIdx.x, t = threadIdx.x;
-    uint32_t total = 0;
-    (void)block_scan256(t < d ? digit_total[t] : 0u, &total, wsum);
-    uint32_t carry = total;
-    uint32_t *row = hist + (size_t)d * ntiles;
-    for (uint32_t c0 = 0; c0 < ntiles; c0 += 256) {
-        const uint32_t i = c0 + t;
-        const uint32_t x = i < ntiles ? row[i] : 0u;
-        uint32_t tot = 0;
-        const uint32_t incl = block_scan256(x, &tot, wsum);
-        if (i < ntiles) row[i] = carry + incl - x;
-        carry += tot;
-    }
-}
-
-__global__ __launch_bounds__(256) void bb_downsweep_kernel(
+// Pass `pass` (digit (key >> 8 pass) & 255): workgroups take tiles from
+// counts->sort_tile[pass] until the live tiles run out.  status: this pass's
+// [tiles][256] look-back words, zeroed; status_next: the next pass's, zeroed
+// here tile by tile.
+__global__ __launch_bounds__(kSortThreads) void bb_onesweep_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out,
-    const BatchCounts *__restrict__ counts, uint32_t shift, const uint32_t *__restrict__ hist) {
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wcnt[4][256];
-    __shared__ uint32_t base_off[256];
+    BatchCounts *__restrict__ counts, uint32_t pass, const uint32_t *__restrict__ digit_total,
+    uint32_t *__restrict__ status, uint32_t *__restrict__ status_next) {
+    __shared__ SortShared<true> sh;
     const uint32_t n = counts->n_bh_particles;
     const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        run[t] = 0;
-        for (int w = 0; w < 4; ++w) wcnt[w][t] = 0;
-        base_off[t] = hist[t * ntiles + tile];
+    const uint32_t gbase = digit_base(digit_total + 256 * pass, sh);  // synchronises
+    for (;;) {
+        if (threadIdx.x == 0) sh.tile = atomicAdd(&counts->sort_tile[pass], 1u);
+        sort_tile_clear(sh);
         __syncthreads();
-        for (int r = 0; r < kSortTile / 256; ++r) {
-            const uint32_t i = tile * kSortTile + r * 256 + t;
-            const bool valid = i < n;
-            const uint32_t key = valid ? keys_in[i] : 0u;
-            const uint32_t val = valid ? idx_in[i] : 0u;
-            const uint32_t d = (key >> shift) & 255u;
-            uint64_t m = __ballot(valid);
-#pragma unroll
-            for (int bit = 0; bit < 8; ++bit) {
-                const bool set = (d >> bit) & 1u;
-                const uint64_t bb = __ballot(set);
-                m &= set ? bb : ~bb;
-            }
-            const uint32_t rank = (uint32_t)__popcll(m & below);
-            if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(m);
-            __syncthreads();
-            if (valid) {
-                uint32_t off = base_off[d] + run[d] + rank;
-                for (uint32_t w = 0; w < wave; ++w) off += wcnt[w][d];
-                keys_out[off] = key;
-                idx_out[off] = val;
-            }
-            __syncthreads();
-            uint32_t tot = 0;
-            for (int w = 0; w < 4; ++w) {
-                tot += wcnt[w][t];
-                wcnt[w][t] = 0;
-            }
-            run[t] += tot;
-            __syncthreads();
-        }
+        const uint32_t tile = sh.tile;
+        if (tile >= ntiles) break;
+        status_next[(size_t)tile * 256 + threadIdx.x] = 0;
+        sort_tile<true>(keys_in, idx_in, keys_out, idx_out, n, 8u * pass, tile, gbase, status,
+                  &counts->sort_stall, sh);
+        __syncthreads();
     }
 }
 
@@ -669,17 +641,20 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     hipStream_t st = b->stream;
     const uint32_t g_part = (uint32_t)std::min<size_t>((b->capacity + 255) / 256, 4096);
     const uint32_t g_tiles = (uint32_t)std::min<size_t>(b->tiles_cap, 2048);
+    const uint32_t g_hist = (uint32_t)std::min<size_t>(b->tiles_cap, 1024);
     bb_keys_kernel<<<g_part, 256, 0, st>>>(d.desc, d.bh_sys, d.bh_prefix, d.remaining, d.counts, d.pos[in],
                                            d.mass, d.pos[out], d.vel[in], d.vel[out], d.keys[0],
-                                           d.idx[0], d.tile_range, d.stats, d.digit_total);
-    // 4 passes: keys[0] -> keys[1] -> keys[0] -> keys[1] -> keys[0]
+                                           d.idx[0], d.tile_range, d.stats, d.digit_total,
+                                           d.status);
+    bb_hist_kernel<<<g_hist, kSortThreads, 0, st>>>(d.keys[0], d.counts, d.digit_total);
+    // 4 passes: keys[0] -> keys[1] -> keys[0] -> keys[1] -> keys[0]; look-back
+    // words alternate between the two status halves
+    const size_t half = 256 * b->tiles_cap;
     for (int pass = 0; pass < 4; ++pass) {
         const int i = pass & 1, o = i ^ 1;
-        uint32_t *totals = d.digit_total + 256 * pass;
-        bb_upsweep_kernel<<<g_tiles, 256, 0, st>>>(d.keys[i], d.counts, 8u * pass, d.hist, totals);
-        bb_digit_scan_kernel<<<256, 256, 0, st>>>(d.hist, d.counts, totals);
-        bb_downsweep_kernel<<<g_tiles, 256, 0, st>>>(d.keys[i], d.idx[i], d.keys[o], d.idx[o],
-                                                     d.counts, 8u * pass, d.hist);
+        bb_onesweep_kernel<<<g_tiles, kSortThreads, 0, st>>>(
+            d.keys[i], d.idx[i], d.keys[o], d.idx[o], d.counts, (uint32_t)pass, d.digit_total,
+            d.status + half * (pass & 1), d.status + half * ((pass + 1) & 1));
     }
     const uint32_t *sk = d.keys[0], *si = d.idx[0];
     const uint32_t g_blk = (uint32_t)std::min<size_t>(b->blocks_cap, 4096);

@@ -28,13 +28,11 @@
 //                iff all W lanes accept it (ForceShader.comp:111-141), then
 //                integrate (ForceShader.comp:145-150).
 #include "bh_device.hpp"
+#include "radix_device.hpp"
 
 namespace gsim {
 namespace {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortRounds = 8;
-constexpr int kSortTile = kSortThreads * kSortRounds;
 
 // ---- 1. keys ----------------------------------------------------------------
 
@@ -152,32 +150,10 @@ __global__ __launch_bounds__(256) void bh_keys_kernel(
     idx[i] = i;
 }
 
-// ---- 2. stable LSD radix sort, one sweep per pass ------------------------------
-// One launch counts the digits of every pass; then each pass is one launch over
-// tiles of kSortTile keys.  A workgroup takes the next tile from a counter (so
-// every lower tile belongs to a workgroup that is already running), counts the
-// tile's digits, publishes them, and sums the counts of the tiles before it by
-// decoupled look-back over per-(tile, digit) status words: 0 (not yet counted),
-// kStA | the tile's count, kStP | the inclusive prefix over tiles 0..tile.  A
-// tile publishes kStA before it reads any predecessor, so every wait ends.
-// Keys are then ranked within the tile in (round, wave, lane) order, which is
-// input order, so the sort is stable.
-
-constexpr uint32_t kStA = 1u << 30, kStP = 2u << 30, kStVal = kStA - 1u;
-constexpr int kLookWindow = 4;          // predecessors read at once per digit
-constexpr uint32_t kSpinLimit = 1u << 20;  // bound on re-reads of one status word
-constexpr int kHistTiles = 1;           // tiles counted per histogram workgroup
-
-__device__ __forceinline__ uint32_t status_load(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ void status_store(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// ---- 2. stable LSD radix sort, one sweep per pass (radix_device.hpp) -----------
 
 // digit_total[pass * 256 + d] += keys of digit d in pass `pass` (zeroed by the
-// keys kernel).
+// keys kernel); one tile per workgroup.
 template <int kPasses>
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
     const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ digit_total) {
@@ -186,16 +162,15 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * (kHistTiles * kSortTile);
-#pragma unroll 4
-    for (int r = 0; r < kHistTiles * kSortRounds; ++r) {
-        const uint32_t i = base + r * kSortThreads + t;
-        if (i < n) {
-            const uint32_t k = keys[i];
+    const uint32_t base = blockIdx.x * kSortTile;
+    uint32_t k[kSortRounds];
 #pragma unroll
-            for (int p = 0; p < kPasses; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 255u], 1u);
-        }
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + t;
+        k[r] = i < n ? keys[i] : 0u;
     }
+#pragma unroll
+    for (int r = 0; r < kSortRounds; ++r) hist_add<kPasses, false>(h, k[r], base + r * kSortThreads + t < n);
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) {
@@ -204,108 +179,19 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
     }
 }
 
-// One pass: digit (key >> shift) & 255.  digit_total: this pass's 256 totals;
-// status: this pass's [tiles][256] words, zeroed; tile_ctr: zeroed.
-// Within the tile, key r * kSortThreads + t is ranked among the equal digits
-// of its wave by ballots; slot[r][wave][d] holds each (round, wave) count,
-// which thread d turns into exclusive offsets in (round, wave) order, so
-// three barriers order the whole tile.
+// One pass, one tile per workgroup: digit (key >> shift) & 255.  digit_total:
+// this pass's 256 totals; status: this pass's [tiles][256] words, zeroed;
+// tile_ctr: zeroed.
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
     uint32_t shift, const uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status,
     uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ stall) {
-    constexpr int kRows = kSortRounds * (kSortThreads / 64);
-    __shared__ uint32_t slot[kRows * 256];
-    __shared__ uint32_t start[256];
-    __shared__ uint32_t wsum[4];
-    __shared__ uint32_t tile_sh;
-    const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    if (t == 0) tile_sh = atomicAdd(tile_ctr, 1u);
-#pragma unroll
-    for (int k = 0; k < kRows; ++k) slot[k * 256 + t] = 0;
-    const uint32_t total = digit_total[t];
-    uint32_t all = 0;
-    const uint32_t gbase = block_scan256(total, &all, wsum) - total;  // synchronises
-    const uint32_t tile = tile_sh;
-    const uint32_t base = tile * kSortTile;
-    uint32_t key[kSortRounds], val[kSortRounds], rank[kSortRounds];
-#pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const uint32_t i = base + r * kSortThreads + t;
-        key[r] = i < n ? keys_in[i] : 0u;
-        val[r] = i < n ? idx_in[i] : 0u;
-    }
-    const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        const bool valid = base + r * kSortThreads + t < n;
-        const uint32_t d = (key[r] >> shift) & 255u;
-        uint64_t m = __ballot(valid);  // lanes of this wave holding the same digit
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
-            const bool set = (d >> bit) & 1u;
-            const uint64_t bb = __ballot(set);
-            m &= set ? bb : ~bb;
-        }
-        rank[r] = (uint32_t)__popcll(m & below);
-        if (valid && rank[r] == 0) slot[(r * (kSortThreads / 64) + wave) * 256 + d] = (uint32_t)__popcll(m);
-    }
-    __syncthreads();
-    uint32_t c = 0;  // keys of digit t before row k, then the tile's count
-#pragma unroll
-    for (int k = 0; k < kRows; ++k) {
-        const uint32_t x = slot[k * 256 + t];
-        slot[k * 256 + t] = c;
-        c += x;
-    }
-    uint32_t *const st = status + t;  // this digit's word of tile j: st[j * 256]
-    status_store(st + (size_t)tile * 256, (tile ? kStA : kStP) | c);
-    uint32_t excl = 0;
-    if (tile) {
-        int j = (int)tile - 1;
-        bool done = false;
-        while (!done) {
-            uint32_t v[kLookWindow];
-#pragma unroll
-            for (int q = 0; q < kLookWindow; ++q)
-                v[q] = j - q >= 0 ? status_load(st + (size_t)(j - q) * 256) : kStP;
-#pragma unroll
-            for (int q = 0; q < kLookWindow; ++q) {
-                if (!done) {
-                    uint32_t x = v[q];
-#pragma unroll 1
-                    for (uint32_t spin = 0; (x >> 30) == 0; ++spin) {
-                        if (spin >= kSpinLimit) {
-                            atomicAdd(stall, 1u);
-                            x = kStP;
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        x = status_load(st + (size_t)(j - q) * 256);
-                    }
-                    excl += x & kStVal;
-                    done = (x & kStP) != 0;
-                }
-            }
-            j -= kLookWindow;
-        }
-        status_store(st + (size_t)tile * 256, kStP | (excl + c));
-    }
-    start[t] = gbase + excl;
-    __syncthreads();
-#pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
-        if (base + r * kSortThreads + t < n) {
-            const uint32_t d = (key[r] >> shift) & 255u;
-            const uint32_t off =
-                start[d] + slot[(r * (kSortThreads / 64) + wave) * 256 + d] + rank[r];
-            if (off < n) {
-                keys_out[off] = key[r];
-                idx_out[off] = val[r];
-            }
-        }
-    }
+    __shared__ SortShared<false> sh;
+    if (threadIdx.x == 0) sh.tile = atomicAdd(tile_ctr, 1u);
+    sort_tile_clear(sh);
+    const uint32_t gbase = digit_base(digit_total, sh);  // synchronises
+    sort_tile<false>(keys_in, idx_in, keys_out, idx_out, n, shift, sh.tile, gbase, status, stall, sh);
 }
 
 // Exclusive scan of one array per workgroup (blockIdx.x selects a, b or c),
@@ -1224,9 +1110,8 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
     // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
     const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
-    const uint32_t hblk = (nblk + kHistTiles - 1) / kHistTiles;
-    if (deep) radix_hist_kernel<4><<<hblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
-    else radix_hist_kernel<3><<<hblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
+    if (deep) radix_hist_kernel<4><<<nblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
+    else radix_hist_kernel<3><<<nblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
     for (int pass = 0; pass < passes; ++pass) {
         const int i = in_of[pass], o = out_of[pass];
         radix_onesweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
