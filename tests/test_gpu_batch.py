@@ -57,6 +57,21 @@ def test_batch_matches_single_system_runs():
             _same(b.read(sid), _alone(p, algo, kw, 6))
 
 
+def test_batch_many_sort_tiles_matches_single_system_runs():
+    """4.5 million Barnes-Hut positions: 2,198 radix tiles, more than the GPU
+    holds at once, taken by persistent workgroups over several rounds."""
+    systems = [(gsim.generate("two-exponential-disks", 1500000, 33.0, 1.0, 10.0, seed=31 + k),
+                "barnes-hut", {"accuracy_parameter": 0.7 + 0.1 * k}) for k in range(3)]
+    systems.append((gsim.generate("random", 2000, 50.0, 1.0, 10.0, seed=34), "direct-sum", {}))
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    with gsim.Batch(cap + 4096, max_systems=8) as b:
+        ids = [b.add(p, algo, **kw) for p, algo, kw in systems]
+        b.step(2)
+        for sid, (p, algo, kw) in zip(ids, systems):
+            _say(f"batch vs alone: {algo} n={p.n} {kw}")
+            _same(b.read(sid), _alone(p, algo, kw, 2))
+
+
 def test_batch_membership_changes_between_steps():
     systems = _systems()
     cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)

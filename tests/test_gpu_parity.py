@@ -275,6 +275,27 @@ def test_bh_tree_bit_exact(case):
     assert np.array_equal(t["node_pos"], ot["node_pos"])
 
 
+@pytest.mark.parametrize("bits", [10, 15])
+def test_bh_tree_bit_exact_many_sort_tiles(bits):
+    """3,000,000 particles: 1,465 radix tiles of 2,048 keys, more than the GPU
+    holds at once, so tiles wait on predecessors that started in an earlier
+    round of workgroups.  Every node array equals the oracle's sparse build
+    (itself pinned to the dense restatement at 10 bits)."""
+    _say(f"bh tree two-disks-3000000 at {bits} bits")
+    p = gsim.generate("two-exponential-disks", 3000000, 33.0, 1.0, 10.0, seed=206)
+    s = _sim(p, "barnes-hut")
+    if bits != 10:
+        s.set_tuning(gsim.TUNE_BH_LEAF_BITS, bits)
+    s.step(1)
+    t = s.tree()
+    s.close()
+    pp = p.padded()
+    ot, nn, _, _, _ = O.bh_build(pp.pos, pp.mass, bits=bits)
+    assert t["total"] == nn
+    for k in ("sorted_src", "counts", "radii", "node_mass", "node_pos"):
+        assert np.array_equal(t[k], ot[k]), k
+
+
 def test_bh_leaf_sizes_at_chunk_boundaries():
     """Leaves of 24 (the largest summed by one lane), 25, 63, 64, 65, 127,
     128, 129, 200 and 1,000 particles: the wave-summed leaves mix full 64-node
