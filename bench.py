@@ -123,6 +123,7 @@ def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = F
     sim.synchronize()
     elapsed = time.perf_counter() - t0
     V, A = (int(x) for x in sim.dump(gsim.DUMP_COUNTERS))  # last step's counts
+    nodes = int(sim.dump(gsim.DUMP_NODE_TOTAL)[0])
     t = sim.timing(reset=True)
     sim.close()
     ms_step = elapsed * 1e3 / steps
@@ -131,9 +132,12 @@ def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = F
     flops = BH_FLOP_VISIT * V + BH_FLOP_ACCEPT * A
     achieved = flops / (trav_ms * 1e-3) / 1e12
     lane_ops = BH_LANEOPS_VISIT * V + BH_LANEOPS_ACCEPT * A
-    # step roofline: build bytes at HBM peak + traversal lane-ops at VALU peak
+    # step roofline: build bytes at HBM peak + traversal lane-ops at VALU peak.
+    # Build bytes 64 per particle (keys 12, one sort pass 16, leaf emission 36)
+    # + 20 per cell node (nodes - particles: none leave the domain here)
     na = (BH_N + 63) // 64 * 64
-    t_star = (64 * na) / (PEAK_HBM_GBS * 1e9) + lane_ops / VALU_LANE_OPS
+    build_bytes = 64 * na + 20 * max(nodes - na, 0)
+    t_star = build_bytes / (PEAK_HBM_GBS * 1e9) + lane_ops / VALU_LANE_OPS
     return {
         "metric": "BH step ms @500k",
         "value": ms_step,
@@ -159,6 +163,8 @@ def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = F
             "frac": achieved / PEAK_FP32_TFLOPS,
             "traffic": BH_TRAFFIC_BYTES,
             "valu_lane_op_frac": lane_ops / (trav_ms * 1e-3) / VALU_LANE_OPS,
+            "build_bytes": build_bytes,
+            "build_hbm_frac": build_bytes / (build_ms * 1e-3) / (PEAK_HBM_GBS * 1e9),
             "step_frac": t_star / (ms_step * 1e-3),
         },
     }

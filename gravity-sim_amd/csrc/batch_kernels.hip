@@ -214,28 +214,8 @@ __global__ __launch_bounds__(256) void bb_prep_kernel(const uint32_t *__restrict
 __global__ __launch_bounds__(1024) void bb_block_scan_kernel(uint32_t *__restrict__ a,
                                                              uint32_t *__restrict__ b,
                                                              const BatchCounts *__restrict__ counts) {
-    __shared__ uint32_t part[1024];
-    uint32_t *data = blockIdx.x == 0 ? a : b;
-    const uint32_t count = (counts->n_bh_particles + 255) / 256;
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t lo = min(count, t * per), hi = min(count, lo + per);
-    uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += data[i];
-    part[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = t ? part[t - 1] : 0u;
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
+    __shared__ uint32_t wsum[16];
+    scan_inplace_1024(blockIdx.x == 0 ? a : b, (counts->n_bh_particles + 255) / 256, wsum);
 }
 
 __global__ __launch_bounds__(256) void bb_emit_kernel(

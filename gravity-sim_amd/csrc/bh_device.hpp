@@ -132,6 +132,49 @@ __device__ __forceinline__ void set_run_length(float4 *__restrict__ node4, uint3
     reinterpret_cast<float *>(node4)[4 * (size_t)slot + 3] = -(float)min(n, 1u << 24);
 }
 
+// Exclusive scan of data[0, count) in place by one workgroup of 1024 threads:
+// chunks of 4096 values, four consecutive values per thread (coalesced), wave
+// shuffles, the 16 wave sums scanned by wave 0, a carry between chunks.
+// wsum: 16 words of LDS.
+__device__ __forceinline__ void scan_inplace_1024(uint32_t *__restrict__ data, uint32_t count,
+                                                  uint32_t *wsum) {
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < count; base += 4096) {
+        const uint32_t i0 = base + 4 * t;
+        uint32_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = i0 + u < count ? data[i0 + u] : 0u;
+        const uint32_t s = (v[0] + v[1]) + (v[2] + v[3]);
+        uint32_t x = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        if (wave == 0) {
+            uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1) {
+                const uint32_t y = __shfl_up(w, off, 64);
+                if (lane >= (uint32_t)off) w += y;
+            }
+            if (lane < 16) wsum[lane] = w;
+        }
+        __syncthreads();
+        uint32_t run = carry + (wave ? wsum[wave - 1] : 0u) + (x - s);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (i0 + u < count) data[i0 + u] = run;
+            run += v[u];
+        }
+        carry += wsum[15];
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
+}
+
 // Inclusive scan of one value across a 256-thread workgroup; returns the
 // inclusive prefix and writes the workgroup total.
 __device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t *total,

@@ -195,33 +195,15 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
 }
 
 // Exclusive scan of one array per workgroup (blockIdx.x selects a, b or c),
-// in place; each thread scans a contiguous chunk.
+// in place.
 __global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t *__restrict__ a,
                                                                uint32_t *__restrict__ b,
                                                                uint32_t *__restrict__ c,
                                                                uint32_t count) {
-    __shared__ uint32_t part[1024];
+    __shared__ uint32_t wsum[16];
     uint32_t *data = blockIdx.x == 0 ? a : blockIdx.x == 1 ? b : c;
     if (!data) return;
-    const uint32_t t = threadIdx.x;
-    const uint32_t per = (count + 1023) / 1024;
-    const uint32_t lo = min(count, t * per), hi = min(count, lo + per);
-    uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += data[i];
-    part[t] = s;
-    __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    uint32_t run = t ? part[t - 1] : 0u;
-    for (uint32_t i = lo; i < hi; ++i) {
-        const uint32_t v = data[i];
-        data[i] = run;
-        run += v;
-    }
+    scan_inplace_1024(data, count, wsum);
 }
 
 // ---- 3./4. slot scan and emission --------------------------------------------
