@@ -115,69 +115,72 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
 }
 
 // kDeep: 30-bit keys of the 32768^2 grid (15-bit leaves) instead of 20-bit cells.
+// One sort tile (kSortTile particles, kSortRounds per thread) per workgroup.
+// The digits of every radix pass are counted into digit_total, this build's
+// half of the totals (zeroed by the previous build's keys kernel, or at
+// allocation), while digit_next, the other half, is zeroed for the next build;
+// the first radix pass's look-back words and counters are zeroed as well.
 template <bool kBox, bool kDeep>
-__global__ __launch_bounds__(256) void bh_keys_kernel(
+__global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
-    uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status, uint32_t status_words,
-    BhCounters *__restrict__ ctr, const float *__restrict__ box) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < 2 * kTiles) tile_range[i] = 0;
-    if (i < 4 * 256) digit_total[i] = 0;
-    for (uint32_t j = i; j < status_words; j += gridDim.x * 256) status[j] = 0;
-    if (i == 0) {
+    uint32_t *__restrict__ digit_total, uint32_t *__restrict__ digit_next,
+    uint32_t *__restrict__ status, uint32_t status_words, BhCounters *__restrict__ ctr,
+    const float *__restrict__ box) {
+    constexpr int kPasses = kDeep ? 4 : 3;
+    __shared__ uint32_t h[kPasses][256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t g = blockIdx.x * kSortThreads + t, stride = gridDim.x * kSortThreads;
+    for (uint32_t j = g; j < 2 * kTiles; j += stride) tile_range[j] = 0;
+    for (uint32_t j = g; j < 4 * 256; j += stride) digit_next[j] = 0;
+    for (uint32_t j = g; j < status_words; j += stride) status[j] = 0;
+    if (g == 0) {
         ctr->next_group = 0;
         ctr->sort_stall = 0;
     }
-    if (i < 4) ctr->sort_tile[i] = 0;
-    if (i >= na) return;
-    const float2 p = pos[i];
-    uint32_t c;
-    if (kDeep) c = kBox ? cell30_box(p.x, p.y, box[0], box[1], box[2]) : cell30(p.x, p.y);
-    else c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
-    if (c == 0xffffffffu) {
-        // InitShader.comp:110-117; the reference leaves vel_out stale, here it
-        // keeps the input velocity.
-        const float k = -kSimSize * 2.0f;
-        pos[i] = make_float2(k, k);
-        pos_out[i] = make_float2(k, k);
-        mass[i] = 0.0f;
-        vel_out[i] = vel[i];
-        c = kDeep ? kKilledKeyDeep : kKilledKey;
-    }
-    keys[i] = c;
-    idx[i] = i;
-}
-
-// ---- 2. stable LSD radix sort, one sweep per pass (radix_device.hpp) -----------
-
-// digit_total[pass * 256 + d] += keys of digit d in pass `pass` (zeroed by the
-// keys kernel); one tile per workgroup.
-template <int kPasses>
-__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
-    const uint32_t *__restrict__ keys, uint32_t n, uint32_t *__restrict__ digit_total) {
-    __shared__ uint32_t h[kPasses][256];
-    const uint32_t t = threadIdx.x;
+    if (g < 4) ctr->sort_tile[g] = 0;
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
-    uint32_t k[kSortRounds];
+    float2 pr[kSortRounds];  // all loads first: the kill below stores to pos
 #pragma unroll
     for (int r = 0; r < kSortRounds; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
-        k[r] = i < n ? keys[i] : 0u;
+        pr[r] = i < na ? pos[i] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) hist_add<kPasses, false>(h, k[r], base + r * kSortThreads + t < n);
+    for (int r = 0; r < kSortRounds; ++r) {
+        const uint32_t i = base + r * kSortThreads + t;
+        if (i >= na) continue;
+        const float2 p = pr[r];
+        uint32_t c;
+        if (kDeep) c = kBox ? cell30_box(p.x, p.y, box[0], box[1], box[2]) : cell30(p.x, p.y);
+        else c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
+        if (c == 0xffffffffu) {
+            // InitShader.comp:110-117; the reference leaves vel_out stale, here it
+            // keeps the input velocity.
+            const float k = -kSimSize * 2.0f;
+            pos[i] = make_float2(k, k);
+            pos_out[i] = make_float2(k, k);
+            mass[i] = 0.0f;
+            vel_out[i] = vel[i];
+            c = kDeep ? kKilledKeyDeep : kKilledKey;
+        }
+        keys[i] = c;
+        idx[i] = i;
+        hist_add<kPasses, false>(h, c, true);
+    }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) {
-        const uint32_t c = h[p][t];
-        if (c) atomicAdd(&digit_total[p * 256 + t], c);
+        const uint32_t n = h[p][t];
+        if (n) atomicAdd(&digit_total[p * 256 + t], n);
     }
 }
+
+// ---- 2. stable LSD radix sort, one sweep per pass (radix_device.hpp) -----------
 
 // One pass, one tile per workgroup: digit (key >> shift) & 255.  digit_total:
 // this pass's 256 totals; status: this pass's [tiles][256] words, zeroed;
@@ -1010,7 +1013,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
         GSIM_ALLOC(b->idx[k], na);
     }
     GSIM_ALLOC(b->status, 4 * 256 * b->sort_blocks);
-    GSIM_ALLOC(b->digit_total, 4 * 256);
+    GSIM_ALLOC(b->digit_total, 2 * 4 * 256);
     GSIM_ALLOC(b->breaker, b->cap);
     GSIM_ALLOC(b->block_cells, b->prep_blocks);
     GSIM_ALLOC(b->block_leaves, b->prep_blocks);
@@ -1041,6 +1044,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->gpre, na / 64 + 1);
 #undef GSIM_ALLOC
     if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
+    if (e == hipSuccess) e = hipMemset(b->digit_total, 0, sizeof(uint32_t) * 2 * 4 * 256);
     if (e == hipSuccess) e = hipMemset(b->gcost, 0, sizeof(uint32_t) * (na / 64));
     if (e == hipSuccess) e = hipMemset(b->gvis, 0, sizeof(unsigned long long) * (na / 64 + 1));
     if (e == hipSuccess) e = hipMemset(b->gacc, 0, sizeof(unsigned long long) * (na / 64 + 1));
@@ -1068,7 +1072,6 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     BhBuffers &b = s->bh;
     const uint32_t na = (uint32_t)s->na;
     const uint32_t g256 = (na + 255) / 256;
-    const uint32_t gkeys = g256 > (2 * kTiles + 255) / 256 ? g256 : (2 * kTiles + 255) / 256;
     const bool deep = b.bits == kDeepBits;
     const int bits = (int)b.bits;
     const float *box = nullptr;
@@ -1081,23 +1084,24 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     const int passes = deep ? 4 : 3;
     const uint32_t nblk = (uint32_t)b.sort_blocks;
     const uint32_t status_words = (uint32_t)passes * nblk * 256u;
+    // digit totals alternate between two halves by build parity
+    uint32_t *const totals = b.digit_total + 1024 * b.parity;
+    uint32_t *const totals_next = b.digit_total + 1024 * (b.parity ^ 1u);
+    b.parity ^= 1u;
 #define GSIM_KEYS(B, D)                                                                    \
-    bh_keys_kernel<B, D><<<gkeys, 256, 0, s->stream>>>(pos, mass, pos_out, vel, vel_out, na, \
-                                                       b.keys[0], b.idx[0], b.tile_range,   \
-                                                       b.digit_total, b.status, status_words, \
-                                                       b.ctr, box)
+    bh_keys_kernel<B, D><<<nblk, kSortThreads, 0, s->stream>>>(                             \
+        pos, mass, pos_out, vel, vel_out, na, b.keys[0], b.idx[0], b.tile_range, totals,     \
+        totals_next, b.status, status_words, b.ctr, box)
     if (box) { if (deep) GSIM_KEYS(true, true); else GSIM_KEYS(true, false); }
     else { if (deep) GSIM_KEYS(false, true); else GSIM_KEYS(false, false); }
 #undef GSIM_KEYS
     // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
     // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
     const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
-    if (deep) radix_hist_kernel<4><<<nblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
-    else radix_hist_kernel<3><<<nblk, kSortThreads, 0, s->stream>>>(b.keys[0], na, b.digit_total);
     for (int pass = 0; pass < passes; ++pass) {
         const int i = in_of[pass], o = out_of[pass];
         radix_onesweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
-            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, b.digit_total + 256 * pass,
+            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass,
             b.status + (size_t)pass * nblk * 256, &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
     }
     b.sorted_slot = deep ? 2 : 1;

@@ -58,7 +58,8 @@ struct BhBuffers {
     uint32_t bits = 10;              // leaf grid bits per axis the buffers are sized for
     uint32_t *keys[3] = {}, *idx[3] = {};
     uint32_t *status = nullptr;      // [4 * sort_blocks * 256] look-back words per pass
-    uint32_t *digit_total = nullptr; // [4 * 256] per-pass digit totals
+    uint32_t *digit_total = nullptr; // [2][4 * 256] per-pass digit totals, two halves
+    uint32_t parity = 0;             // the half the next build counts into
     uint32_t *breaker = nullptr;     // [cap] sorted position where cell node #o starts
     uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
     uint32_t *block_leaves = nullptr;
