@@ -469,7 +469,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     Partials P;
     unsigned long long vis = 0;
     (void)walk_partials<true>(root, end, stage, queue, node4, ncount, px, py, real, nreal, d.eps2,
-                              d.theta2, 0xfu, P, vis, acc);
+                              d.theta2, 0xfu, P, vis, acc, kPrioStepBatch);
     if (real) {
         float ax, ay;
         partial_sum(P, ax, ay);

@@ -673,7 +673,8 @@ __device__ __forceinline__ void walk_group(
     Partials P;
     unsigned long long vis = 0, acc = 0;
     const uint32_t cost = walk_partials<kStaged>(0u, total, stage, queue, node4, ncount, px, py,
-                                                 real, nreal, eps2, theta2, 0xfu, P, vis, acc);
+                                                 real, nreal, eps2, theta2, 0xfu, P, vis, acc,
+                                                 kPrioStepSingle);
     if (lane == 0) {  // this group's visits and acceptances, summed on a dump
         gvis[g] = vis;
         gacc[g] = acc;
@@ -825,7 +826,7 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         unsigned long long vis = 0, acc = 0;
         const uint32_t cost = walk_partials<true>(0u, total_nodes, stage[w], runq[w], node4, ncount,
                                                   px, py, real, nreal, eps2, theta2, parts, P, vis,
-                                                  acc);
+                                                  acc, kPrioStepSingle);
         if (lane == 0 && (parts & 1u)) {  // the counting wave of the group
             gvis[g] = vis;
             gacc[g] = acc;
