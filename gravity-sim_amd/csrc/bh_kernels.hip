@@ -659,7 +659,8 @@ __device__ __forceinline__ void walk_group(
     float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
     float theta2, float kv, float kp, unsigned long long *__restrict__ gvis,
-    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gcost = nullptr) {
+    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gcost = nullptr,
+    uint32_t *__restrict__ gchunk = nullptr) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[g * 64 + lane];
     const bool real = src < na;  // ForceShader.comp:80-92
@@ -672,9 +673,13 @@ __device__ __forceinline__ void walk_group(
     const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
     Partials P;
     unsigned long long vis = 0, acc = 0;
+    // start at the issue priority this group reached in the last step
+    const uint32_t prio0 = gchunk ? gchunk[g] / kPrioCarry : 0u;
+    uint32_t chunks = 0;
     const uint32_t cost = walk_partials<kStaged>(0u, total, stage, queue, node4, ncount, px, py,
                                                  real, nreal, eps2, theta2, 0xfu, P, vis, acc,
-                                                 kPrioStepSingle);
+                                                 kPrioStepSingle, prio0, &chunks);
+    if (gchunk && lane == 0) gchunk[g] = chunks;
     if (lane == 0) {  // this group's visits and acceptances, summed on a dump
         gvis[g] = vis;
         gacc[g] = acc;
@@ -871,7 +876,7 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, unsigned long long *__restrict__ gvis,
-    unsigned long long *__restrict__ gacc) {
+    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
     __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -889,14 +894,14 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
             if (g >= groups) break;
             walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4,
                                              ncount, pos_out, vel_out, acc_dbg, na, eps2,
-                                             theta2, kv, kp, gvis, gacc);
+                                             theta2, kv, kp, gvis, gacc, nullptr, gchunk);
         }
     } else {
         const uint32_t g = blockIdx.x * 4 + w;
         if (g >= groups) return;  // whole wave beyond the last group
         walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
                                          pos_out, vel_out, acc_dbg, na, eps2, theta2, kv,
-                                         kp, gvis, gacc);
+                                         kp, gvis, gacc, nullptr, gchunk);
     }
 }
 
@@ -1040,6 +1045,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->box, 4);
     GSIM_ALLOC(b->box_part, kBoxParts);
     GSIM_ALLOC(b->gcost, na / 64);
+    GSIM_ALLOC(b->gchunk, na / 64 + 1);
     GSIM_ALLOC(b->gvis, na / 64 + 1);
     GSIM_ALLOC(b->gacc, na / 64 + 1);
     GSIM_ALLOC(b->gpre, na / 64 + 1);
@@ -1047,6 +1053,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
     if (e == hipSuccess) e = hipMemset(b->digit_total, 0, sizeof(uint32_t) * 2 * 4 * 256);
     if (e == hipSuccess) e = hipMemset(b->gcost, 0, sizeof(uint32_t) * (na / 64));
+    if (e == hipSuccess) e = hipMemset(b->gchunk, 0, sizeof(uint32_t) * (na / 64 + 1));
     if (e == hipSuccess) e = hipMemset(b->gvis, 0, sizeof(unsigned long long) * (na / 64 + 1));
     if (e == hipSuccess) e = hipMemset(b->gacc, 0, sizeof(unsigned long long) * (na / 64 + 1));
     if (e == hipSuccess) e = hipMemset(b->ncount, 0, sizeof(uint32_t) * b->cap);
@@ -1062,7 +1069,7 @@ void bh_free(BhBuffers *b) {
     void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
-                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gpre, b->gvis, b->gacc, b->box,
+                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->gpre, b->gvis, b->gacc, b->box,
                     b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
@@ -1176,7 +1183,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #define GSIM_TRAV(D, S, P)                                                                 \
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
-        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc)
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
             else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }

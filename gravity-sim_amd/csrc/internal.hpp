@@ -88,6 +88,7 @@ struct BhBuffers {
     // balanced traversal: each group's walk cost from the previous step and
     // the exclusive prefix of (cost + 1) that cuts the groups into segments
     uint32_t *gcost = nullptr;       // [Na/64], zero before the first walk
+    uint32_t *gchunk = nullptr;      // [Na/64 + 1] chunks each group summed last step (issue priority)
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
     // each group's real-lane node visits and acceptances in the last walk,
     // written by the group's wave and summed on a dump (one same-address
