@@ -49,7 +49,7 @@ void free_batch(gsim_batch *b) {
                     d.keys[0], d.keys[1], d.idx[0], d.idx[1], d.status, d.digit_total,
                     d.block_cells, d.block_leaves, d.sdep, d.cpos, d.sorted_src, d.prod,
                     d.leaf_sum, d.leaf_first, d.leaf_key, d.tile_range, d.live_tiles, d.top, d.node4,
-                    d.ncount, d.breaker, d.wave_acc};
+                    d.ncount, d.breaker, d.wave_acc, d.ichunk};
     for (void *p : ptrs) (void)hipFree(p);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -159,6 +159,8 @@ gsim_status upload_membership(gsim_batch *b) {
     GSIM_HIP(up(b->d.ds_sys, ds_sys.data(), ds_sys.size() * 4));
     GSIM_HIP(up(b->d.ds_prefix, ds_prefix.data(), ds_prefix.size() * 4));
     GSIM_HIP(up(b->d.items, items.data(), items.size() * sizeof(uint2)));
+    // the work items changed: their last-step chunk counts no longer apply
+    GSIM_HIP(hipMemsetAsync(b->d.ichunk, 0, sizeof(uint32_t) * b->item_capacity, b->stream));
     GSIM_HIP(up(b->d.remaining, remaining.data(), remaining.size() * 4));
     GSIM_HIP(up(b->d.counts, &c, sizeof(c)));
     b->h_desc = desc;
@@ -269,9 +271,11 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     GSIM_BALLOC(d.ncount, b->node_capacity);
     GSIM_BALLOC(d.breaker, b->node_capacity);
     GSIM_BALLOC(d.wave_acc, gsim::kMaxForceWaves);
+    GSIM_BALLOC(d.ichunk, b->item_capacity);
 #undef GSIM_BALLOC
     if (e == hipSuccess) e = hipMemset(d.counts, 0, sizeof(gsim::BatchCounts));
     if (e == hipSuccess) e = hipMemset(d.wave_acc, 0, sizeof(unsigned long long) * gsim::kMaxForceWaves);
+    if (e == hipSuccess) e = hipMemset(d.ichunk, 0, sizeof(uint32_t) * b->item_capacity);
     if (e == hipSuccess) e = hipMemset(d.ncount, 0, sizeof(uint32_t) * b->node_capacity);
     if (e == hipSuccess) e = hipMemset(d.node4, 0, sizeof(float4) * b->node_capacity);
     if (e != hipSuccess) {
@@ -472,8 +476,8 @@ gsim_status gsim_batch_query(const gsim_batch *b, gsim_batch_info *out) {
 }
 
 size_t gsim_batch_bytes_per_particle(void) {
-    // state 36 B, Barnes-Hut scratch 66 B, 3 node slots x 28 B, partials 32 B
-    return 36 + 66 + 3 * 28 + 32;
+    // state 36 B, Barnes-Hut scratch 67 B, 3 node slots x 28 B, partials 32 B
+    return 36 + 67 + 3 * 28 + 32;
 }
 
 gsim_status gsim_batch_set_waves(gsim_batch *b, uint32_t waves_per_simd) {

@@ -72,6 +72,7 @@ struct BatchDevice {
     uint32_t *keys[2] = {}, *idx[2] = {};
     uint32_t *status = nullptr;       // [2][tiles_cap * 256] radix look-back words, by pass parity
     uint32_t *digit_total = nullptr;  // [4 * 256] per-pass digit totals
+    uint32_t *ichunk = nullptr;       // [item_capacity] chunks each work item summed last step (issue priority)
     uint32_t *block_cells = nullptr, *block_leaves = nullptr;
     int8_t *sdep = nullptr;
     uint32_t *cpos = nullptr, *sorted_src = nullptr;

@@ -453,7 +453,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
                               const float4 *__restrict__ node4,
                               const uint32_t *__restrict__ ncount, float2 *__restrict__ pos_out,
                               float2 *__restrict__ vel_out, uint32_t node_cap,
-                              unsigned long long &acc) {
+                              unsigned long long &acc, uint32_t *__restrict__ chunks_io) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[d.sorted_base + g * 64 + lane];
     const bool real = src != kBatchSentinel;
@@ -468,8 +468,12 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     const uint32_t root = __builtin_amdgcn_readfirstlane(st.root);
     Partials P;
     unsigned long long vis = 0;
+    // start at the issue priority this work item reached in the last step
+    const uint32_t prio0 = *chunks_io / kPrioCarry;
+    uint32_t chunks = 0;
     (void)walk_partials<true>(root, end, stage, queue, node4, ncount, px, py, real, nreal, d.eps2,
-                              d.theta2, 0xfu, P, vis, acc, kPrioStepBatch);
+                              d.theta2, 0xfu, P, vis, acc, kPrioStepBatch, prio0, &chunks);
+    if (lane == 0) *chunks_io = chunks;
     if (real) {
         float ax, ay;
         partial_sum(P, ax, ay);
@@ -529,7 +533,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     const float2 *__restrict__ vel, const float *__restrict__ mass,
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ partial,
-    uint32_t node_cap, unsigned long long *__restrict__ wave_acc) {
+    uint32_t node_cap, unsigned long long *__restrict__ wave_acc, uint32_t *__restrict__ ichunk) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t w = threadIdx.x >> 6;
@@ -547,7 +551,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             // kernel has already written the system's output)
             if (!live || st.n_in == 0) continue;
             bb_walk_group(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel, node4, ncount,
-                          pos_out, vel_out, node_cap, acc);
+                          pos_out, vel_out, node_cap, acc, &ichunk[it]);
         } else if (live) {
             bb_ds_tile(item.y, d, pos, mass, partial);
         }
@@ -657,7 +661,7 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.remaining, d.stats, d.counts, d.sorted_src,
                                                d.pos[in], d.vel[in], d.mass, d.node4, d.ncount,
                                                d.pos[out], d.vel[out], d.partial,
-                                               (uint32_t)b->node_capacity, d.wave_acc);
+                                               (uint32_t)b->node_capacity, d.wave_acc, d.ichunk);
     bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.remaining, d.ds_sys, d.ds_prefix,
                                                  d.counts, d.partial, d.pos[in], d.vel[in],
                                                  d.pos[out], d.vel[out]);
