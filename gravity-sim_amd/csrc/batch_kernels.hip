@@ -469,7 +469,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     Partials P;
     unsigned long long vis = 0;
     // start at the issue priority this work item reached in the last step
-    const uint32_t prio0 = *chunks_io / kPrioCarry;
+    const uint32_t prio0 = *chunks_io / kPrioCarryBatch;
     uint32_t chunks = 0;
     (void)walk_partials<true>(root, end, stage, queue, node4, ncount, px, py, real, nreal, d.eps2,
                               d.theta2, 0xfu, P, vis, acc, kPrioStepBatch, prio0, &chunks);

@@ -439,7 +439,10 @@ constexpr uint32_t kPrioStepBatch = 32;
 // A single-system group also starts at level (chunks it summed last step) /
 // kPrioCarry: the heavy groups lead from the first instruction.  Measured on
 // the same workload: 24 -> traversal a further -4% (16: none, 32: -2%, 64: +1%).
+// The batched force launch does the same per work item with kPrioCarryBatch
+// (16 best of 12, 16, 24, 40 on the mixed batch).
 constexpr uint32_t kPrioCarry = 24;
+constexpr uint32_t kPrioCarryBatch = 16;
 
 __device__ __forceinline__ void set_issue_priority(uint32_t level) {
     if (level >= 3) __builtin_amdgcn_s_setprio(3);
