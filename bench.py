@@ -273,8 +273,19 @@ def bench_serve(seconds: float, budget_gb: float = 200.0, max_active: int = 0,
     th = threading.Thread(target=client)
     t0 = time.perf_counter()
     th.start()
-    th.join()
-    drained = srv.drain(timeout_s)
+    while th.is_alive():  # arrivals: a progress line every 20 s of them
+        th.join(20.0)
+        if th.is_alive():
+            st = srv.stats()
+            log(f"  arrivals: {len(rids)} submitted, {st['completed']} done")
+    # drain in slices, reporting what has completed, until idle or timeout_s
+    deadline = time.perf_counter() + timeout_s
+    drained = False
+    while not drained and time.perf_counter() < deadline:
+        drained = srv.drain(min(30.0, max(deadline - time.perf_counter(), 0.0)))
+        if not drained:
+            st = srv.stats()
+            log(f"  draining: {st['completed']} of {len(rids)} done")
     wall = time.perf_counter() - t0
     st = srv.stats()
     lat, queue = [], []
