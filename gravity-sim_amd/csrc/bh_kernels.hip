@@ -464,17 +464,20 @@ __device__ __forceinline__ CellSums child_sums(const uint32_t *__restrict__ keys
     uint32_t c = slot + 1u, f = k, n = 0u, y = 1u;
     // at most four children (distinct quadrants)
     for (int it = 0; it < 4 && f < n_in && c < n_nodes; ++it) {
+        // one round of loads per child: both candidate sums are read before
+        // the count says which one the child is
         const uint32_t key = keys[f];
-        if ((key >> (shift + 2u)) != prefix) break;
         const uint32_t cy = ncount[c];
+        const float4 p = prod[f];
+        const float4 s = nsum[c];
+        if ((key >> (shift + 2u)) != prefix) break;
         float4 v;
         uint32_t cn;
         if (cy == 1u) {  // a particle node
-            const float4 p = prod[f];
             v = make_float4(0.0f + p.x, 0.0f + p.y, 0.0f + p.z, 0.0f);
             cn = 1u;
         } else {
-            v = nsum[c];
+            v = s;
             cn = __float_as_uint(v.w);
         }
         const uint32_t q = (key >> shift) & 3u;
