@@ -30,6 +30,11 @@ struct RankStats {
     uint32_t pad;
 };
 
+// The leaf kernel lists the (system, depth-5 tile) pairs holding leaves in
+// kLiveLists lists picked by a hash of the pair, each with its own counter:
+// one counter for all of them serialises thousands of same-address atomics.
+constexpr uint32_t kLiveLists = 64;
+
 struct BatchCounts {
     uint32_t n_bh;            // Barnes-Hut systems (ranks)
     uint32_t n_bh_particles;  // sum of their padded counts = sorted positions
@@ -39,9 +44,10 @@ struct BatchCounts {
     uint32_t n_cells;         // cell nodes of all trees (emit)
     uint32_t n_leaves;        // occupied leaves of all trees (emit)
     uint32_t n_sys;           // all systems (direct sum and Barnes-Hut)
-    uint32_t n_live_tiles;    // (system, depth-5 tile) pairs holding leaves (leaf kernel)
+    uint32_t pad;
     uint32_t sort_stall;      // radix look-back waits that hit their bound (never expected)
     uint32_t sort_tile[4];    // tile counter of each radix pass
+    uint32_t n_live[kLiveLists];  // (system, depth-5 tile) pairs holding leaves, per list
     unsigned long long accepts;  // Barnes-Hut real-lane acceptances, all steps
 };
 
@@ -79,7 +85,7 @@ struct BatchDevice {
     float4 *prod = nullptr, *leaf_sum = nullptr;
     uint32_t *leaf_first = nullptr, *leaf_key = nullptr;
     uint32_t *tile_range = nullptr;
-    uint32_t *live_tiles = nullptr;  // tiles with leaves, in no particular order
+    uint32_t *live_tiles = nullptr;  // [kLiveLists][live_per_list] tiles with leaves, any order
     TileCell *top = nullptr;
     // nodes of all trees
     float4 *node4 = nullptr;

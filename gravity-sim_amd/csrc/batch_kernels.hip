@@ -52,13 +52,13 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     for (uint32_t t = tid; t < nb * 2 * kTiles; t += stride) tile_range[t] = 0;
     for (uint32_t t = tid; t < nb; t += stride) stats[t] = RankStats{0u, 0u, 0u, 0u};
     for (uint32_t t = tid; t < 4 * 256; t += stride) digit_total[t] = 0;
+    for (uint32_t t = tid; t < kLiveLists; t += stride) counts->n_live[t] = 0;
     // the first radix pass's look-back words (each pass clears the next one's)
     const uint32_t status_words = (n + kSortTile - 1) / kSortTile * 256;
     for (uint32_t t = tid; t < status_words; t += stride) status[t] = 0;
     if (tid == 0) {
         counts->n_cells = 0;
         counts->n_leaves = 0;
-        counts->n_live_tiles = 0;
         counts->sort_stall = 0;
     }
     if (tid < 4) counts->sort_tile[tid] = 0;
@@ -283,6 +283,14 @@ __global__ __launch_bounds__(256) void bb_emit_kernel(
 
 // ---- leaf sums and particle runs ------------------------------------------------
 
+// The live list of (system r, depth-5 tile c), T = r kTiles + c: (c + 23 r)
+// mod 64.  Each system gives each list exactly kTiles / 64 of its tiles, so a
+// list never holds more than max_systems kTiles / 64 (its capacity), and the
+// odd stride spreads the systems' few central tiles over the lists.
+__device__ __forceinline__ uint32_t live_list(uint32_t T) {
+    return ((T & (kTiles - 1u)) + 23u * (T / kTiles)) & (kLiveLists - 1u);
+}
+
 __device__ __forceinline__ uint32_t tile_of(uint32_t key) {
     return (key >> kBatchRankShift) * kTiles + ((key >> 10) & (kTiles - 1));
 }
@@ -293,7 +301,7 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
     const uint32_t *__restrict__ cpos, const uint32_t *__restrict__ breaker,
     BatchCounts *counts, const uint32_t *__restrict__ bh_prefix,
     const RankStats *__restrict__ stats, float4 *__restrict__ leaf_sum,
-    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ live_tiles,
+    uint32_t *__restrict__ tile_range, uint32_t *__restrict__ live_tiles, uint32_t live_per_list,
     float4 *__restrict__ node4) {
     __shared__ float4 stage[4][64];
     const uint32_t n = counts->n_bh_particles, nl = counts->n_leaves, ncells = counts->n_cells;
@@ -321,7 +329,8 @@ __global__ __launch_bounds__(256) void bb_leaf_kernel(
             const uint32_t T = tile_of(key);
             if (L == 0 || tile_of(leaf_key[L - 1]) != T) {
                 tile_range[2 * T] = L;
-                live_tiles[atomicAdd(&counts->n_live_tiles, 1u)] = T;
+                const uint32_t q = live_list(T);
+                live_tiles[q * live_per_list + atomicAdd(&counts->n_live[q], 1u)] = T;
             }
             if (L + 1 == nl || tile_of(leaf_key[L + 1]) != T) tile_range[2 * T + 1] = L + 1;
         }
@@ -368,13 +377,16 @@ __global__ __launch_bounds__(256) void bb_tile_kernel(
     const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
     const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
     const BatchCounts *__restrict__ counts, const uint32_t *__restrict__ live_tiles,
-    float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top) {
+    uint32_t live_per_list, float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
+    TileCell *__restrict__ top) {
     __shared__ TileCell l10[1024];
     __shared__ TileCell up[341];
-    const uint32_t nlive = counts->n_live_tiles;
+    // gridDim.x / kLiveLists workgroups per list (gridDim.x a multiple of kLiveLists)
+    const uint32_t q = blockIdx.x % kLiveLists, per = gridDim.x / kLiveLists;
+    const uint32_t nlive = counts->n_live[q];
     const uint32_t t = threadIdx.x;
-    for (uint32_t i = blockIdx.x; i < nlive; i += gridDim.x) {  // only tiles holding leaves
-        const uint32_t T = live_tiles[i];
+    for (uint32_t i = blockIdx.x / kLiveLists; i < nlive; i += per) {  // only tiles holding leaves
+        const uint32_t T = live_tiles[q * live_per_list + i];
         const uint32_t lb = tile_range[2 * T], le = tile_range[2 * T + 1];
         for (uint32_t c = t; c < 1024; c += 256)
             l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
@@ -624,6 +636,7 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     BatchDevice &d = b->d;
     hipStream_t st = b->stream;
     const uint32_t g_part = (uint32_t)std::min<size_t>((b->capacity + 255) / 256, 4096);
+    const uint32_t live_per_list = (uint32_t)(((size_t)b->max_systems * kTiles + kLiveLists - 1) / kLiveLists);
     const uint32_t g_tiles = (uint32_t)std::min<size_t>(b->tiles_cap, 2048);
     const uint32_t g_hist = (uint32_t)std::min<size_t>(b->tiles_cap, 1024);
     bb_keys_kernel<<<g_part, 256, 0, st>>>(d.desc, d.bh_sys, d.bh_prefix, d.remaining, d.counts, d.pos[in],
@@ -650,11 +663,11 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
                                           d.sorted_src, d.breaker, d.stats);
     bb_leaf_kernel<<<4096, 256, 0, st>>>(d.leaf_first, d.leaf_key, d.prod, sk, d.cpos, d.breaker,
                                          d.counts, d.bh_prefix, d.stats, d.leaf_sum,
-                                         d.tile_range, d.live_tiles, d.node4);
-    const uint32_t g_tile = (uint32_t)std::min<size_t>((size_t)b->max_systems * kTiles, 4096);
+                                         d.tile_range, d.live_tiles, live_per_list, d.node4);
+    const uint32_t g_tile = 4096;  // 64 workgroups per live list
     bb_tile_kernel<<<g_tile, 256, 0, st>>>(d.tile_range, d.leaf_first, d.leaf_key, d.leaf_sum,
-                                           d.sdep, d.cpos, d.counts, d.live_tiles, d.node4,
-                                           d.ncount, d.top);
+                                           d.sdep, d.cpos, d.counts, d.live_tiles, live_per_list,
+                                           d.node4, d.ncount, d.top);
     bb_top_kernel<<<std::min<uint32_t>(b->max_systems, 256), 1024, 0, st>>>(
         d.top, d.tile_range, d.sdep, d.cpos, d.counts, d.node4, d.ncount);
     const uint32_t waves = 1024u * b->waves_per_simd;
