@@ -240,18 +240,18 @@ __device__ __forceinline__ float4 lane_leaf_sum(const float4 *__restrict__ prod,
 
 // A larger leaf [f, e) (wave-uniform bounds) by the whole wave: 64 products
 // per coalesced load (the next chunk in flight while the current one is
-// summed) staged in the wave's LDS as packed 12-byte (m*x, m*y, m) records,
-// so three 16-byte broadcast reads bring four; a full chunk is unrolled so the
-// reads run ahead of the three dependent add chains, which set the pace
-// (every lane computes the same running sum).
+// summed) staged in the wave's LDS as packed 12-byte (m*x, m*y, m) records.
+// Lanes 0, 1 and 2 each sum one component, so the wave issues one add per
+// product; each component's order is the leaf's particle order, from +0.  A
+// full chunk is unrolled so the reads run ahead of the dependent add chain.
 __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__restrict__ prod,
                                                 uint32_t f, uint32_t e, uint32_t lane) {
     typedef float v4 __attribute__((ext_vector_type(4)));  // stays in registers
     const v4 *vp = reinterpret_cast<const v4 *>(prod);
     const v4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     float *st = reinterpret_cast<float *>(stage);
-    const v4 *st4 = reinterpret_cast<const v4 *>(stage);
-    float sx = 0.0f, sy = 0.0f, sm = 0.0f;
+    const uint32_t comp = lane < 3u ? lane : 0u;  // the component this lane sums
+    float s = 0.0f;
     v4 next = f + lane < e ? vp[f + lane] : zero;
     for (uint32_t base = f; base < e; base += 64) {
         const uint32_t cnt = min(64u, e - base);
@@ -264,27 +264,18 @@ __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__r
         if (cnt == 64u) {
 #pragma unroll
             for (int k = 0; k < 64; k += 16) {
-                v4 r[12];
+                float r[16];
 #pragma unroll
-                for (int u = 0; u < 12; ++u) r[u] = st4[3 * (k / 4) + u];
-                const float *q = reinterpret_cast<const float *>(r);
+                for (int u = 0; u < 16; ++u) r[u] = st[3 * (k + u) + comp];
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    sx = sx + q[3 * u];
-                    sy = sy + q[3 * u + 1];
-                    sm = sm + q[3 * u + 2];
-                }
+                for (int u = 0; u < 16; ++u) s = s + r[u];
             }
         } else {
-            for (uint32_t k = 0; k < cnt; ++k) {
-                sx = sx + st[3 * k];
-                sy = sy + st[3 * k + 1];
-                sm = sm + st[3 * k + 2];
-            }
+            for (uint32_t k = 0; k < cnt; ++k) s = s + st[3 * k + comp];
         }
         __builtin_amdgcn_wave_barrier();
     }
-    return make_float4(sx, sy, sm, 0.0f);
+    return make_float4(__shfl(s, 0, 64), __shfl(s, 1, 64), __shfl(s, 2, 64), 0.0f);
 }
 
 // ---- the walk of one W = 64 group --------------------------------------------
