@@ -233,6 +233,10 @@ __global__ __launch_bounds__(256) void bb_emit_kernel(
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     for (uint32_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
         const uint32_t k = blk * 256 + threadIdx.x;
+        // the particle's gather does not depend on the scan below: start it first
+        const uint32_t i = k < n ? sidx[k] : 0u;
+        const float2 p = k < n ? pos[i] : make_float2(0.0f, 0.0f);
+        const float m = k < n ? mass[i] : 0.0f;
         const BPos q = bpos_info(keys, n, k);
         uint32_t a = q.ns, b = q.lf;
         wave_scan2(a, b);
@@ -258,9 +262,6 @@ __global__ __launch_bounds__(256) void bb_emit_kernel(
             sorted_src[k] = kBatchSentinel;
             continue;
         }
-        const uint32_t i = sidx[k];
-        const float2 p = pos[i];
-        const float m = mass[i];
         const uint32_t slot = k + C + q.ns;
         node4[slot] = make_float4(p.x, p.y, m, 0.0f);
         ncount[slot] = 1u;

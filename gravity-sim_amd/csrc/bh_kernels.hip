@@ -279,6 +279,10 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     uint32_t *__restrict__ gran_first, uint32_t *__restrict__ gran_key) {
     __shared__ uint32_t wa[4], wb[4], wc[4];
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    // the particle's gather does not depend on the scan below: start it first
+    const uint32_t i = k < na ? sidx[k] : 0u;
+    const float2 p = k < na ? pos[i] : make_float2(0.0f, 0.0f);
+    const float m = k < na ? mass[i] : 0.0f;
     const PosInfo q = pos_info(keys, na, k, bits);
     uint32_t a = q.ns, b = q.lf;
     wave_scan2(a, b);
@@ -314,9 +318,6 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         return;
     }
     const uint32_t C = ea, L = eb;
-    const uint32_t i = sidx[k];
-    const float2 p = pos[i];
-    const float m = mass[i];
     const uint32_t slot = k + C + q.ns;
     // ParticleSortShader.comp:99-107 (canonical order)
     node4[slot] = make_float4(p.x, p.y, m, 0.0f);
