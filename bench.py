@@ -42,8 +42,8 @@ DS_SYM_ISSUED_PER_PAIR = 13.3125
 DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
 # HBM bytes per bh_traverse64_kernel launch (the default traversal: one wave
 # per group, staged runs) at 500,032 particles (same counters):
-# 2 x 75,828 + 33,945 KiB
-BH_TRAFFIC_BYTES = (2 * 75828 + 33945) * 1024
+# 2 x 75,874 + 33,931 KiB
+BH_TRAFFIC_BYTES = (2 * 75874 + 33931) * 1024
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
