@@ -60,6 +60,9 @@ void free_sim(gsim_sim *s) {
     for (auto &p : s->pending)
         for (hipEvent_t e : p.e) s->event_pool.push_back(e);
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+    if (s->ev_join) (void)hipEventDestroy(s->ev_join);
+    if (s->side) (void)hipStreamDestroy(s->side);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
 }
@@ -286,6 +289,12 @@ gsim_status gsim_create(const gsim_params *params, const float *pos_xy, const fl
         e = gsim::bh_alloc(&s->bh, na, gsim::kTreeDepth);
         if (e != hipSuccess)
             return bail(fail(GSIM_ERROR_OUT_OF_MEMORY, "Barnes-Hut buffers: %s", hipGetErrorString(e)));
+        // the heavy list's walks run on a second stream (default priority),
+        // forked from and joined back to s->stream by events
+        e = hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev_join, hipEventDisableTiming);
+        if (e != hipSuccess) return bail(hip_fail(e, "side stream"));
     }
     *out = s;
     return GSIM_OK;

@@ -664,7 +664,8 @@ __device__ __forceinline__ void walk_group(
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
     float theta2, float kv, float kp, unsigned long long *__restrict__ gvis,
     unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gcost = nullptr,
-    uint32_t *__restrict__ gchunk = nullptr) {
+    uint32_t *__restrict__ gchunk = nullptr, const uint8_t *__restrict__ heavy = nullptr) {
+    if (heavy && heavy[g]) return;  // walked by bh_traverse_heavy_kernel
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[g * 64 + lane];
     const bool real = src < na;  // ForceShader.comp:80-92
@@ -698,6 +699,163 @@ __device__ __forceinline__ void walk_group(
         if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
     }
     if (gcost && lane == 0) gcost[g] = cost;
+}
+
+// The static traversal's heavy list from the last step's chunk counts
+// (gchunk): the first kMaxHeavy groups whose count exceeds split_x times the
+// mean go to plan[0, n_heavy) and are flagged in heavy[] (every other flag is
+// cleared).  With no counts yet (the first step) the list is empty.  One
+// workgroup.
+__global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__restrict__ gchunk,
+                                                             uint32_t groups, uint32_t split_x,
+                                                             uint32_t *__restrict__ plan,
+                                                             uint8_t *__restrict__ heavy,
+                                                             BhCounters *__restrict__ ctr) {
+    __shared__ uint32_t wsum[16];
+    __shared__ unsigned long long total_sh;
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned long long part = 0;
+    for (uint32_t g = t; g < groups; g += 1024) part += gchunk[g];
+    if (t == 0) total_sh = 0;
+    __syncthreads();
+    atomicAdd(&total_sh, part);
+    __syncthreads();
+    const unsigned long long total = total_sh;
+    uint32_t carry = 0;  // heavy groups before this chunk of 1024
+    for (uint32_t base = 0; base < groups; base += 1024) {
+        const uint32_t g = base + t;
+        const bool h = g < groups && total > 0 &&
+                       (unsigned long long)gchunk[g] * groups > (unsigned long long)split_x * total;
+        uint32_t x = h ? 1u : 0u;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off, 64);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        uint32_t before = carry + (h ? x - 1u : x);
+        uint32_t chunk_total = 0;
+        for (uint32_t w = 0; w < 16; ++w) {
+            if (w < wave) before += wsum[w];
+            chunk_total += wsum[w];
+        }
+        const bool listed = h && before < kMaxHeavy;
+        if (listed) plan[before] = g;
+        if (g < groups) heavy[g] = listed ? 1u : 0u;
+        carry += chunk_total;
+        __syncthreads();  // wsum is rewritten by the next chunk
+    }
+    if (t == 0) ctr->n_heavy = min(carry, kMaxHeavy);
+}
+
+// The heavy list, one workgroup of four waves per group, beside the main
+// traversal (which skips these groups): wave 0 decides -- the walk of
+// walk_partials, same votes and counts -- and queues the particle runs in the
+// workgroup's LDS, then all four waves sum the queued chunks, wave k those of
+// partial k, wave 0 also holding the accepted cells; at the end the shares
+// (split_share) are added in order by wave 0, which integrates.  The same
+// bits as one wave walking the group, with the summing spread over four SIMDs.
+template <bool kDebugAccel>
+__global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
+    const uint32_t *__restrict__ plan, const BhCounters *__restrict__ ctr,
+    const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
+    const float2 *__restrict__ vel, const float4 *__restrict__ node4,
+    const uint32_t *__restrict__ ncount, float2 *__restrict__ pos_out,
+    float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap,
+    float eps2, float theta2, float kv, float kp, unsigned long long *__restrict__ gvis,
+    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk) {
+    __shared__ float4 stage[4][64];
+    __shared__ uint2 runq[kRunQueue];
+    __shared__ uint32_t qs[3];  // queued runs, their first chunk number, walk finished
+    __shared__ float2 share[4][64];
+    if (blockIdx.x >= ctr->n_heavy) return;  // the whole workgroup
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t g = __builtin_amdgcn_readfirstlane(plan[blockIdx.x]);
+    // node count is bounded by the node arrays; every step advances ind by at
+    // least one, so the walk ends
+    const uint32_t end = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
+    const uint32_t src = sorted_src[g * 64 + lane];
+    const bool real = src < na;  // ForceShader.comp:80-92
+    float px = 0.0f, py = 0.0f;
+    if (real) {
+        const float2 p = pos[src];
+        px = p.x;
+        py = p.y;
+    }
+    const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
+    __builtin_amdgcn_s_setprio(3);  // these groups are the launch's tail
+    Partials P;
+    unsigned long long vis = 0, acc = 0;
+    uint32_t ind = 0, chunk = 0;  // wave 0's walk
+    while (true) {
+        if (w == 0) {
+            const uint32_t qch = chunk;
+            uint32_t qn = 0;
+            while (ind < end && qn < kRunQueue) {
+                const float4 nd = node4[ind];
+                const uint32_t cnt = ncount[ind];
+                if (nd.w <= 0.0f) {  // a particle node: w = -(particle nodes left in its run)
+                    const uint32_t pr = (uint32_t)(-nd.w);
+                    const uint32_t n = __builtin_amdgcn_readfirstlane(min(max(pr, 1u), end - ind));
+                    vis += (unsigned long long)n * nreal;
+                    acc += (unsigned long long)n * nreal;
+                    if (lane == 0) runq[qn] = make_uint2(ind, n);
+                    ++qn;
+                    chunk += (n + 63u) >> 6;
+                    ind += n;
+                    continue;
+                }
+                const float dx = nd.x - px;
+                const float dy = nd.y - py;
+                const float r2 = (dx * dx + dy * dy) + eps2;  // ForceShader.comp:124-125
+                const bool ok = (nd.w <= r2 * theta2) || !real;
+                vis += nreal;
+                if (__ballot(ok) == ~0ull) {
+                    const float inv = __builtin_amdgcn_rsqf(r2);
+                    const float sc = (nd.z * inv) * (inv * inv);
+                    P.xc = __builtin_fmaf(dx, sc, P.xc);
+                    P.yc = __builtin_fmaf(dy, sc, P.yc);
+                    acc += nreal;
+                    ind = __builtin_amdgcn_readfirstlane(ind + max(cnt, 1u));
+                } else {
+                    ind = __builtin_amdgcn_readfirstlane(ind + 1u);
+                }
+            }
+            if (lane == 0) {
+                qs[0] = qn;
+                qs[1] = qch;
+                qs[2] = ind >= end ? 1u : 0u;
+            }
+        }
+        __syncthreads();
+        const uint32_t qn = __builtin_amdgcn_readfirstlane(qs[0]);
+        const uint32_t qch = __builtin_amdgcn_readfirstlane(qs[1]);
+        const uint32_t done = __builtin_amdgcn_readfirstlane(qs[2]);
+        if (qn) sum_run_queue(runq, qn, qch, stage[w], node4, px, py, eps2, 1u << w, P);
+        __syncthreads();  // runq and qs are rewritten by the next batch
+        if (done) break;
+    }
+    float ax, ay;
+    split_share(P, w, ax, ay);
+    share[w][lane] = make_float2(ax, ay);
+    __syncthreads();
+    __builtin_amdgcn_s_setprio(0);
+    if (w != 0) return;
+    if (lane == 0) {
+        gvis[g] = vis;
+        gacc[g] = acc;
+        gchunk[g] = chunk;
+    }
+    if (real) {
+        ax = ((share[0][lane].x + share[1][lane].x) + share[2][lane].x) + share[3][lane].x;
+        ay = ((share[0][lane].y + share[1][lane].y) + share[2][lane].y) + share[3][lane].y;
+        const float2 v = vel[src];
+        const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
+        pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+        vel_out[src] = make_float2(nvx, nvy);
+        if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+    }
 }
 
 // Exclusive prefix of (cost + 1) over the groups, and the total at [groups]
@@ -880,7 +1038,8 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     BhCounters *__restrict__ ctr, float2 *__restrict__ pos_out, float2 *__restrict__ vel_out,
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, unsigned long long *__restrict__ gvis,
-    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk) {
+    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk,
+    const uint8_t *__restrict__ heavy) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
     __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -905,7 +1064,7 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
         if (g >= groups) return;  // whole wave beyond the last group
         walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
                                          pos_out, vel_out, acc_dbg, na, eps2, theta2, kv,
-                                         kp, gvis, gacc, nullptr, gchunk);
+                                         kp, gvis, gacc, nullptr, gchunk, heavy);
     }
 }
 
@@ -1050,6 +1209,8 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->box_part, kBoxParts);
     GSIM_ALLOC(b->gcost, na / 64);
     GSIM_ALLOC(b->gchunk, na / 64 + 1);
+    GSIM_ALLOC(b->plan, kMaxHeavy);
+    GSIM_ALLOC(b->heavy, na / 64 + 1);
     GSIM_ALLOC(b->gvis, na / 64 + 1);
     GSIM_ALLOC(b->gacc, na / 64 + 1);
     GSIM_ALLOC(b->gpre, na / 64 + 1);
@@ -1073,7 +1234,7 @@ void bh_free(BhBuffers *b) {
     void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
-                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->gpre, b->gvis, b->gacc, b->box,
+                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->plan, b->heavy, b->gpre, b->gvis, b->gacc, b->box,
                     b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
@@ -1178,6 +1339,29 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         }
         const bool persistent = (s->trav_mode & 1u) != 0, staged = (s->trav_mode & 2u) != 0;
         uint32_t g = grid;
+        // static staged launch with more groups than the GPU holds at once:
+        // last step's heaviest groups go to four-wave workgroups on the side
+        // stream (forked after the plan, joined back below), beside the main
+        // launch, whose later rounds leave them room.  When every group is
+        // resident from the start the side workgroups only find room at the
+        // end and lengthen the launch (two disks at 500k: +5%), so they stay
+        // off there.
+        const uint8_t *heavy = nullptr;
+        if (!persistent && staged && s->trav_split && s->side && groups > kHeavyMinGroups) {
+            bh_heavy_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gchunk, groups, s->trav_split, b.plan,
+                                                            b.heavy, b.ctr);
+            hipError_t e = hipEventRecord(s->ev_fork, s->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->ev_fork, 0);
+            if (e != hipSuccess) return e;
+#define GSIM_HEAVY(D)                                                                        \
+    bh_traverse_heavy_kernel<D><<<kMaxHeavy, 256, 0, s->side>>>(                              \
+        b.plan, b.ctr, b.sorted_src, pos, vel, b.node4, b.ncount, pos_out, vel_out, acc, na, \
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk)
+            if (dbg) GSIM_HEAVY(true);
+            else GSIM_HEAVY(false);
+#undef GSIM_HEAVY
+            heavy = b.heavy;
+        }
         if (persistent) {
             // waves_per_simd waves on each of the 1024 SIMDs, at most one per group
             uint32_t waves = 1024u * s->trav_waves_per_simd;
@@ -1187,7 +1371,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #define GSIM_TRAV(D, S, P)                                                                 \
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
-        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk)
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk, heavy)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
             else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }
@@ -1196,6 +1380,11 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
             else { if (staged) GSIM_TRAV(false, true, false); else GSIM_TRAV(false, false, false); }
         }
 #undef GSIM_TRAV
+        if (heavy) {  // the next launch on the stream waits for the heavy list too
+            hipError_t e = hipEventRecord(s->ev_join, s->side);
+            if (e == hipSuccess) e = hipStreamWaitEvent(s->stream, s->ev_join, 0);
+            if (e != hipSuccess) return e;
+        }
     } else {
         if (dbg)
             bh_traverse32_kernel<true><<<grid, 256, 0, s->stream>>>(

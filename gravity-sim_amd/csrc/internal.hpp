@@ -44,9 +44,16 @@ struct BhCounters {
     uint32_t next_group;         // work counter of the persistent traversal
     uint32_t n_gran;             // 15-bit leaves: occupied depth-10 cells
     uint32_t sort_stall;         // look-back waits that hit their bound (never expected)
-    uint32_t pad;
+    uint32_t n_heavy;            // static traversal: groups on the heavy list this step
     uint32_t sort_tile[4];       // tile counter of each radix pass
 };
+
+// Static traversal: at most this many groups per step go to the heavy list
+// (walked by four waves each beside the main launch).
+constexpr uint32_t kMaxHeavy = 512;
+// ... and only when the launch has more groups than the GPU holds at once
+// (256 CUs x 4 SIMDs x 8 waves of the static traversal).
+constexpr uint32_t kHeavyMinGroups = 8192;
 
 struct TileCell {  // one cell of the dense tile/top reduction
     uint32_t n, y, f, pad;
@@ -89,6 +96,8 @@ struct BhBuffers {
     // the exclusive prefix of (cost + 1) that cuts the groups into segments
     uint32_t *gcost = nullptr;       // [Na/64], zero before the first walk
     uint32_t *gchunk = nullptr;      // [Na/64 + 1] chunks each group summed last step (issue priority)
+    uint32_t *plan = nullptr;        // [kMaxHeavy] the heavy list
+    uint8_t *heavy = nullptr;        // [Na/64 + 1] 1: on the heavy list this step
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
     // each group's real-lane node visits and acceptances in the last walk,
     // written by the group's wave and summed on a dump (one same-address
@@ -141,6 +150,8 @@ struct gsim_sim {
     gsim_params params{};
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;        // Barnes-Hut: the heavy list's walks, beside the main one
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t n = 0, na = 0;
     // FP32 constants squared/multiplied on the host as the reference does
     // (DirectSimulation.cpp:179, BarnesHutSimulation.cpp:575-578).
@@ -151,7 +162,8 @@ struct gsim_sim {
                                        // bit 2 cost-balanced segments (staged);
                                        // default 2: one wave per group, staged
     uint32_t trav_segments = 512;      // balanced traversal: workgroups (2 per CU)
-    uint32_t trav_split = 5;           // balanced traversal: split groups above this x mean cost
+    uint32_t trav_split = 2;           // static: heavy list above this x mean chunk count;
+                                       // balanced: split above this x mean cost (0: never)
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
     uint32_t ds_rcp = 0;               // direct sum: 1/r2 from the transcendental unit
     // state

@@ -296,6 +296,33 @@ def test_bh_tree_bit_exact_many_sort_tiles(bits):
         assert np.array_equal(t[k], ot[k]), k
 
 
+@pytest.mark.parametrize("split", [1, 2])
+def test_bh_heavy_list_walks_match_the_main_walk(split):
+    """With more groups than the GPU holds at once (9,375 here), from the
+    second step on the groups whose walk summed more than `split` times the
+    mean chunk count last step are walked by four waves each on a second
+    stream beside the main launch: every step's result is bit-identical to one
+    wave per group (heavy list off), and so are the counts."""
+    _say(f"bh heavy list, split {split}")
+    p = gsim.generate("two-exponential-disks", 600000, 33.0, 1.0, 10.0, seed=207)
+    out = []
+    for sp in (0, split):
+        s = _sim(p, "barnes-hut")
+        s.set_tuning(gsim.TUNE_BH_SPLIT, sp)
+        steps = []
+        for _ in range(3):
+            s.step(1)
+            steps.append((s.dump(gsim.DUMP_ACCEL), s.dump(gsim.DUMP_COUNTERS)))
+        g = s.read()
+        s.close()
+        out.append((steps, g))
+    for (a0, c0), (a1, c1) in zip(out[0][0], out[1][0]):
+        assert np.array_equal(a0, a1)
+        assert np.array_equal(c0, c1)
+    assert np.array_equal(out[0][1].pos, out[1][1].pos)
+    assert np.array_equal(out[0][1].vel, out[1][1].vel)
+
+
 def test_bh_leaf_sizes_at_chunk_boundaries():
     """Leaves of 24 (the largest summed by one lane), 25, 63, 64, 65, 127,
     128, 129, 200 and 1,000 particles: the wave-summed leaves mix full 64-node
