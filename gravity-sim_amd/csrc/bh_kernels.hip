@@ -702,30 +702,63 @@ __device__ __forceinline__ void walk_group(
 }
 
 // The static traversal's heavy list from the last step's chunk counts
-// (gchunk): the first kMaxHeavy groups whose count exceeds split_x times the
-// mean go to plan[0, n_heavy) and are flagged in heavy[] (every other flag is
-// cleared).  With no counts yet (the first step) the list is empty.  One
-// workgroup.
+// (gchunk): with q = floor(4 x chunks / mean) - 4 x split_x, the groups with
+// q >= qt go to plan[0, n_heavy) and are flagged in heavy[] (every other flag
+// is cleared), qt >= 0 the smallest threshold that lists at most `cap` groups
+// (the heaviest, whatever their index); q saturates at 63, and if more than
+// `cap` groups reach it the first `cap` of them by index are listed.  With no
+// counts yet (the first step) the list is empty.  One workgroup.
 __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__restrict__ gchunk,
                                                              uint32_t groups, uint32_t split_x,
+                                                             uint32_t cap,
                                                              uint32_t *__restrict__ plan,
                                                              uint8_t *__restrict__ heavy,
                                                              BhCounters *__restrict__ ctr) {
     __shared__ uint32_t wsum[16];
+    __shared__ uint32_t hist[64];
+    __shared__ uint32_t qt_sh;
     __shared__ unsigned long long total_sh;
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
     unsigned long long part = 0;
     for (uint32_t g = t; g < groups; g += 1024) part += gchunk[g];
     if (t == 0) total_sh = 0;
+    if (t < 64) hist[t] = 0;
     __syncthreads();
     atomicAdd(&total_sh, part);
     __syncthreads();
     const unsigned long long total = total_sh;
+    const bool any = total > 0 && split_x > 0 && cap > 0;
+    auto quarter = [&](uint32_t g) -> uint32_t {  // q as above, 64 below the multiple
+        const unsigned long long q = 4ull * gchunk[g] * groups / total;
+        const unsigned long long lo = 4ull * split_x;
+        return q < lo ? 64u : (q - lo > 63ull ? 63u : (uint32_t)(q - lo));
+    };
+    if (any)
+        for (uint32_t g = t; g < groups; g += 1024) {
+            const uint32_t q = quarter(g);
+            if (q < 64) atomicAdd(&hist[q], 1u);
+        }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t qt = 64;  // 64: nothing listed
+        if (any) {
+            uint32_t above = 0;  // groups with q >= k
+            for (int k = 63; k >= 0; --k) {
+                above += hist[k];
+                if (above > cap && k < 63) break;
+                qt = (uint32_t)k;
+            }
+        }
+        qt_sh = qt;
+    }
+    __syncthreads();
+    const uint32_t qt = qt_sh;
     uint32_t carry = 0;  // heavy groups before this chunk of 1024
     for (uint32_t base = 0; base < groups; base += 1024) {
         const uint32_t g = base + t;
-        const bool h = g < groups && total > 0 &&
-                       (unsigned long long)gchunk[g] * groups > (unsigned long long)split_x * total;
+        uint32_t q = 64;
+        if (g < groups && qt < 64) q = quarter(g);
+        const bool h = q < 64 && q >= qt;
         uint32_t x = h ? 1u : 0u;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -740,13 +773,13 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
             if (w < wave) before += wsum[w];
             chunk_total += wsum[w];
         }
-        const bool listed = h && before < kMaxHeavy;
+        const bool listed = h && before < cap;
         if (listed) plan[before] = g;
         if (g < groups) heavy[g] = listed ? 1u : 0u;
         carry += chunk_total;
         __syncthreads();  // wsum is rewritten by the next chunk
     }
-    if (t == 0) ctr->n_heavy = min(carry, kMaxHeavy);
+    if (t == 0) ctr->n_heavy = min(carry, cap);
 }
 
 // The heavy list, one workgroup of four waves per group, beside the main
@@ -1348,8 +1381,8 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         // off there.
         const uint8_t *heavy = nullptr;
         if (!persistent && staged && s->trav_split && s->side && groups > kHeavyMinGroups) {
-            bh_heavy_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gchunk, groups, s->trav_split, b.plan,
-                                                            b.heavy, b.ctr);
+            bh_heavy_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gchunk, groups, s->trav_split,
+                                                            kMaxHeavy, b.plan, b.heavy, b.ctr);
             hipError_t e = hipEventRecord(s->ev_fork, s->stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->ev_fork, 0);
             if (e != hipSuccess) return e;

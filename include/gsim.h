@@ -176,10 +176,10 @@ typedef enum gsim_tuning {
                                          wave, 0 = by size                              */
     GSIM_TUNE_BH_SEGMENTS = 6,        /* balanced traversal: workgroups of 16 waves
                                          (default 512, two per CU)                      */
-    GSIM_TUNE_BH_SPLIT = 7,           /* groups whose last walk exceeds this multiple
+    GSIM_TUNE_BH_SPLIT = 7,           /* groups whose last walk reaches this multiple
                                          of the mean are walked by 4 waves: in the
-                                         default traversal (chunk count, at most 512
-                                         groups, beside the main launch) and the
+                                         default traversal (chunk count, the heaviest
+                                         512 at most, beside the main launch) and the
                                          balanced one (cost); 0 = never; default 2     */
     GSIM_TUNE_BH_ADAPTIVE_BOX = 8,    /* 0/1, beyond the reference and off by default:
                                          each step's tree spans the bounding square of

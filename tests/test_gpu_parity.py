@@ -299,8 +299,8 @@ def test_bh_tree_bit_exact_many_sort_tiles(bits):
 @pytest.mark.parametrize("split", [1, 2])
 def test_bh_heavy_list_walks_match_the_main_walk(split):
     """With more groups than the GPU holds at once (9,375 here), from the
-    second step on the groups whose walk summed more than `split` times the
-    mean chunk count last step are walked by four waves each on a second
+    second step on the heaviest groups (at least `split` times the mean chunk
+    count last step, at most 512) are walked by four waves each on a second
     stream beside the main launch: every step's result is bit-identical to one
     wave per group (heavy list off), and so are the counts."""
     _say(f"bh heavy list, split {split}")
