@@ -115,12 +115,12 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
 }
 
 // kDeep: 30-bit keys of the 32768^2 grid (15-bit leaves) instead of 20-bit cells.
-// One sort tile (kSortTile particles, kSortRounds per thread) per workgroup.
+// One sort tile (kR x kSortThreads particles, kR per thread) per workgroup.
 // The digits of every radix pass are counted into digit_total, this build's
 // half of the totals (zeroed by the previous build's keys kernel, or at
 // allocation), while digit_next, the other half, is zeroed for the next build;
 // the first radix pass's look-back words and counters are zeroed as well.
-template <bool kBox, bool kDeep>
+template <bool kBox, bool kDeep, int kR>
 __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
@@ -143,15 +143,15 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * kSortTile;
-    float2 pr[kSortRounds];  // all loads first: the kill below stores to pos
+    const uint32_t base = blockIdx.x * (kR * kSortThreads);
+    float2 pr[kR];  // all loads first: the kill below stores to pos
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
         pr[r] = i < na ? pos[i] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
         if (i >= na) continue;
         const float2 p = pr[r];
@@ -184,13 +184,14 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
 
 // One pass, one tile per workgroup: digit (key >> shift) & 255.  digit_total:
 // this pass's 256 totals; status: this pass's [tiles][256] words, zeroed;
-// tile_ctr: zeroed.
+// tile_ctr: zeroed.  Tiles of kR x kSortThreads keys.
+template <int kR>
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
     uint32_t shift, const uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status,
     uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ stall) {
-    __shared__ SortShared<false> sh;
+    __shared__ SortShared<false, kR> sh;
     if (threadIdx.x == 0) sh.tile = atomicAdd(tile_ctr, 1u);
     sort_tile_clear(sh);
     const uint32_t gbase = digit_base(digit_total, sh);  // synchronises
@@ -1288,27 +1289,42 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
         box = b.box;
     }
     const int passes = deep ? 4 : 3;
-    const uint32_t nblk = (uint32_t)b.sort_blocks;
+    // one-sweep passes are fastest with about one tile per CU: the look-back
+    // chain grows with the tile count (500k: 245 tiles of 2048; 1M: 245 of
+    // 4096 beats 489 of 2048 by 14 us a build).  The buffers hold the tiles
+    // of the smaller size.
+    const bool big = (size_t)na > kSortBigTiles * kSortTile;
+    const uint32_t tile_keys = big ? 2u * kSortTile : (uint32_t)kSortTile;
+    const uint32_t nblk = (na + tile_keys - 1) / tile_keys;
     const uint32_t status_words = (uint32_t)passes * nblk * 256u;
     // digit totals alternate between two halves by build parity
     uint32_t *const totals = b.digit_total + 1024 * b.parity;
     uint32_t *const totals_next = b.digit_total + 1024 * (b.parity ^ 1u);
     b.parity ^= 1u;
-#define GSIM_KEYS(B, D)                                                                    \
-    bh_keys_kernel<B, D><<<nblk, kSortThreads, 0, s->stream>>>(                             \
+#define GSIM_KEYS(B, D, R)                                                                 \
+    bh_keys_kernel<B, D, R><<<nblk, kSortThreads, 0, s->stream>>>(                          \
         pos, mass, pos_out, vel, vel_out, na, b.keys[0], b.idx[0], b.tile_range, totals,     \
         totals_next, b.status, status_words, b.ctr, box)
-    if (box) { if (deep) GSIM_KEYS(true, true); else GSIM_KEYS(true, false); }
-    else { if (deep) GSIM_KEYS(false, true); else GSIM_KEYS(false, false); }
+#define GSIM_KEYS_R(R)                                                                     \
+    if (box) { if (deep) GSIM_KEYS(true, true, R); else GSIM_KEYS(true, false, R); }         \
+    else { if (deep) GSIM_KEYS(false, true, R); else GSIM_KEYS(false, false, R); }
+    if (big) { GSIM_KEYS_R(2 * kSortRounds) } else { GSIM_KEYS_R(kSortRounds) }
+#undef GSIM_KEYS_R
 #undef GSIM_KEYS
     // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
     // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
     const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
     for (int pass = 0; pass < passes; ++pass) {
         const int i = in_of[pass], o = out_of[pass];
-        radix_onesweep_kernel<<<nblk, kSortThreads, 0, s->stream>>>(
-            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass,
-            b.status + (size_t)pass * nblk * 256, &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
+        uint32_t *const st = b.status + (size_t)pass * nblk * 256;
+        if (big)
+            radix_onesweep_kernel<2 * kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
+                b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
+                &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
+        else
+            radix_onesweep_kernel<kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
+                b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
+                &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
     }
     b.sorted_slot = deep ? 2 : 1;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];

@@ -48,6 +48,16 @@ struct BhCounters {
     uint32_t sort_tile[4];       // tile counter of each radix pass
 };
 
+// Radix sort passes (single and batched builds): tiles of kSortTile keys,
+// kSortRounds per thread of a kSortThreads workgroup.  Look-back status
+// buffers hold 256 words per tile.
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 8;
+constexpr int kSortTile = kSortThreads * kSortRounds;
+// The single-system build sorts tiles of 2 x kSortTile keys when tiles of
+// kSortTile would number more than this.
+constexpr size_t kSortBigTiles = 384;
+
 // Static traversal: at most this many groups per step go to the heavy list
 // (walked by four waves each beside the main launch).
 constexpr uint32_t kMaxHeavy = 512;

@@ -19,22 +19,23 @@
 // (kSpinLimit re-reads, counted in *stall).
 #pragma once
 #include "bh_device.hpp"
+#include "internal.hpp"
 
 namespace gsim {
 
-constexpr int kSortThreads = 256;
-constexpr int kSortRounds = 8;
-constexpr int kSortTile = kSortThreads * kSortRounds;
-constexpr int kSortRows = kSortRounds * (kSortThreads / 64);
 constexpr uint32_t kStA = 1u << 30, kStP = 2u << 30, kStVal = kStA - 1u;
 constexpr int kLookWindow = 4;             // predecessors read at once per digit
 constexpr uint32_t kSpinLimit = 1u << 20;  // bound on re-reads of one status word
 
-template <bool kStage>
+// kR: rounds of kSortThreads keys per tile (kSortRounds unless the launch
+// picks another tile size).
+template <bool kStage, int kR = kSortRounds>
 struct SortShared {
-    uint32_t slot[kSortRows * 256];  // per-(round, wave) digit counts, then offsets
-    uint32_t stage_key[kStage ? kSortTile : 1];  // the tile in digit order
-    uint32_t stage_val[kStage ? kSortTile : 1];
+    static constexpr int kRows = kR * (kSortThreads / 64);
+    static constexpr int kTile = kR * kSortThreads;
+    uint32_t slot[kRows * 256];  // per-(round, wave) digit counts, then offsets
+    uint32_t stage_key[kStage ? kTile : 1];  // the tile in digit order
+    uint32_t stage_val[kStage ? kTile : 1];
     uint32_t first[256];             // tile position of the first key of each digit
     uint32_t adj[256];               // output position - tile position, per digit
     uint32_t wsum[4];
@@ -119,9 +120,9 @@ __device__ __forceinline__ void look_back(const uint32_t *st, int j, uint32_t *e
 
 // Exclusive prefix of this pass's digit totals for digit threadIdx.x
 // (synchronises the workgroup).
-template <bool kStage>
+template <bool kStage, int kR>
 __device__ __forceinline__ uint32_t digit_base(const uint32_t *__restrict__ digit_total,
-                                               SortShared<kStage> &sh) {
+                                               SortShared<kStage, kR> &sh) {
     const uint32_t total = digit_total[threadIdx.x];
     uint32_t all = 0;
     return block_scan256(total, &all, sh.wsum) - total;
@@ -129,36 +130,38 @@ __device__ __forceinline__ uint32_t digit_base(const uint32_t *__restrict__ digi
 
 // Clears this thread's column of sh.slot (sort_tile's precondition, ordered
 // by a barrier of the caller).
-template <bool kStage>
-__device__ __forceinline__ void sort_tile_clear(SortShared<kStage> &sh) {
+template <bool kStage, int kR>
+__device__ __forceinline__ void sort_tile_clear(SortShared<kStage, kR> &sh) {
 #pragma unroll
-    for (int k = 0; k < kSortRows; ++k) sh.slot[k * 256 + threadIdx.x] = 0;
+    for (int k = 0; k < SortShared<kStage, kR>::kRows; ++k) sh.slot[k * 256 + threadIdx.x] = 0;
 }
 
 // Sorts tile `tile` of keys_in/idx_in on digit (key >> shift) & 255 into
 // keys_out/idx_out.  gbase: digit_base() of this thread's digit; status: this
 // pass's [tiles][256] words; sh.slot cleared by sort_tile_clear() before a
 // barrier.  Ends with sh in use: a barrier must precede its next use.
-template <bool kStage>
+template <bool kStage, int kR>
 __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           const uint32_t *__restrict__ idx_in,
                                           uint32_t *__restrict__ keys_out,
                                           uint32_t *__restrict__ idx_out, uint32_t n,
                                           uint32_t shift, uint32_t tile, uint32_t gbase,
                                           uint32_t *__restrict__ status,
-                                          uint32_t *__restrict__ stall, SortShared<kStage> &sh) {
+                                          uint32_t *__restrict__ stall,
+                                          SortShared<kStage, kR> &sh) {
+    constexpr int kRows = SortShared<kStage, kR>::kRows;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const uint32_t base = tile * kSortTile;
-    uint32_t key[kSortRounds], val[kSortRounds], rank[kSortRounds];
+    const uint32_t base = tile * SortShared<kStage, kR>::kTile;
+    uint32_t key[kR], val[kR], rank[kR];
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
         key[r] = i < n ? keys_in[i] : 0u;
         val[r] = i < n ? idx_in[i] : 0u;
     }
     const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const bool valid = base + r * kSortThreads + t < n;
         const uint32_t d = (key[r] >> shift) & 255u;
         const uint64_t m = digit_peers(d, valid);
@@ -168,7 +171,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
     __syncthreads();
     uint32_t c = 0;  // keys of digit t before row k, then the tile's count
 #pragma unroll
-    for (int k = 0; k < kSortRows; ++k) {
+    for (int k = 0; k < kRows; ++k) {
         const uint32_t x = sh.slot[k * 256 + t];
         sh.slot[k * 256 + t] = c;
         c += x;
@@ -194,7 +197,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
     __syncthreads();
     if (!kStage) {
 #pragma unroll
-        for (int r = 0; r < kSortRounds; ++r) {
+        for (int r = 0; r < kR; ++r) {
             if (base + r * kSortThreads + t < n) {
                 const uint32_t d = (key[r] >> shift) & 255u;
                 const uint32_t off =
@@ -208,7 +211,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
         return;
     }
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < kR; ++r) {
         if (base + r * kSortThreads + t < n) {
             const uint32_t d = (key[r] >> shift) & 255u;
             const uint32_t at =
@@ -219,7 +222,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < kSortRounds; ++r) {
+    for (int r = 0; r < kR; ++r) {
         const uint32_t at = r * kSortThreads + t;
         if (at < tile_n) {
             const uint32_t k = sh.stage_key[at];

@@ -277,9 +277,10 @@ def test_bh_tree_bit_exact(case):
 
 @pytest.mark.parametrize("bits", [10, 15])
 def test_bh_tree_bit_exact_many_sort_tiles(bits):
-    """3,000,000 particles: 1,465 radix tiles of 2,048 keys, more than the GPU
-    holds at once, so tiles wait on predecessors that started in an earlier
-    round of workgroups.  Every node array equals the oracle's sparse build
+    """3,000,000 particles: 733 radix tiles of 4,096 keys (the tile size of
+    builds above 786,432 particles), more than the GPU holds at once (two per
+    CU), so tiles wait on predecessors that started in an earlier round of
+    workgroups.  Every node array equals the oracle's sparse build
     (itself pinned to the dense restatement at 10 bits)."""
     _say(f"bh tree two-disks-3000000 at {bits} bits")
     p = gsim.generate("two-exponential-disks", 3000000, 33.0, 1.0, 10.0, seed=206)
