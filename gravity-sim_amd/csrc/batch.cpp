@@ -227,7 +227,7 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     b->node_capacity = 3 * cap + 64;
     b->partial_capacity = 4 * cap;
     b->item_capacity = cap / 8 + cap / 64 + 16 * (size_t)max_systems + 64;
-    b->tiles_cap = (cap + gsim::kSortTile - 1) / gsim::kSortTile;
+    b->tiles_cap = (cap + gsim::kBatchSortTile - 1) / gsim::kBatchSortTile;
     b->blocks_cap = (cap + 255) / 256;
     b->free_ranges[0] = (uint32_t)cap;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);

@@ -53,6 +53,11 @@ struct BatchCounts {
 
 constexpr uint32_t kUnlimitedSteps = 0xffffffffu;
 
+// Radix tiles of the batched sort: kBatchSortRounds keys per thread (3,072 a
+// tile; 2,048 measured 1.7% slower over the mixed 256-system batch).
+constexpr int kBatchSortRounds = 12;
+constexpr int kBatchSortTile = kBatchSortRounds * kSortThreads;
+
 constexpr uint32_t kBatchRankShift = 21;                   // key = rank << 21 | cell (bit 20 = killed)
 constexpr uint32_t kBatchMaxSystems = 1u << (32 - kBatchRankShift);  // 2048
 constexpr uint32_t kBatchSentinel = 0xffffffffu;

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     for (uint32_t t = tid; t < 4 * 256; t += stride) digit_total[t] = 0;
     for (uint32_t t = tid; t < kLiveLists; t += stride) counts->n_live[t] = 0;
     // the first radix pass's look-back words (each pass clears the next one's)
-    const uint32_t status_words = (n + kSortTile - 1) / kSortTile * 256;
+    const uint32_t status_words = (n + kBatchSortTile - 1) / kBatchSortTile * 256;
     for (uint32_t t = tid; t < status_words; t += stride) status[t] = 0;
     if (tid == 0) {
         counts->n_cells = 0;
@@ -108,15 +108,16 @@ __global__ __launch_bounds__(kSortThreads) void bb_hist_kernel(
     for (int p = 0; p < 4; ++p) h[p][t] = 0;
     __syncthreads();
     const uint32_t n = counts->n_bh_particles;
-    for (uint32_t base = blockIdx.x * kSortTile; base < n; base += gridDim.x * kSortTile) {
-        uint32_t k[kSortRounds];
+    for (uint32_t base = blockIdx.x * kBatchSortTile; base < n; base += gridDim.x * kBatchSortTile) {
+        uint32_t k[kBatchSortRounds];
 #pragma unroll
-        for (int r = 0; r < kSortRounds; ++r) {
+        for (int r = 0; r < kBatchSortRounds; ++r) {
             const uint32_t i = base + r * kSortThreads + t;
             k[r] = i < n ? keys[i] : 0u;
         }
 #pragma unroll
-        for (int r = 0; r < kSortRounds; ++r) hist_add<4, true>(h, k[r], base + r * kSortThreads + t < n);
+        for (int r = 0; r < kBatchSortRounds; ++r)
+            hist_add<4, true>(h, k[r], base + r * kSortThreads + t < n);
     }
     __syncthreads();
 #pragma unroll
@@ -135,9 +136,9 @@ __global__ __launch_bounds__(kSortThreads) void bb_onesweep_kernel(
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out,
     BatchCounts *__restrict__ counts, uint32_t pass, const uint32_t *__restrict__ digit_total,
     uint32_t *__restrict__ status, uint32_t *__restrict__ status_next) {
-    __shared__ SortShared<true> sh;
+    __shared__ SortShared<true, kBatchSortRounds> sh;
     const uint32_t n = counts->n_bh_particles;
-    const uint32_t ntiles = (n + kSortTile - 1) / kSortTile;
+    const uint32_t ntiles = (n + kBatchSortTile - 1) / kBatchSortTile;
     const uint32_t gbase = digit_base(digit_total + 256 * pass, sh);  // synchronises
     for (;;) {
         if (threadIdx.x == 0) sh.tile = atomicAdd(&counts->sort_tile[pass], 1u);
