@@ -516,7 +516,7 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
             *written = 2 * sizeof(uint64_t);
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
             {
-                const size_t groups = na / 64;
+                const size_t groups = na / s->W;
                 std::vector<unsigned long long> v(groups), a(groups);
                 if (groups) {
                     GSIM_HIP(hipMemcpy(v.data(), s->bh.gvis, groups * 8, hipMemcpyDeviceToHost));

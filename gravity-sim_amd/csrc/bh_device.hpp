@@ -340,7 +340,10 @@ constexpr uint32_t kRunQueue = 64;
 // ch0: each chunk this wave owns ((parts >> (chunk & 3)) & 1) is loaded with
 // one coalesced load while the previous one is summed (across run
 // boundaries), staged in the wave's LDS as packed 12-byte (x, y, m) records,
-// summed node by node from zero and added to its partial.
+// summed node by node from zero and added to its partial.  kHalf: the wave
+// holds one 32-particle group twice (lane l and l + 32 the same particle) and
+// each half sums its 32 nodes of every chunk (nodes 0-31: lanes 0-31).
+template <bool kHalf = false>
 __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, uint32_t ch0,
                                               float4 *stage, const float4 *__restrict__ node4,
                                               float px, float py, float eps2, uint32_t parts,
@@ -380,8 +383,12 @@ __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, u
         st[3 * lane + 2] = cur.z;
         __builtin_amdgcn_wave_barrier();
         float cx = 0.0f, cy = 0.0f;
-        uint32_t j = 0;
-        for (; j + 8 <= cnt; j += 8) {
+        uint32_t j = 0, jend = cnt;
+        if (kHalf) {
+            j = (lane >> 5) << 5;
+            jend = min(cnt, j + 32u);
+        }
+        for (; j + 8 <= jend; j += 8) {
             f4v r[6];
 #pragma unroll
             for (int u = 0; u < 6; ++u) r[u] = st4[3 * (j / 4) + u];
@@ -390,7 +397,7 @@ __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, u
             for (int u = 0; u < 8; ++u)
                 accumulate(q[3 * u], q[3 * u + 1], q[3 * u + 2], px, py, eps2, cx, cy);
         }
-        for (; j < cnt; ++j)
+        for (; j < jend; ++j)
             accumulate(st[3 * j], st[3 * j + 1], st[3 * j + 2], px, py, eps2, cx, cy);
         add_partial(P, k, cx, cy);
         __builtin_amdgcn_wave_barrier();
@@ -442,13 +449,16 @@ __device__ __forceinline__ void set_issue_priority(uint32_t level) {
     else __builtin_amdgcn_s_setprio(0);
 }
 
-template <bool kStaged>
+// kHalf (staged only): one 32-particle group held twice by the wave, see
+// sum_run_queue; the votes are the group's, as both halves vote alike.
+template <bool kStaged, bool kHalf = false>
 __device__ __forceinline__ uint32_t walk_partials(
     uint32_t ind, uint32_t end, float4 *stage, uint2 *queue, const float4 *__restrict__ node4,
     const uint32_t *__restrict__ ncount, float px, float py, bool real, uint32_t nreal,
     float eps2, float theta2, uint32_t parts, Partials &P, unsigned long long &vis,
     unsigned long long &acc, uint32_t prio_step, uint32_t prio0 = 0,
     uint32_t *chunks_out = nullptr) {
+    static_assert(kStaged || !kHalf, "half-wave groups need the staged walk");
     const uint32_t lane = threadIdx.x & 63;
     const bool first = (parts & 1u) != 0;
     uint32_t chunk = 0, cells = 0, runs = 0, qn = 0, qch = 0;
@@ -530,7 +540,7 @@ __device__ __forceinline__ uint32_t walk_partials(
               }
           }
           __builtin_amdgcn_wave_barrier();
-          sum_run_queue(queue, qn, qch, stage, node4, px, py, eps2, parts, P);
+          sum_run_queue<kHalf>(queue, qn, qch, stage, node4, px, py, eps2, parts, P);
           qn = 0;
           qch = chunk;
       }

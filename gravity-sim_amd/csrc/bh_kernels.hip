@@ -1102,8 +1102,12 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     }
 }
 
-// W = 32: each half-wave is one group with its own node index (the published
-// runs' subgroup width).  Both halves advance together; a finished half idles.
+// W = 32 (the published runs' subgroup width): one 32-particle group per
+// wave, particle slot g * 32 + l held by lanes l and l + 32, walked by the
+// staged W = 64 walk (both halves vote alike, so the 64-lane vote is the
+// group's: the same visits and acceptances); each half sums 32 nodes of every
+// 64-node chunk.  Per partial k the halves' sums are added, then the
+// acceleration is (((c + p0) + p1) + p2) + p3 as at W = 64.
 template <bool kDebugAccel>
 __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     const uint32_t *__restrict__ sorted_src, const float2 *__restrict__ pos,
@@ -1112,65 +1116,46 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
     uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp,
     unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc) {
-    const uint32_t slot = blockIdx.x * 256 + threadIdx.x;
-    if ((slot & ~63u) >= na) return;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint64_t hm = lane < 32 ? 0x00000000ffffffffull : 0xffffffff00000000ull;
-    const uint32_t src = sorted_src[slot];
-    const bool real = src < na;
+    __shared__ float4 stage[4][64];
+    __shared__ uint2 runq[4][kRunQueue];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t g = blockIdx.x * 4 + w;
+    if (g >= na / 32) return;  // whole wave beyond the last group
+    // node count is bounded by the node arrays; every step advances the walk
+    const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
+    const uint32_t src = sorted_src[g * 32 + (lane & 31u)];
+    const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
         const float2 p = pos[src];
         px = p.x;
         py = p.y;
     }
-    const uint32_t total = min(ctr->n_nodes, cap);
-    float ax = 0.0f, ay = 0.0f;
-    uint32_t ind = 0;
+    const uint32_t nreal = (uint32_t)__popcll(__ballot(real) & 0xffffffffull);
+    Partials P;
     unsigned long long vis = 0, acc = 0;
-    while (true) {
-        const bool active = ind < total;
-        if (__ballot(active) == 0ull) break;
-        float4 nd = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        uint32_t cnt = 1u;
-        if (active) {
-            nd = node4[ind];
-            cnt = ncount[ind];
-        }
-        const float dx = nd.x - px;
-        const float dy = nd.y - py;
-        const float r2 = (dx * dx + dy * dy) + eps2;
-        const bool ok = !active || !real || (nd.w <= r2 * theta2);
-        const bool all = (__ballot(ok) & hm) == hm;
-        if (active) {
-            vis += real ? 1u : 0u;
-            if (all) {
-                const float inv = __builtin_amdgcn_rsqf(r2);
-                const float s = ((nd.z * inv) * inv) * inv;
-                ax = ax + dx * s;
-                ay = ay + dy * s;
-                ind += max(cnt, 1u);
-                acc += real ? 1u : 0u;
-            } else {
-                ind += 1u;
-            }
-        }
+    (void)walk_partials<true, true>(0u, total, stage[w], runq[w], node4, ncount, px, py, real,
+                                    nreal, eps2, theta2, 0xfu, P, vis, acc, kPrioStepSingle);
+    P.x0 += __shfl_xor(P.x0, 32, 64);
+    P.y0 += __shfl_xor(P.y0, 32, 64);
+    P.x1 += __shfl_xor(P.x1, 32, 64);
+    P.y1 += __shfl_xor(P.y1, 32, 64);
+    P.x2 += __shfl_xor(P.x2, 32, 64);
+    P.y2 += __shfl_xor(P.y2, 32, 64);
+    P.x3 += __shfl_xor(P.x3, 32, 64);
+    P.y3 += __shfl_xor(P.y3, 32, 64);
+    if (lane == 0) {  // this group's visits and acceptances, summed on a dump
+        gvis[g] = vis;
+        gacc[g] = acc;
     }
-    if (real) {
+    if (real && lane < 32) {
+        float ax, ay;
+        partial_sum(P, ax, ay);
         const float2 v = vel[src];
         const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
         pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
         vel_out[src] = make_float2(nvx, nvy);
         if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        vis += __shfl_xor(vis, off, 64);
-        acc += __shfl_xor(acc, off, 64);
-    }
-    if (lane == 0) {  // the wave's two half-wave groups, summed on a dump
-        gvis[slot >> 6] = vis;
-        gacc[slot >> 6] = acc;
     }
 }
 
@@ -1245,8 +1230,8 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->gchunk, na / 64 + 1);
     GSIM_ALLOC(b->plan, kMaxHeavy);
     GSIM_ALLOC(b->heavy, na / 64 + 1);
-    GSIM_ALLOC(b->gvis, na / 64 + 1);
-    GSIM_ALLOC(b->gacc, na / 64 + 1);
+    GSIM_ALLOC(b->gvis, na / 32 + 1);  // one slot per group of W = 32 or 64
+    GSIM_ALLOC(b->gacc, na / 32 + 1);
     GSIM_ALLOC(b->gpre, na / 64 + 1);
 #undef GSIM_ALLOC
     if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
@@ -1435,12 +1420,13 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
             if (e != hipSuccess) return e;
         }
     } else {
+        const uint32_t g32 = (na / 32 + 3) / 4;  // four 32-particle groups per workgroup
         if (dbg)
-            bh_traverse32_kernel<true><<<grid, 256, 0, s->stream>>>(
+            bh_traverse32_kernel<true><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
                 (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc);
         else
-            bh_traverse32_kernel<false><<<grid, 256, 0, s->stream>>>(
+            bh_traverse32_kernel<false><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
                 (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc);
     }

@@ -112,8 +112,8 @@ struct BhBuffers {
     // each group's real-lane node visits and acceptances in the last walk,
     // written by the group's wave and summed on a dump (one same-address
     // atomic per wave cost ~5% of the walk)
-    unsigned long long *gvis = nullptr;  // [Na/64 + 1]
-    unsigned long long *gacc = nullptr;  // [Na/64 + 1]
+    unsigned long long *gvis = nullptr;  // [Na/32 + 1] one per group (W = 32 or 64)
+    unsigned long long *gacc = nullptr;  // [Na/32 + 1]
     // adaptive bounding box (opt-in): {cx, cy, h, r0} and per-block extremes
     float *box = nullptr;            // [4]
     float4 *box_part = nullptr;      // [256] (xmin, xmax, ymin, ymax)
