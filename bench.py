@@ -3,7 +3,8 @@
 Headline: direct-sum Ginteractions/s on the 50,000-particle exponential disk
 (scale 20, radius 100; G=1, softening 0.2, dt=1e-3).  The same JSON line also
 carries the Barnes-Hut step at 500,000 particles (two exponential disks,
-theta=1, W=64) because the headline metric names both.
+theta=1, W=32: the group width of the reference's published runs; --bh-width
+64 for wave-wide groups) because the headline metric names both.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--bh-steps K2] [--no-cpu]
                     [--batch-steps K3] [--no-batch] [--serve-seconds S]
@@ -40,10 +41,11 @@ DS_SYM_ISSUED_PER_PAIR = 13.3125
 # x 2 + WRITE_SIZE x 1 (factors from a 256 MiB copy), 2 x 2,417 + 48,950 KiB;
 # profiles/ds50k_bh500k_counters.md
 DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
-# HBM bytes per bh_traverse64_kernel launch (the default traversal: one wave
-# per group, staged runs) at 500,032 particles (same counters):
-# 2 x 75,874 + 33,931 KiB
-BH_TRAFFIC_BYTES = (2 * 75874 + 33931) * 1024
+# HBM bytes per traversal launch at 500,032 particles (same counters), by
+# group width: bh_traverse64_kernel (one wave per group, staged runs)
+# 2 x 75,874 + 33,931 KiB; bh_traverse32_kernel (one 32-particle group per
+# wave) 2 x 82,905 + 33,739 KiB
+BH_TRAFFIC_BYTES = {64: (2 * 75874 + 33931) * 1024, 32: (2 * 82905 + 33739) * 1024}
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
@@ -103,11 +105,13 @@ def bench_direct_sum(steps: int, warmup: int) -> dict:
     }
 
 
-def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = False) -> dict:
+def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = False,
+                     width: int = 32) -> dict:
     """leaf_bits = 15 / box: the opt-in tree options beyond the reference
-    (their trees differ from the reference's, so vs_baseline is null)."""
+    (their trees differ from the reference's, so vs_baseline is null).
+    width: the traversal group width W (32, the published runs', or 64)."""
     p = gsim.generate("two-exponential-disks", BH_N, 33.0, 1.0, 10.0, 1.0, seed=3)
-    sim = gsim.Simulation(p, "barnes-hut", accuracy_parameter=1.0, group_width=64)
+    sim = gsim.Simulation(p, "barnes-hut", accuracy_parameter=1.0, group_width=width)
     if leaf_bits != 10:
         sim.set_tuning(gsim.TUNE_BH_LEAF_BITS, leaf_bits)
     if box:
@@ -150,18 +154,18 @@ def bench_barnes_hut(steps: int, warmup: int, leaf_bits: int = 10, box: bool = F
         "visits": V,
         "accepts": A,
         "config": {"workload": "barnes-hut, two exponential disks of 250,000 (radius 33, "
-                               "scale 7, centres +-40, bulk vx -+2), theta=1, W=64, "
+                               f"scale 7, centres +-40, bulk vx -+2), theta=1, W={width}, "
                                "G=1, softening 0.2, dt=1e-3", "particles": BH_N,
-                   "theta": 1.0, "group_width": 64, "leaf_bits": leaf_bits,
+                   "theta": 1.0, "group_width": width, "leaf_bits": leaf_bits,
                    "adaptive_box": box},
         "roofline": {
             "bound": "valu",
-            "kernel": "bh_traverse64_kernel",
+            "kernel": f"bh_traverse{width}_kernel",
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": BH_TRAFFIC_BYTES,
+            "traffic": BH_TRAFFIC_BYTES[width] if leaf_bits == 10 and not box else None,
             "valu_lane_op_frac": lane_ops / (trav_ms * 1e-3) / VALU_LANE_OPS,
             "build_bytes": build_bytes,
             "build_hbm_frac": build_bytes / (build_ms * 1e-3) / (PEAK_HBM_GBS * 1e9),
@@ -366,9 +370,10 @@ def cpu_baseline(budget_s: float = 12.0) -> dict:
                       f"({threads} threads) and one on 1 thread, integrate excluded"}
 
 
-def cpu_baseline_bh(steps: int = 3) -> dict:
+def cpu_baseline_bh(steps: int = 3, width: int = 32) -> dict:
     """The oracle's whole Barnes-Hut step (dense-pyramid build, group walk,
-    integrate) on the same 500k workload, on up to 16 threads: ms per step."""
+    integrate) on the same 500k workload and group width, on up to 16
+    threads: ms per step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, used here only as the CPU baseline
 
@@ -377,14 +382,15 @@ def cpu_baseline_bh(steps: int = 3) -> dict:
     eps2 = oracle.square_f32(0.2)
     theta2 = oracle.square_f32(1.0)
     pos, vel, mass = p.pos, p.vel, p.mass
-    pos, vel, mass, _, _ = oracle.bh_step(pos, vel, mass, 1.0, 1e-3, eps2, theta2, 64, threads)
+    pos, vel, mass, _, _ = oracle.bh_step(pos, vel, mass, 1.0, 1e-3, eps2, theta2, width, threads)
     t0 = time.perf_counter()
     for _ in range(steps):
-        pos, vel, mass, _, _ = oracle.bh_step(pos, vel, mass, 1.0, 1e-3, eps2, theta2, 64, threads)
+        pos, vel, mass, _, _ = oracle.bh_step(pos, vel, mass, 1.0, 1e-3, eps2, theta2, width,
+                                              threads)
     el = time.perf_counter() - t0
     return {"value": el * 1e3 / steps, "unit": "ms/step", "cores": threads, "kind": "port",
             "sample": f"{steps + 1} whole Barnes-Hut steps of the oracle at 500,032 "
-                      f"particles ({threads} threads)"}
+                      f"particles, W={width} ({threads} threads)"}
 
 
 def copy_calibration(mib: int = 256, reps: int = 3) -> None:
@@ -405,6 +411,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--bh-steps", type=int, default=20)
+    ap.add_argument("--bh-width", type=int, default=32, choices=(32, 64),
+                    help="Barnes-Hut group width W (32: the reference's published runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--bh-leaf-bits", type=int, default=10, choices=(10, 15),
@@ -432,10 +440,10 @@ def main() -> int:
     log(f"  {ds['value']:.1f} Ginteractions/s, {ds['ms_per_step']:.4f} ms/step")
     bh = None
     if not args.no_bh:
-        log(f"barnes-hut 500k: steps {args.bh_steps}, {args.bh_leaf_bits}-bit leaves"
-            + (", adaptive box" if args.bh_adaptive_box else ""))
+        log(f"barnes-hut 500k: steps {args.bh_steps}, {args.bh_leaf_bits}-bit leaves, "
+            f"W={args.bh_width}" + (", adaptive box" if args.bh_adaptive_box else ""))
         bh = bench_barnes_hut(args.bh_steps, max(args.warmup, 1), args.bh_leaf_bits,
-                              args.bh_adaptive_box)
+                              args.bh_adaptive_box, args.bh_width)
         log(f"  {bh['value']:.4f} ms/step (build {bh['build_ms']:.4f}, traverse "
             f"{bh['traverse_ms']:.4f})")
     bt = None
@@ -457,7 +465,7 @@ def main() -> int:
         log(f"  {cpu['value']:.2f} Ginteractions/s on {cpu['cores']} threads, "
             f"{cpu['one_core_value']:.3f} on one")
         if bh is not None:
-            bh["cpu_baseline"] = cpu_baseline_bh()
+            bh["cpu_baseline"] = cpu_baseline_bh(width=args.bh_width)
             log(f"  Barnes-Hut 500k: {bh['cpu_baseline']['value']:.1f} ms/step on "
                 f"{bh['cpu_baseline']['cores']} threads")
     ref_ginter = DS_N * DS_N / (REF_DS_MS_50K * 1e-3) / 1e9
