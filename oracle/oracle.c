@@ -9,9 +9,20 @@
  *                          boundary known-answer cases, 30->20 bit prefix.
  *   oracle_ds_accel_f32    pinned: two-body and ring closed forms, Newton III,
  *                          FP64 sum, bit-exact vs BH traversal at theta = 0.
- *   oracle_ds_accel_f64    pinned: closed forms (two-body, ring, lattice).
- *   oracle_ds_step         pinned: free particle x + v*dt, two-body step.
- *   oracle_ds_step_f64     pinned: two-body orbit energy and momentum drift.
+ *   oracle_ds_accel_f64    pinned: closed forms (two-body, ring centre),
+ *                          Newton III on 512 random bodies.
+ *   oracle_ds_step         pinned: free particle x + v*dt; one two-body
+ *                          step (G != 1) within FP32 rounding of the float64
+ *                          physics; five steps of 128 bodies within FP32
+ *                          rounding of oracle_ds_step_f64.
+ *   oracle_ds_step_f64     pinned: free particle exact over 1 and 7 steps;
+ *                          two bodies from rest and two moving bodies over
+ *                          two steps against Newton's law in numpy (unequal
+ *                          masses, G != 1, softening comparable to the
+ *                          separation); momentum and uniform centre-of-mass
+ *                          motion over 1,000 steps of 64 bodies; a softened
+ *                          circular orbit closing after one period with
+ *                          first-order convergence in dt.
  *   oracle_bh_build        pinned: recursive pointer-quadtree brute force on
  *                          small N, y = n + #cells invariants, dyadic COM.
  *   oracle_bh_build_sparse pinned: bit-identical to oracle_bh_build /
@@ -21,7 +32,9 @@
  *   oracle_bh_traverse     pinned: recursive group-vote brute force for
  *                          W in {1,32,64}, theta = 0 == sorted direct sum,
  *                          theta -> inf == pull of the global COM.
- *   oracle_bh_integrate    pinned: free particle, kill semantics.
+ *   oracle_bh_integrate    pinned: kill semantics (killed particles keep
+ *                          their velocity); a theta = 0 step with G != 1
+ *                          within FP32 rounding of oracle_ds_step_f64.
  */
 #include "oracle.h"
 

@@ -141,6 +141,147 @@ def test_ds_zero_softening_gives_nan_like_reference():
     assert np.isnan(a).all()  # self term 0 * inf
 
 
+# ---- direct-sum steps: the FP32 step and the FP64 trajectory yardstick -----------
+# SimShader.comp:53-69: a = sum_j m_j d (d.d + eps^2)^(-3/2); v' = v + a (dt G);
+# p' = p + ((v + v') / 2) dt.  The expected values below come from Newton's law
+# written out in numpy (float64), from conservation laws and from the orbit's
+# geometry, never from the oracle's code.
+
+def _two_body_expect(pos, mass, vel, G, dt, eps2):
+    """One step of two bodies in float64, written from the physics: each body
+    is pulled toward the other by G m_other d / (|d|^2 + eps^2)^(3/2)."""
+    d = pos[1].astype(np.float64) - pos[0].astype(np.float64)
+    k = 1.0 / (d @ d + eps2) ** 1.5
+    acc = np.array([mass[1] * d * k, -mass[0] * d * k]) * G
+    v1 = vel.astype(np.float64) + acc * dt
+    p1 = pos.astype(np.float64) + (vel.astype(np.float64) + v1) * 0.5 * dt
+    return p1, v1
+
+
+def test_ds_steps_f64_free_particle_is_exact():
+    pos = np.array([[1.25, -2.5]], np.float64)
+    vel = np.array([[3.0, -4.0]], np.float64)
+    for steps in (1, 7):
+        p, v = O.ds_steps_f64(pos, vel, np.array([7.0]), 0.9, 1e-3, 0.04, steps)
+        assert np.array_equal(v, vel)
+        expect = pos.copy()
+        for _ in range(steps):
+            expect = expect + vel * 1e-3  # (v + v) / 2 == v exactly
+        assert np.array_equal(p, expect)
+
+
+@pytest.mark.parametrize("G,eps", [(1.0, 0.2), (0.7, 1.5), (2.5, 0.01)])
+def test_ds_steps_f64_two_bodies_from_rest(G, eps):
+    """Unequal masses (a mass-index swap fails), G != 1 (a dropped or misplaced
+    G fails), softening comparable to the separation (a missing or unsquared
+    eps fails), and the half-step position update (p' = p + v' dt / 2 from
+    rest: a full-step or missing update fails)."""
+    pos = np.array([[0.5, -1.0], [2.0, 1.0]])
+    mass = np.array([2.0, 5.0])
+    vel = np.zeros((2, 2))
+    dt, eps2 = 1e-2, eps * eps
+    p, v = O.ds_steps_f64(pos, vel, mass, G, dt, eps2, 1)
+    pe, ve = _two_body_expect(pos, mass, vel, G, dt, eps2)
+    assert np.allclose(v, ve, rtol=1e-14, atol=0)
+    assert np.allclose(p, pe, rtol=1e-14, atol=0)
+    # the pull points from each body toward the other
+    assert np.dot(v[0], pos[1] - pos[0]) > 0 and np.dot(v[1], pos[0] - pos[1]) > 0
+
+
+def test_ds_steps_f64_two_moving_bodies_two_steps():
+    """Two steps with nonzero velocities: the second step must use the first
+    step's positions and velocities."""
+    pos = np.array([[-1.0, 0.25], [1.5, -0.5]])
+    vel = np.array([[0.3, -0.2], [-0.1, 0.4]])
+    mass = np.array([3.0, 1.5])
+    G, dt, eps2 = 1.3, 5e-3, 0.09
+    p, v = O.ds_steps_f64(pos, vel, mass, G, dt, eps2, 2)
+    pe, ve = _two_body_expect(pos, mass, vel, G, dt, eps2)
+    pe, ve = _two_body_expect(pe, mass, ve, G, dt, eps2)
+    assert np.allclose(v, ve, rtol=1e-13, atol=0)
+    assert np.allclose(p, pe, rtol=1e-13, atol=0)
+
+
+def test_ds_steps_f64_conserves_momentum_and_com_motion():
+    """64 random bodies, 1,000 steps: forces are pairwise equal and opposite,
+    so sum m v is constant and the centre of mass moves uniformly
+    (sum m p' = sum m p + dt (P + P') / 2 = sum m p + dt P) to rounding.  A
+    wrong mass index or an unsymmetric softening breaks both."""
+    rng = np.random.default_rng(11)
+    n = 64
+    pos = rng.uniform(-10, 10, size=(n, 2))
+    vel = rng.normal(0, 1, size=(n, 2))
+    mass = rng.uniform(1, 10, size=n)
+    G, dt, eps2, steps = 1.0, 1e-3, 0.04, 1000
+    p, v = O.ds_steps_f64(pos, vel, mass, G, dt, eps2, steps)
+    P0 = (mass[:, None] * vel).sum(0)
+    P1 = (mass[:, None] * v).sum(0)
+    scale = (mass[:, None] * np.abs(v)).sum()
+    assert np.abs(P1 - P0).max() < 1e-13 * scale * 10
+    com0 = (mass[:, None] * pos).sum(0) / mass.sum()
+    com1 = (mass[:, None] * p).sum(0) / mass.sum()
+    assert np.allclose(com1, com0 + P0 / mass.sum() * dt * steps, rtol=0, atol=1e-10)
+    # and the bodies did interact: velocities changed by far more than rounding
+    assert np.abs(v - vel).max() > 1e-2
+
+
+def test_ds_steps_f64_circular_orbit():
+    """Two equal masses on the softened circular orbit: separation D, speed
+    u = sqrt(G m D^2 / (2 (D^2 + eps^2)^(3/2))) each, period T = pi D / u.
+    After one period the separation is still D and each body is back at its
+    start, to the integrator's error, which falls linearly with dt (its
+    velocity update is first order)."""
+    G, m, D, eps2 = 1.0, 4.0, 2.0, 0.04
+    u = np.sqrt(G * m * D * D / (2.0 * (D * D + eps2) ** 1.5))
+    T = np.pi * D / u
+    drift, errs = [], []
+    for nsteps in (2000, 4000):
+        dt = T / nsteps
+        pos = np.array([[-D / 2, 0.0], [D / 2, 0.0]])
+        vel = np.array([[0.0, -u], [0.0, u]])
+        p, v = O.ds_steps_f64(pos, vel, np.array([m, m]), G, dt, eps2, nsteps)
+        drift.append(abs(np.linalg.norm(p[1] - p[0]) - D))
+        errs.append(np.abs(p - pos).max())
+    assert drift[0] < 0.025 * D and errs[0] < 0.05 * D
+    # first-order convergence: halving dt halves both errors
+    assert 1.8 < drift[0] / drift[1] < 2.2
+    assert 1.8 < errs[0] / errs[1] < 2.2
+
+
+def test_ds_step_f32_two_bodies_within_rounding():
+    """The FP32 step against the float64 physics: within a few FP32 ulp of the
+    velocity change and of the position update."""
+    pos = np.array([[0.5, -1.0], [2.0, 1.0]], np.float32)
+    vel = np.array([[0.25, 0.5], [-0.75, 0.125]], np.float32)
+    mass = np.array([2.0, 5.0], np.float32)
+    G, dt, eps2 = 0.7, 1e-2, H.sq32(0.2)
+    po, vo = O.ds_step(pos, vel, mass, G, dt, eps2)
+    # the oracle rounds dt and G to FP32 as the shader's constants are
+    pe, ve = _two_body_expect(pos, mass, vel, float(np.float32(G)), float(np.float32(dt)), eps2)
+    dv = ve - vel
+    assert np.allclose(vo - vel, dv, rtol=1e-5, atol=0)
+    assert np.allclose(po, pe, rtol=0, atol=4 * np.finfo(np.float32).eps * np.abs(pe).max())
+
+
+def test_ds_step_f32_tracks_f64_trajectory():
+    """The FP32 step and the FP64 yardstick agree to FP32 rounding over a few
+    steps of a 128-body cluster, and the FP64 run is not the FP32 one."""
+    rng = np.random.default_rng(12)
+    n = 128
+    pos = rng.uniform(-20, 20, size=(n, 2)).astype(np.float32)
+    vel = rng.normal(0, 0.5, size=(n, 2)).astype(np.float32)
+    mass = rng.uniform(1, 10, size=n).astype(np.float32)
+    G, dt, eps2 = 1.0, 1e-3, H.sq32(0.2)
+    p32, v32 = pos, vel
+    for _ in range(5):
+        p32, v32 = O.ds_step(p32, v32, mass, G, dt, eps2)
+    p64, v64 = O.ds_steps_f64(pos, vel, mass, G, dt, eps2, 5)
+    disp = np.abs(p64 - pos).max()
+    assert np.abs(p32 - p64).max() < 1e-4 * disp + 8 * np.finfo(np.float32).eps * 20
+    assert np.abs(v32 - v64).max() < 1e-5 * np.abs(v64 - vel).max() + 1e-6
+    assert np.abs(v64 - vel).max() > 1e-3
+
+
 # ---- Barnes-Hut build ------------------------------------------------------------
 
 def _compare_tree(t, b):
@@ -454,3 +595,21 @@ def test_bh_step_integrates_and_keeps_killed_velocity():
     np_ = (pk[1:] + ((vel[1:] + nv).astype(np.float32) * kp)).astype(np.float32)
     assert np.array_equal(vo[1:], nv)
     assert np.array_equal(po[1:], np_)
+
+
+@pytest.mark.parametrize("G,dt", [(1.0, 1e-3), (0.6, 4e-3)])
+def test_bh_step_theta0_tracks_f64_direct_sum_step(G, dt):
+    """At theta = 0 every cell opens, so one Barnes-Hut step is a direct-sum
+    step in another summation order: it must agree with the FP64 yardstick
+    (pinned above against Newton's law) to FP32 rounding, which pins the
+    integrate's G, dt and half-step factors independently of its code."""
+    pos, mass = _input(n=192, seed=13, spread=10.0)
+    rng = np.random.default_rng(14)
+    vel = rng.normal(0, 2, size=pos.shape).astype(np.float32)
+    po, vo, mo, V, A = O.bh_step(pos, vel, mass, G, dt, H.sq32(0.2), 0.0, 64)
+    p64, v64 = O.ds_steps_f64(pos, vel, mass, G, dt, H.sq32(0.2), 1)
+    dv = np.abs(v64 - vel).max()
+    assert dv > 1e-2  # the step does feel the forces
+    assert np.abs(vo - v64).max() < 1e-5 * dv + 4 * np.finfo(np.float32).eps * np.abs(vel).max()
+    assert np.abs(po - p64).max() < 4 * np.finfo(np.float32).eps * np.abs(pos).max()
+    assert np.array_equal(mo, mass)
