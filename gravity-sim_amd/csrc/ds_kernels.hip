@@ -10,6 +10,10 @@
 // in SGPRs: no LDS traffic and no VALU cycles spent on broadcast.  Each slice
 // writes a partial acceleration; the combine kernel adds the S partials in
 // slice order (deterministic) and integrates.
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
 #include "internal.hpp"
 
 namespace gsim {
@@ -368,3 +372,135 @@ hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
 }
 
 }  // namespace gsim
+
+// ---- allocation-free direct-sum step ---------------------------------------------
+//
+// gsim_ds_step_noalloc runs one one-sided direct-sum step on caller-owned
+// device arrays.  Its only scratch, the per-slice partial accelerations, lives
+// in `scratch`, a file-scope device array that every GPU holds for the life of
+// the process, so the call allocates nothing and can run where hipMalloc
+// cannot (under stream capture, inside a serving loop that forbids it).  In
+// host code `scratch` names a host-side stand-in, never device memory: the
+// device address comes from hipGetSymbolAddress, looked up once per device
+// with that device current, and only that address reaches a kernel.  Every
+// call on one GPU shares its array; the partial kernel writes every entry the
+// combine reads (slices x Na), so nothing needs clearing between calls, and
+// calls that overlap on one GPU race on it.
+
+namespace gsim {
+namespace {
+
+constexpr size_t kNoallocScratchEntries = GSIM_NOALLOC_SCRATCH_BYTES / sizeof(float2);
+static __device__ __align__(256) float2 scratch[kNoallocScratchEntries];
+
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+float2 *g_noalloc_ptr[kNoallocMaxDevices] = {};
+
+// Looks up this device's copy of `scratch` (and checks it is a gfx950) on the
+// first call per device; later calls read the cached address.  The caller has
+// made `device` current.
+gsim_status noalloc_setup(int device, float2 **out) {
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) {
+        *out = g_noalloc_ptr[device];
+        return GSIM_OK;
+    }
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess)
+        return fail(GSIM_ERROR_DEVICE, "hipGetDeviceProperties failed: %s", hipGetErrorString(e));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(GSIM_ERROR_NO_DEVICE, "device %d is %s; libgsim is built for gfx950 only",
+                    device, prop.gcnArchName);
+    void *ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(GSIM_ERROR_DEVICE, "hipGetSymbolAddress(scratch) failed: %s",
+                    hipGetErrorString(e));
+    g_noalloc_ptr[device] = static_cast<float2 *>(ptr);
+    *out = g_noalloc_ptr[device];
+    return GSIM_OK;
+}
+
+// Launches the partial and combine kernels; the caller has made the device
+// current and checked every argument.
+gsim_status noalloc_launch(float2 *part, const gsim_params *p, const float2 *pos,
+                           const float2 *vel, const float *mass, float2 *pos_out,
+                           float2 *vel_out, uint32_t na, hipStream_t stream) {
+    uint32_t R = 2, S = 1;
+    ds_pick_shape(na, &R, &S);  // the shape gsim_step picks for a one-sided handle
+    const size_t fit = kNoallocScratchEntries / na;
+    if (S > fit) S = (uint32_t)fit;
+    const uint32_t jlen = ((na + S - 1) / S + 63) / 64 * 64;
+    const uint32_t slices = (na + jlen - 1) / jlen;
+    // constants as gsim_create forms them (DirectSimulation.cpp:179, SimShader.comp:68)
+    const float eps2 = p->softening_len * p->softening_len;
+    const float dt = p->simulation_time;
+    const float kv = dt * p->gravitational_const;
+    const dim3 grid((na + kDsThreads * 2 - 1) / (kDsThreads * 2), slices);
+    ds_partial_kernel<2, false><<<grid, kDsThreads, 0, stream>>>(pos, mass, part, na, jlen, eps2);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        ds_combine_kernel<<<(na + 255) / 256, 256, 0, stream>>>(part, slices, pos, vel, pos_out,
+                                                                vel_out, nullptr, na, kv, dt);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess)
+        return fail(GSIM_ERROR_DEVICE, "direct-sum launch failed: %s", hipGetErrorString(e));
+    return GSIM_OK;
+}
+
+}  // namespace
+}  // namespace gsim
+
+extern "C" gsim_status gsim_ds_step_noalloc(const gsim_params *params, const float *pos_xy,
+                                            const float *vel_xy, const float *mass,
+                                            float *pos_out_xy, float *vel_out_xy, size_t n,
+                                            void *stream) {
+    using gsim::fail;
+    gsim::clear_error();
+    if (!params || !pos_xy || !vel_xy || !mass || !pos_out_xy || !vel_out_xy)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_ds_step_noalloc: NULL argument");
+    if (params->algorithm != GSIM_DIRECT_SUM)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "gsim_ds_step_noalloc: only the direct-sum algorithm runs without allocation");
+    if (n == 0 || n % 64 != 0)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "gsim_ds_step_noalloc: n = %zu must be a nonzero multiple of 64 (pad with "
+                    "zero-mass particles)", n);
+    if (n > gsim::kNoallocScratchEntries)
+        return fail(GSIM_ERROR_CAPACITY,
+                    "gsim_ds_step_noalloc: %zu particles exceed the %zu the %u-byte static "
+                    "scratch holds", n, gsim::kNoallocScratchEntries,
+                    (unsigned)GSIM_NOALLOC_SCRATCH_BYTES);
+    if (pos_out_xy == pos_xy || vel_out_xy == vel_xy || pos_out_xy == vel_xy ||
+        vel_out_xy == pos_xy)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "gsim_ds_step_noalloc: output arrays must not alias the inputs");
+    const float vals[] = {params->gravitational_const, params->simulation_time,
+                          params->softening_len};
+    for (float v : vals)
+        if (!std::isfinite(v))
+            return fail(GSIM_ERROR_INVALID_ARGUMENT, "simulation parameters must be finite");
+    const int device = params->device;
+    if (device < 0 || device >= gsim::kNoallocMaxDevices)
+        return fail(GSIM_ERROR_NO_DEVICE, "gsim_ds_step_noalloc: device %d out of range", device);
+    int prev = -1;
+    hipError_t e = hipGetDevice(&prev);
+    if (e != hipSuccess)
+        return fail(GSIM_ERROR_NO_DEVICE, "hipGetDevice failed: %s", hipGetErrorString(e));
+    if (prev != device && (e = hipSetDevice(device)) != hipSuccess)
+        return fail(GSIM_ERROR_NO_DEVICE, "hipSetDevice(%d) failed: %s", device,
+                    hipGetErrorString(e));
+    float2 *part = nullptr;
+    gsim_status st = gsim::noalloc_setup(device, &part);
+    if (st == GSIM_OK)
+        st = gsim::noalloc_launch(part, params, reinterpret_cast<const float2 *>(pos_xy),
+                                  reinterpret_cast<const float2 *>(vel_xy), mass,
+                                  reinterpret_cast<float2 *>(pos_out_xy),
+                                  reinterpret_cast<float2 *>(vel_out_xy), (uint32_t)n,
+                                  static_cast<hipStream_t>(stream));
+    if (prev != device) (void)hipSetDevice(prev);
+    return st;
+}

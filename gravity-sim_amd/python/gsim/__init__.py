@@ -173,6 +173,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_load_particles": (st, [C.c_char_p, fp, fp, fp, C.c_size_t,
                                      C.POINTER(C.c_size_t)]),
         "gsim_save_particles": (st, [C.c_char_p, fp, fp, fp, C.c_size_t]),
+        "gsim_ds_step_noalloc": (st, [C.POINTER(Params), vp, vp, vp, vp, vp, C.c_size_t,
+                                      vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -196,6 +198,15 @@ def default_params() -> Params:
     p = Params()
     load_library().gsim_default_params(C.byref(p))
     return p
+
+
+def ds_step_noalloc(params: Params, pos_xy: int, vel_xy: int, mass: int, pos_out_xy: int,
+                    vel_out_xy: int, n: int, stream: int | None = None) -> None:
+    """One direct-sum step on caller-owned device arrays (addresses as ints),
+    with no device allocation: gsim_ds_step_noalloc.  Asynchronous on
+    `stream` (a hipStream_t address; None = the null stream)."""
+    _check(load_library().gsim_ds_step_noalloc(C.byref(params), pos_xy, vel_xy, mass,
+                                               pos_out_xy, vel_out_xy, n, stream))
 
 
 def required_alignment(algorithm: int) -> int:

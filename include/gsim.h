@@ -201,6 +201,29 @@ typedef enum gsim_tuning {
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 
+/* ---- allocation-free direct-sum step ---------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP stream capture, serving loops that forbid hipMalloc):
+ * one direct-sum step, the one-sided kernels of gsim_step with the slice count
+ * gsim_set_ds_shape(sim, 2, 0) picks (capped so slices x n fits the scratch),
+ * on caller-owned DEVICE arrays of n particles (n a nonzero multiple of 64;
+ * pad with {0,0,0,0,0} particles as gsim_create does).  Inputs are read and
+ * the outputs written on `stream` (a hipStream_t; NULL = the null stream) of
+ * params->device, asynchronously; outputs must not alias inputs.  G,
+ * simulation_time and softening_len mean what they mean for gsim_create;
+ * simulation_speed, accuracy_parameter and group_width are ignored.
+ * The partial sums live in a static device array of
+ * GSIM_NOALLOC_SCRATCH_BYTES that each GPU the call runs on holds until the
+ * process ends, so n is at most GSIM_NOALLOC_SCRATCH_BYTES / 8 (4,194,304);
+ * larger n returns GSIM_ERROR_CAPACITY.  The call allocates nothing.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU (two
+ * streams, two threads) race and must be serialised by the caller. */
+#define GSIM_NOALLOC_SCRATCH_BYTES (32u << 20)
+gsim_status gsim_ds_step_noalloc(const gsim_params *params, const float *pos_xy,
+                                 const float *vel_xy, const float *mass,
+                                 float *pos_out_xy, float *vel_out_xy, size_t n,
+                                 void *stream);
+
 /* ---- batched form: many independent systems stepped together ---------------
  * A batch is one device arena holding many systems of uneven size (direct sum
  * or Barnes-Hut, each with its own parameters).  Every stage of a step runs
