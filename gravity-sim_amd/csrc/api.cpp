@@ -368,6 +368,14 @@ gsim_status gsim_set_ds_shape(gsim_sim *s, uint32_t R, uint32_t S) {
 gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
     gsim::clear_error();
     if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_set_tuning: NULL handle");
+    // A W = 32 handle has one traversal launch (bh_traverse32_kernel): the
+    // W = 64 launch-shape knobs would be silently ignored, so they are refused.
+    if (s->params.algorithm == GSIM_BARNES_HUT && s->W == 32 &&
+        (knob == GSIM_TUNE_BH_WAVES_PER_SIMD || knob == GSIM_TUNE_BH_TRAVERSAL ||
+         knob == GSIM_TUNE_BH_SPLIT || knob == GSIM_TUNE_BH_SEGMENTS))
+        return fail(GSIM_ERROR_INVALID_ARGUMENT,
+                    "tuning knob %d selects W = 64 traversal launches; a group width 32 "
+                    "handle has one launch shape", knob);
     switch (knob) {
         case GSIM_TUNE_DS_RCP:
             s->ds_rcp = value ? 1u : 0u;

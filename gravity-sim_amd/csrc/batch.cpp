@@ -148,8 +148,9 @@ gsim_status upload_membership(gsim_batch *b) {
     c.n_ds = (uint32_t)ds_sys.size();
     c.n_ds_particles = ds_prefix.back();
     c.n_items = (uint32_t)items.size();
-    // the acceptance total carries over
+    // the acceptance and visit totals carry over
     GSIM_HIP(hipMemcpy(&c.accepts, &b->d.counts->accepts, sizeof(c.accepts), hipMemcpyDeviceToHost));
+    GSIM_HIP(hipMemcpy(&c.visits, &b->d.counts->visits, sizeof(c.visits), hipMemcpyDeviceToHost));
     auto up = [&](void *dst, const void *src, size_t bytes) -> hipError_t {
         return bytes ? hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
     };
@@ -270,11 +271,11 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     GSIM_BALLOC(d.node4, b->node_capacity);
     GSIM_BALLOC(d.ncount, b->node_capacity);
     GSIM_BALLOC(d.breaker, b->node_capacity);
-    GSIM_BALLOC(d.wave_acc, gsim::kMaxForceWaves);
+    GSIM_BALLOC(d.wave_acc, 2 * gsim::kMaxForceWaves);
     GSIM_BALLOC(d.ichunk, b->item_capacity);
 #undef GSIM_BALLOC
     if (e == hipSuccess) e = hipMemset(d.counts, 0, sizeof(gsim::BatchCounts));
-    if (e == hipSuccess) e = hipMemset(d.wave_acc, 0, sizeof(unsigned long long) * gsim::kMaxForceWaves);
+    if (e == hipSuccess) e = hipMemset(d.wave_acc, 0, sizeof(unsigned long long) * 2 * gsim::kMaxForceWaves);
     if (e == hipSuccess) e = hipMemset(d.ichunk, 0, sizeof(uint32_t) * b->item_capacity);
     if (e == hipSuccess) e = hipMemset(d.ncount, 0, sizeof(uint32_t) * b->node_capacity);
     if (e == hipSuccess) e = hipMemset(d.node4, 0, sizeof(float4) * b->node_capacity);
@@ -431,6 +432,17 @@ gsim_status gsim_batch_accepts(gsim_batch *b, uint64_t *accepts) {
     return GSIM_OK;
 }
 
+gsim_status gsim_batch_visits(gsim_batch *b, uint64_t *visits) {
+    gsim::clear_error();
+    if (!b || !visits) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_visits: NULL argument");
+    GSIM_HIP(hipSetDevice(b->device));
+    GSIM_HIP(hipStreamSynchronize(b->stream));
+    unsigned long long v = 0;
+    GSIM_HIP(hipMemcpy(&v, &b->d.counts->visits, sizeof(v), hipMemcpyDeviceToHost));
+    *visits = v;
+    return GSIM_OK;
+}
+
 gsim_status gsim_batch_synchronize(gsim_batch *b) {
     gsim::clear_error();
     if (!b) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_synchronize: NULL handle");
@@ -518,6 +530,9 @@ gsim_status gsim_batch_autotune(gsim_batch *b, uint32_t steps_per_trial, uint32_
     std::vector<uint32_t> rem_snapshot(b->h_counts.n_sys);
     if (!rem_snapshot.empty())
         GSIM_HIP(hipMemcpy(rem_snapshot.data(), b->d.remaining, 4 * rem_snapshot.size(), hipMemcpyDeviceToHost));
+    // ... and add to the acceptance and visit totals: restore those too
+    unsigned long long work0[2] = {0, 0};
+    GSIM_HIP(hipMemcpy(work0, &b->d.counts->accepts, sizeof(work0), hipMemcpyDeviceToHost));
     auto restore = [&]() -> hipError_t {
         hipError_t r = hipStreamSynchronize(b->stream);
         if (r == hipSuccess) r = hipMemcpy(b->d.pos[cur0], snap_pos, sizeof(float2) * cap, hipMemcpyDeviceToDevice);
@@ -525,6 +540,8 @@ gsim_status gsim_batch_autotune(gsim_batch *b, uint32_t steps_per_trial, uint32_
         if (r == hipSuccess) r = hipMemcpy(b->d.mass, snap_mass, sizeof(float) * cap, hipMemcpyDeviceToDevice);
         if (r == hipSuccess && !rem_snapshot.empty())
             r = hipMemcpy(b->d.remaining, rem_snapshot.data(), 4 * rem_snapshot.size(), hipMemcpyHostToDevice);
+        if (r == hipSuccess)
+            r = hipMemcpy(&b->d.counts->accepts, work0, sizeof(work0), hipMemcpyHostToDevice);
         b->cur = cur0;
         b->steps = steps0;
         return r;

@@ -5,6 +5,7 @@
 // within the arena's capacity.
 #pragma once
 
+#include <cstddef>
 #include <map>
 #include <vector>
 
@@ -49,7 +50,11 @@ struct BatchCounts {
     uint32_t sort_tile[4];    // tile counter of each radix pass
     uint32_t n_live[kLiveLists];  // (system, depth-5 tile) pairs holding leaves, per list
     unsigned long long accepts;  // Barnes-Hut real-lane acceptances, all steps
+    unsigned long long visits;   // Barnes-Hut real-lane node visits, all steps
 };
+// the two totals are copied together
+static_assert(offsetof(BatchCounts, visits) == offsetof(BatchCounts, accepts) + 8,
+              "accepts and visits must be adjacent");
 
 constexpr uint32_t kUnlimitedSteps = 0xffffffffu;
 
@@ -95,10 +100,10 @@ struct BatchDevice {
     // nodes of all trees
     float4 *node4 = nullptr;
     uint32_t *ncount = nullptr, *breaker = nullptr;
-    // each force wave's Barnes-Hut acceptances in the last step, summed into
-    // counts->accepts by the step's last kernel (a same-address atomic per
-    // wave costs more than the sum)
-    unsigned long long *wave_acc = nullptr;  // [kMaxForceWaves]
+    // each force wave's Barnes-Hut acceptances, then its node visits, in the
+    // last step, summed into counts->accepts / visits by the step's last
+    // kernel (a same-address atomic per wave costs more than the sum)
+    unsigned long long *wave_acc = nullptr;  // [2 * kMaxForceWaves]
 };
 
 struct BatchSystem {

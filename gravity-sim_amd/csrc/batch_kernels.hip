@@ -467,7 +467,8 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
                               const float4 *__restrict__ node4,
                               const uint32_t *__restrict__ ncount, float2 *__restrict__ pos_out,
                               float2 *__restrict__ vel_out, uint32_t node_cap,
-                              unsigned long long &acc, uint32_t *__restrict__ chunks_io) {
+                              unsigned long long &vis, unsigned long long &acc,
+                              uint32_t *__restrict__ chunks_io) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[d.sorted_base + g * 64 + lane];
     const bool real = src != kBatchSentinel;
@@ -481,7 +482,6 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     const uint32_t end = __builtin_amdgcn_readfirstlane(min(st.end_slot, node_cap));
     const uint32_t root = __builtin_amdgcn_readfirstlane(st.root);
     Partials P;
-    unsigned long long vis = 0;
     // start at the issue priority this work item reached in the last step
     const uint32_t prio0 = *chunks_io / kPrioCarryBatch;
     uint32_t chunks = 0;
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t waves = gridDim.x * 4;
     const uint32_t nitems = counts->n_items;
-    unsigned long long acc = 0;
+    unsigned long long acc = 0, vis = 0;
     for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w); it < nitems;
          it += waves) {
         const uint2 item = items[it];
@@ -565,35 +565,49 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             // kernel has already written the system's output)
             if (!live || st.n_in == 0) continue;
             bb_walk_group(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel, node4, ncount,
-                          pos_out, vel_out, node_cap, acc, &ichunk[it]);
+                          pos_out, vel_out, node_cap, vis, acc, &ichunk[it]);
         } else if (live) {
             bb_ds_tile(item.y, d, pos, mass, partial);
         }
     }
-    if ((threadIdx.x & 63) == 0) wave_acc[blockIdx.x * 4 + w] = acc;
+    if ((threadIdx.x & 63) == 0) {
+        wave_acc[blockIdx.x * 4 + w] = acc;
+        wave_acc[kMaxForceWaves + blockIdx.x * 4 + w] = vis;
+    }
 }
 
 // End of a step: spend one step of every limited budget, and add the force
-// waves' acceptances to the running total (one atomic per block).
+// waves' acceptances and visits to the running totals (one atomic each per
+// block).
 __global__ __launch_bounds__(256) void bb_finish_kernel(uint32_t *__restrict__ remaining,
                                                         BatchCounts *__restrict__ counts,
                                                         const unsigned long long *__restrict__ wave_acc,
                                                         uint32_t nwaves) {
-    __shared__ unsigned long long part[4];
+    __shared__ unsigned long long part[2][4];
     for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < counts->n_sys; s += gridDim.x * 256) {
         const uint32_t r = remaining[s];
         if (r != 0u && r != kUnlimitedSteps) remaining[s] = r - 1u;
     }
-    unsigned long long a = 0;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nwaves; i += gridDim.x * 256)
+    unsigned long long a = 0, v = 0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nwaves; i += gridDim.x * 256) {
         a += wave_acc[i];
+        v += wave_acc[kMaxForceWaves + i];
+    }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_xor(a, off, 64);
-    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = a;
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off, 64);
+        v += __shfl_xor(v, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        part[0][threadIdx.x >> 6] = a;
+        part[1][threadIdx.x >> 6] = v;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
-        if (t) atomicAdd(&counts->accepts, t);
+        const unsigned long long ta = part[0][0] + part[0][1] + part[0][2] + part[0][3];
+        const unsigned long long tv = part[1][0] + part[1][1] + part[1][2] + part[1][3];
+        if (ta) atomicAdd(&counts->accepts, ta);
+        if (tv) atomicAdd(&counts->visits, tv);
     }
 }
 

@@ -1238,8 +1238,8 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     if (e == hipSuccess) e = hipMemset(b->digit_total, 0, sizeof(uint32_t) * 2 * 4 * 256);
     if (e == hipSuccess) e = hipMemset(b->gcost, 0, sizeof(uint32_t) * (na / 64));
     if (e == hipSuccess) e = hipMemset(b->gchunk, 0, sizeof(uint32_t) * (na / 64 + 1));
-    if (e == hipSuccess) e = hipMemset(b->gvis, 0, sizeof(unsigned long long) * (na / 64 + 1));
-    if (e == hipSuccess) e = hipMemset(b->gacc, 0, sizeof(unsigned long long) * (na / 64 + 1));
+    if (e == hipSuccess) e = hipMemset(b->gvis, 0, sizeof(unsigned long long) * (na / 32 + 1));
+    if (e == hipSuccess) e = hipMemset(b->gacc, 0, sizeof(unsigned long long) * (na / 32 + 1));
     if (e == hipSuccess) e = hipMemset(b->ncount, 0, sizeof(uint32_t) * b->cap);
     if (e == hipSuccess) e = hipMemset(b->node4, 0, sizeof(float4) * b->cap);
     return e;

@@ -325,6 +325,9 @@ gsim_status gsim_server_query(gsim_server *s, gsim_server_info *out) {
     }
     gsim_batch_accepts(s->batch, &accepts);
     out->bh_interactions = accepts;
+    uint64_t visits = 0;
+    gsim_batch_visits(s->batch, &visits);
+    out->bh_visits = visits;
     return GSIM_OK;
 }
 

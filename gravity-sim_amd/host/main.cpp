@@ -35,17 +35,22 @@ const char *const kHelp =
     "\t--particles-out: The optional output file in which the variables of the particles will be written once the simulation is finished.\n"
     "\t--particle-count: The number of particles to generate. All generation parameters are ignored if an input file is specified.\n"
     "\t--generate-type: The variant to use for the particle system generation. One of the following options:\n"
-    "\t\trandom, galaxy, galaxy-collision, symmetrical-galaxy-collision\n"
+    "\t\trandom: Randomly distributes particles within the generation confines.\n"
+    "\t\tgalaxy: Simulates the approximate structure of a spiral galaxy spanning the generation confines.\n"
+    "\t\tgalaxy-collision: Simulates the collision of two spiral galaxies, each spanning a third of the generation confines.\n"
+    "\t\tsymmetrical-galaxy-collision: Simulates the collision of two symmetrical spiral galaxies, each spanning a third of the generation confines.\n"
     "\t--generate-size: The radius of the resulting generation's size.\n"
     "\t--min-mass: The minimum mass of the generated particles.\n"
     "\t--max-mass: The maximum mass of the generated particles.\n"
     "\t--gravitational-const: The gravitational constant used for the simulation. Defaulted to 1.\n"
     "\t--simulation-time: The time interval length, in seconds, simulated in one instance. Defaulted to 1e-3.\n"
     "\t--simulation-speed: The speed factor at which the simulation is run. Defaulted to 1.\n"
-    "\t--softening-len: The softening length used to soften close interactions. Defaulted to 0.2.\n"
-    "\t--accuracy-parameter: The Barnes-Hut accuracy parameter. Defaulted to 1.\n"
-    "\t--simulation-algorithm: direct-sum or barnes-hut.\n"
-    "\t--simulation-count: The number of simulations to run before closing the program.\n"
+    "\t--softening-len: The softening length used to soften the extreme forces that would usually result from close interactions. Defaulted to 0.2.\n"
+    "\t--accuracy-parameter: The accuracy parameter used to calibrate force approximation. Only used for Barnes-Hut simulations. Defaulted to 1.\n"
+    "\t--simulation-algorithm: The simulation algorithm used to calculate the gravitational forces. One of the following options:\n"
+    "\t\tdirect-sum: The direct-sum method, calculating every interaction between particles.\n"
+    "\t\tbarnes-hut: The Barnes-Hut algorithm, organizing all particles in a quadtree.\n"
+    "\t--simulation-count: The number of simulations to run before closing the program. No limit will be used if this parameter isn't specified.\n"
     "\t--seed: The seed of the particle generation. Defaulted to the current time.\n"
     "\t--group-width: The Barnes-Hut traversal group width, 64 or 32. Defaulted to 64.\n"
     "\t--device: The GPU to run on. Defaulted to 0.\n"
@@ -56,7 +61,7 @@ const char *const kHelp =
     "Available options:\n"
     "\t--help: Displays the current message and exits the program.\n"
     "\t--log-detailed: Outputs non-crucial logs that might be useful for debugging or additional information.\n"
-    "\t--no-graphics: Runs the simulations in the background (required by this build).\n"
+    "\t--no-graphics: Doesn't display the live positions of all particles, instead running the simulations in the background (required by this build, which has no renderer).\n"
     "\t--benchmark: Benchmarks the required runtime for all simulations. Ignored if --no-graphics isn't specified.\n";
 
 enum Level { DEBUG = 1, INFO = 2, WARNING = 4, ERROR = 8, FATAL = 16 };

@@ -106,6 +106,7 @@ class ServerInfo(C.Structure):
         ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
         ("span_s", C.c_double), ("ds_interactions", C.c_uint64),
         ("bh_interactions", C.c_uint64), ("capacity_particles", C.c_uint64),
+        ("bh_visits", C.c_uint64),
     ]
 
 
@@ -157,6 +158,7 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_batch_set_steps": (st, [vp, C.c_uint32, C.c_uint64]),
         "gsim_batch_steps_done": (st, [vp, C.c_uint32, C.POINTER(C.c_uint64)]),
         "gsim_batch_accepts": (st, [vp, C.POINTER(C.c_uint64)]),
+        "gsim_batch_visits": (st, [vp, C.POINTER(C.c_uint64)]),
         "gsim_server_create": (st, [C.c_int32, C.c_size_t, C.c_uint32, C.c_size_t,
                                     C.POINTER(vp)]),
         "gsim_server_submit": (st, [vp, C.POINTER(Params), fp, fp, fp, C.c_size_t,
@@ -461,6 +463,11 @@ class Batch:
     def accepts(self) -> int:
         v = C.c_uint64(0)
         _check(_lib.gsim_batch_accepts(self._h, C.byref(v)))
+        return v.value
+
+    def visits(self) -> int:
+        v = C.c_uint64(0)
+        _check(_lib.gsim_batch_visits(self._h, C.byref(v)))
         return v.value
 
     def autotune(self, steps_per_trial: int = 3) -> int:

@@ -162,7 +162,9 @@ gsim_status gsim_set_ds_shape(gsim_sim *sim, uint32_t targets_per_lane,
 /* Launch tuning that never changes results beyond the documented tolerance
  * (visit/accept counts and trees are unaffected by every knob), and two
  * opt-in options beyond the reference, GSIM_TUNE_BH_ADAPTIVE_BOX and
- * GSIM_TUNE_BH_LEAF_BITS. */
+ * GSIM_TUNE_BH_LEAF_BITS.  The traversal launch knobs (WAVES_PER_SIMD,
+ * TRAVERSAL, SEGMENTS, SPLIT) shape the W = 64 launches only: on a Barnes-Hut
+ * handle of group width 32 they return GSIM_ERROR_INVALID_ARGUMENT. */
 typedef enum gsim_tuning {
     GSIM_TUNE_DS_RCP = 1,             /* 0/1: 1/r^2 from v_rcp instead of a multiply  */
     GSIM_TUNE_BH_WAVES_PER_SIMD = 2,  /* 1..8: persistent traversal width             */
@@ -263,6 +265,8 @@ gsim_status gsim_batch_set_steps(gsim_batch *batch, uint32_t system_id, uint64_t
 gsim_status gsim_batch_steps_done(gsim_batch *batch, uint32_t system_id, uint64_t *steps);
 /* Barnes-Hut real-lane acceptances (interactions) over all steps so far. */
 gsim_status gsim_batch_accepts(gsim_batch *batch, uint64_t *accepts);
+/* Barnes-Hut real-lane node visits (vote evaluations) over all steps so far. */
+gsim_status gsim_batch_visits(gsim_batch *batch, uint64_t *visits);
 gsim_status gsim_batch_synchronize(gsim_batch *batch);
 gsim_status gsim_batch_read(gsim_batch *batch, uint32_t system_id,
                             float *pos_xy, float *vel_xy, float *mass);
@@ -301,6 +305,7 @@ typedef struct gsim_server_info {
     uint64_t ds_interactions;      /* direct sum: padded N^2 per step, completed requests */
     uint64_t bh_interactions;      /* Barnes-Hut accepted interactions, all steps so far  */
     uint64_t capacity_particles;   /* arena capacity the budget bought        */
+    uint64_t bh_visits;            /* Barnes-Hut real-lane node visits, all steps so far  */
 } gsim_server_info;
 
 /* hbm_budget_bytes sizes the arena (gsim_batch_bytes_per_particle per padded

@@ -11,6 +11,8 @@
  *                          FP64 sum, bit-exact vs BH traversal at theta = 0.
  *   oracle_ds_accel_f64    pinned: closed forms (two-body, ring centre),
  *                          Newton III on 512 random bodies.
+ *   oracle_ds_accel_f64_at pinned: equal, row for row, to oracle_ds_accel_f64
+ *                          on the listed targets (any order, repeats).
  *   oracle_ds_step         pinned: free particle x + v*dt; one two-body
  *                          step (G != 1) within FP32 rounding of the float64
  *                          physics; five steps of 128 bodies within FP32
@@ -154,6 +156,38 @@ void oracle_ds_accel_f64(size_t na, const float *pos, const float *mass,
                          double eps2, double *acc, int nthreads) {
     ds64_ctx c = {na, pos, mass, eps2, acc};
     parallel_for(na, nthreads, ds64_range, &c);
+}
+
+typedef struct {
+    ds64_ctx base;
+    const uint32_t *targets;
+} ds64t_ctx;
+
+static void ds64t_range(size_t lo, size_t hi, void *p) {
+    ds64t_ctx *c = (ds64t_ctx *)p;
+    for (size_t k = lo; k < hi; ++k) {
+        const size_t i = c->targets[k];
+        const double px = c->base.pos[2 * i], py = c->base.pos[2 * i + 1];
+        double ax = 0.0, ay = 0.0;
+        for (size_t j = 0; j < c->base.na; ++j) {
+            const double dx = (double)c->base.pos[2 * j] - px;
+            const double dy = (double)c->base.pos[2 * j + 1] - py;
+            const double r2 = dx * dx + dy * dy + c->base.eps2;
+            const double r = 1.0 / sqrt(r2);
+            const double s = (double)c->base.mass[j] * r * r * r;
+            ax += dx * s;
+            ay += dy * s;
+        }
+        c->base.acc[2 * k] = ax;
+        c->base.acc[2 * k + 1] = ay;
+    }
+}
+
+void oracle_ds_accel_f64_at(size_t na, const float *pos, const float *mass, double eps2,
+                            const uint32_t *targets, size_t ntargets, double *acc,
+                            int nthreads) {
+    ds64t_ctx c = {{na, pos, mass, eps2, acc}, targets};
+    parallel_for(ntargets, nthreads, ds64t_range, &c);
 }
 
 void oracle_ds_step(size_t na, const float *pos_in, const float *vel_in,

@@ -47,6 +47,11 @@ void oracle_ds_accel_f32(size_t na, const float *pos, const float *mass,
  * yardstick for every FP32 result. */
 void oracle_ds_accel_f64(size_t na, const float *pos, const float *mass,
                          double eps2, double *acc, int nthreads);
+/* FP64 direct sum for the listed targets only: acc[2k..2k+1] is the
+ * acceleration of particle targets[k] (the error yardstick on a sample). */
+void oracle_ds_accel_f64_at(size_t na, const float *pos, const float *mass, double eps2,
+                            const uint32_t *targets, size_t ntargets, double *acc,
+                            int nthreads);
 
 /* One direct-sum step (SimShader.comp:37-74 with the host constants of
  * DirectSimulation.cpp:175-181): v' = v + a*(dt*G); p' = p + ((v+v')*0.5)*dt.

@@ -35,6 +35,8 @@ def lib() -> C.CDLL:
         u64 = C.POINTER(C.c_uint64)
         L.oracle_ds_accel_f32.argtypes = [C.c_size_t, fp, fp, C.c_float, fp, C.c_int]
         L.oracle_ds_accel_f64.argtypes = [C.c_size_t, fp, fp, C.c_double, dp, C.c_int]
+        L.oracle_ds_accel_f64_at.argtypes = [C.c_size_t, fp, fp, C.c_double, up, C.c_size_t,
+                                             dp, C.c_int]
         L.oracle_ds_step.argtypes = [C.c_size_t, fp, fp, fp, fp, fp, C.c_float, C.c_float,
                                      C.c_float, C.c_int]
         L.oracle_ds_step_f64.argtypes = [C.c_size_t, dp, dp, dp, C.c_double, C.c_double,
@@ -104,6 +106,18 @@ def ds_accel_f64(pos, mass, eps2: float, nthreads: int = 0) -> np.ndarray:
     lib().oracle_ds_accel_f64(n, _f(pos), _f(mass), float(eps2),
                               acc.ctypes.data_as(C.POINTER(C.c_double)),
                               nthreads or nthreads_default())
+    return acc
+
+
+def ds_accel_f64_at(pos, mass, eps2: float, targets, nthreads: int = 0) -> np.ndarray:
+    """FP64 direct-sum accelerations of the listed targets only, (k, 2)."""
+    pos, mass = _f32(pos), _f32(mass)
+    t = np.ascontiguousarray(targets, np.uint32)
+    acc = np.zeros((t.shape[0], 2), np.float64)
+    lib().oracle_ds_accel_f64_at(mass.shape[0], _f(pos), _f(mass), float(eps2),
+                                 t.ctypes.data_as(C.POINTER(C.c_uint32)), t.shape[0],
+                                 acc.ctypes.data_as(C.POINTER(C.c_double)),
+                                 nthreads or nthreads_default())
     return acc
 
 

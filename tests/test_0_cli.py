@@ -24,6 +24,31 @@ def test_help_prints_and_exits():
     r = run("--help")
     assert r.returncode == 0
     assert "--simulation-algorithm" in r.stdout and "--particles-out" in r.stdout
+    # every parameter and option of the reference's help (Main.cpp:20-48),
+    # each generate type and algorithm with its own description line
+    for flag in ("--log-file:", "--particles-in:", "--particles-out:", "--particle-count:",
+                 "--generate-type:", "--generate-size:", "--min-mass:", "--max-mass:",
+                 "--gravitational-const:", "--simulation-time:", "--simulation-speed:",
+                 "--softening-len:", "--accuracy-parameter:", "--simulation-algorithm:",
+                 "--simulation-count:", "--help:", "--log-detailed:", "--no-graphics:",
+                 "--benchmark:"):
+        assert "\t" + flag in r.stdout, flag
+    for choice in ("random", "galaxy", "galaxy-collision", "symmetrical-galaxy-collision",
+                   "direct-sum", "barnes-hut"):
+        assert re.search(r"^\t\t" + re.escape(choice) + r": \S", r.stdout, re.M), choice
+
+
+def test_readme_example_arguments_parse():
+    """The README's command line passes every valued argument in the
+    reference's --key=value form (Main.cpp:168-238), which is the only form
+    the CLI parses."""
+    readme = open(os.path.join(ROOT, "README.md")).read()
+    m = re.search(r"gravity-sim_amd/bin/gravity-sim (.*?)\n\n", readme, re.S)
+    assert m, "README example not found"
+    args = m.group(1).replace("\\\n", " ").split()
+    assert all(a.startswith("--") for a in args)
+    valued = [a for a in args if a not in ("--no-graphics", "--benchmark")]
+    assert all("=" in a for a in valued), valued
 
 
 @pytest.mark.parametrize("args,msg", [

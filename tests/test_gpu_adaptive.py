@@ -62,12 +62,13 @@ def test_adaptive_tree_bit_exact(case):
         assert np.array_equal(t[k], ot[k]), k
 
 
+@pytest.mark.parametrize("W", [64, 32])
 @pytest.mark.parametrize("case", [0, 1])
-def test_adaptive_traversal_counts_and_forces(case):
+def test_adaptive_traversal_counts_and_forces(case, W):
     name, pos, mass = list(_inputs())[case]
-    _say(f"adaptive traversal {name}")
+    _say(f"adaptive traversal {name} W={W}")
     p = gsim.Particles(pos, np.zeros_like(pos), mass)
-    s = gsim.Simulation(p, "barnes-hut", accuracy_parameter=0.7)
+    s = gsim.Simulation(p, "barnes-hut", accuracy_parameter=0.7, group_width=W)
     s.set_tuning(gsim.TUNE_BH_ADAPTIVE_BOX, 1)
     s.set_debug(gsim.DEBUG_ACCEL)
     s.step(1)
@@ -78,7 +79,7 @@ def test_adaptive_traversal_counts_and_forces(case):
     pp = p.padded()
     box = O.bh_box(pp.pos)
     _, _, _, pk, mk = O.bh_build(pp.pos, pp.mass, box=box)
-    oa, oV, oA, _, _, oa64 = O.bh_traverse(t, pk, mk, EPS2, H.sq32(0.7), W=64, f64=True)
+    oa, oV, oA, _, _, oa64 = O.bh_traverse(t, pk, mk, EPS2, H.sq32(0.7), W=W, f64=True)
     assert (V, A) == (oV, oA)
     inside = t["sorted_src"][t["sorted_src"] < len(mk)]
     assert H.system_rel_err(a[inside], oa[inside]) <= 1e-5

@@ -168,12 +168,21 @@ def test_batch_step_budgets_freeze_exactly():
 def test_batch_accepts_match_single_path_counters():
     p = gsim.generate("galaxy", 20000, 100.0, 1.0, 10.0, seed=21)
     with gsim.Simulation(p, "barnes-hut", accuracy_parameter=0.7) as s:
-        want = 0
+        want_v = want_a = 0
         for _ in range(3):
             s.step(1)
-            want += int(s.dump(gsim.DUMP_COUNTERS)[1])
+            v, a = (int(x) for x in s.dump(gsim.DUMP_COUNTERS))
+            want_v += v
+            want_a += a
     with gsim.Batch(32768, max_systems=4) as b:
         sid = b.add(p, "barnes-hut", accuracy_parameter=0.7)
         b.set_steps(sid, 3)
         b.step(5)  # the two steps past the budget add nothing
-        assert b.accepts() == want
+        assert b.accepts() == want_a
+        assert b.visits() == want_v
+    # autotuner trials run on a snapshot: they leave both totals untouched
+    with gsim.Batch(32768, max_systems=4) as b:
+        sid = b.add(p, "barnes-hut", accuracy_parameter=0.7)
+        b.step(3)
+        b.autotune(steps_per_trial=2)
+        assert (b.visits(), b.accepts()) == (want_v, want_a)

@@ -36,8 +36,8 @@ def _inputs():
     yield "tiny-3000", p.pos, p.mass
 
 
-def _sim(p, boxed, theta=1.0, bits=15):
-    s = gsim.Simulation(p, "barnes-hut", accuracy_parameter=theta)
+def _sim(p, boxed, theta=1.0, bits=15, W=64):
+    s = gsim.Simulation(p, "barnes-hut", accuracy_parameter=theta, group_width=W)
     s.set_tuning(gsim.TUNE_BH_LEAF_BITS, bits)
     if boxed:
         s.set_tuning(gsim.TUNE_BH_ADAPTIVE_BOX, 1)
@@ -67,13 +67,14 @@ def test_leaf15_tree_bit_exact(case, boxed):
         assert np.array_equal(t[k], ot[k]), k
 
 
+@pytest.mark.parametrize("W", [64, 32])
 @pytest.mark.parametrize("boxed", [False, True])
 @pytest.mark.parametrize("case", [0, 1])
-def test_leaf15_traversal_counts_and_forces(case, boxed):
+def test_leaf15_traversal_counts_and_forces(case, boxed, W):
     name, pos, mass = list(_inputs())[case]
-    _say(f"15-bit traversal {name} box={boxed}")
+    _say(f"15-bit traversal {name} box={boxed} W={W}")
     p = gsim.Particles(pos, np.zeros_like(pos), mass)
-    s = _sim(p, boxed, theta=0.7)
+    s = _sim(p, boxed, theta=0.7, W=W)
     s.set_debug(gsim.DEBUG_ACCEL)
     s.step(1)
     a = s.dump(gsim.DUMP_ACCEL)
@@ -83,7 +84,7 @@ def test_leaf15_traversal_counts_and_forces(case, boxed):
     s.close()
     pp = p.padded()
     _, _, _, pk, mk = O.bh_build(pp.pos, pp.mass, box=box, bits=15)
-    oa, oV, oA, _, _, oa64 = O.bh_traverse(t, pk, mk, EPS2, H.sq32(0.7), W=64, f64=True)
+    oa, oV, oA, _, _, oa64 = O.bh_traverse(t, pk, mk, EPS2, H.sq32(0.7), W=W, f64=True)
     assert (V, A) == (oV, oA)
     inside = t["sorted_src"][t["sorted_src"] < len(mk)]
     assert H.system_rel_err(a[inside], oa[inside]) <= 1e-5

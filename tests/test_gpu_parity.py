@@ -511,3 +511,20 @@ def test_timing_and_counters_api():
     assert t["steps"] == 3 and t["force_ms"] > 0 and t["build_ms"] > 0
     assert abs(t["step_ms"] - t["force_ms"] - t["build_ms"]) < 1e-2 + 0.05 * t["step_ms"]
     s.close()
+
+
+def test_w32_handle_refuses_w64_launch_knobs():
+    """A group width 32 handle has one traversal launch shape: the W = 64
+    launch knobs are refused instead of silently ignored."""
+    p = gsim.generate("random", 2000, 60.0, 1.0, 10.0, seed=209)
+    knobs = (gsim.TUNE_BH_WAVES_PER_SIMD, gsim.TUNE_BH_TRAVERSAL, gsim.TUNE_BH_SPLIT,
+             gsim.TUNE_BH_SEGMENTS)
+    with gsim.Simulation(p, "barnes-hut", group_width=32) as s:
+        for knob in knobs:
+            with pytest.raises(gsim.GsimError) as e:
+                s.set_tuning(knob, 2)
+            assert e.value.status == 1
+        s.set_tuning(gsim.TUNE_BH_ADAPTIVE_BOX, 0)  # not a launch knob
+    with gsim.Simulation(p, "barnes-hut", group_width=64) as s:
+        for knob in knobs:
+            s.set_tuning(knob, 2)

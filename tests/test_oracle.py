@@ -118,6 +118,16 @@ def test_ds_newton_third_law_and_f64_agreement():
     assert H.system_rel_err(a32, a64) < 2e-6
 
 
+def test_ds_accel_f64_at_equals_full_rows():
+    rng = np.random.default_rng(15)
+    pos = rng.uniform(-30, 30, size=(700, 2)).astype(np.float32)
+    mass = rng.uniform(1, 10, size=700).astype(np.float32)
+    full = O.ds_accel_f64(pos, mass, 0.04)
+    targets = np.array([699, 0, 5, 5, 321, 64], np.uint32)
+    at = O.ds_accel_f64_at(pos, mass, 0.04, targets, nthreads=3)
+    assert np.array_equal(at, full[targets])
+
+
 def test_ds_step_free_particle_and_thread_invariance():
     pos = np.array([[1.0, 2.0]], np.float32)
     vel = np.array([[3.0, -4.0]], np.float32)
