@@ -421,6 +421,11 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
             if (value > 1000) return fail(GSIM_ERROR_INVALID_ARGUMENT, "split threshold must be in [0, 1000]");
             s->trav_split = value;
             return GSIM_OK;
+        case GSIM_TUNE_BH_XCD_ORDER:
+            if (value > 4096)
+                return fail(GSIM_ERROR_INVALID_ARGUMENT, "XCD run length must be in [0, 4096]");
+            s->xcd_order = value;
+            return GSIM_OK;
         case GSIM_TUNE_BH_SEGMENTS:
             if (value < 1 || value > 65536)
                 return fail(GSIM_ERROR_INVALID_ARGUMENT, "segments must be in [1, 65536]");

@@ -50,6 +50,26 @@ __device__ __forceinline__ uint32_t cell30(float x, float y) {
     return 0xffffffffu;
 }
 
+// Workgroups of a launch are dispatched to the 8 XCDs round-robin (workgroup
+// b to XCD b mod 8), and each XCD has its own L2.  xcd_block renumbers them so
+// that XCD x runs runs of c consecutive logical workgroups (logical blocks
+// [8ck + cx, 8ck + cx + c) for k = 0, 1, ...), spatially adjacent groups,
+// which walk nearly the same cells and leaf runs, then sharing an L2; the
+// last nb mod 8c workgroups keep their number.  From 8c >= nb on, each XCD
+// takes one contiguous range (q or q + 1 workgroups for nb = 8q + r).
+// c <= 1 is the plain round-robin order.  A bijection on [0, nb).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb, uint32_t c) {
+    if (c <= 1u) return b;
+    const uint32_t x = b & 7u, i = b >> 3;
+    if (8u * c >= nb) {
+        const uint32_t q = nb >> 3, r = nb & 7u;
+        return x * q + min(x, r) + i;
+    }
+    const uint32_t span = 8u * c, whole = nb / span * span;
+    if (b >= whole) return b;
+    return (i / c) * span + x * c + i % c;
+}
+
 // Radius base of the fixed domain: S*S*2 (TreeMoveShader.comp:70-71).
 constexpr float kFixedR0 = kSimSize * kSimSize * 2.0f;
 
@@ -292,8 +312,21 @@ __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__r
 // private memory).
 typedef float f4v __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ f4v load_node(const float4 *__restrict__ node4, uint32_t i, bool ok) {
-    return ok ? reinterpret_cast<const f4v *>(node4)[i] : f4v{0.0f, 0.0f, 0.0f, 0.0f};
+// The (x, y, mass) words of node i, or zeros: a run sum uses no more, and a
+// 16-byte load whose fourth word is dead lets the compiler reuse that
+// register while the load is still in flight, which forces a wait for the
+// load at the reuse (a prefetch issued before the previous chunk's sum would
+// then stall at that sum's first instruction).
+typedef float f3v __attribute__((ext_vector_type(3)));
+__device__ __forceinline__ f3v load_node3(const float4 *__restrict__ node4, uint32_t i, bool ok) {
+    f3v r = {0.0f, 0.0f, 0.0f};
+    if (ok) {
+        const float *p = reinterpret_cast<const float *>(node4 + i);
+        r.x = p[0];
+        r.y = p[1];
+        r.z = p[2];
+    }
+    return r;
 }
 
 struct Partials {
@@ -339,7 +372,8 @@ constexpr uint32_t kRunQueue = 64;
 // The run chunks [queue[0], queue[qn]) of one queue batch, chunk numbers from
 // ch0: each chunk this wave owns ((parts >> (chunk & 3)) & 1) is loaded with
 // one coalesced load while the previous one is summed (across run
-// boundaries), staged in the wave's LDS as packed 12-byte (x, y, m) records,
+// boundaries; a chunk is staged in LDS before the next one's load is issued),
+// staged in the wave's LDS as packed 12-byte (x, y, m) records,
 // summed node by node from zero and added to its partial.  kHalf: the wave
 // holds one 32-particle group twice (lane l and l + 32 the same particle) and
 // each half sums its 32 nodes of every chunk (nodes 0-31: lanes 0-31).
@@ -370,17 +404,20 @@ __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, u
     if (qn == 0) return;
     run_at(0, base, rend);
     while (e < qn && !owned(ch)) step(e, base, rend, ch);
-    f4v cur = load_node(node4, base + lane, e < qn && lane < min(64u, rend - base));
+    f3v cur = load_node3(node4, base + lane, e < qn && lane < min(64u, rend - base));
     while (e < qn) {
         const uint32_t cnt = min(64u, rend - base), k = ch & 3u;
+        // stage this chunk first: its load (the previous iteration's
+        // prefetch) is the only one in flight, so the wait for it cannot
+        // also wait for the next chunk's load, issued right after
+        st[3 * lane] = cur.x;
+        st[3 * lane + 1] = cur.y;
+        st[3 * lane + 2] = cur.z;
         uint32_t ne = e, nb = base, nr = rend, nch = ch;
         do {
             step(ne, nb, nr, nch);
         } while (ne < qn && !owned(nch));
-        const f4v next = load_node(node4, nb + lane, ne < qn && lane < min(64u, nr - nb));
-        st[3 * lane] = cur.x;
-        st[3 * lane + 1] = cur.y;
-        st[3 * lane + 2] = cur.z;
+        const f3v next = load_node3(node4, nb + lane, ne < qn && lane < min(64u, nr - nb));
         __builtin_amdgcn_wave_barrier();
         float cx = 0.0f, cy = 0.0f;
         uint32_t j = 0, jend = cnt;
@@ -471,6 +508,11 @@ __device__ __forceinline__ uint32_t walk_partials(
       while (ind < end && qn < kRunQueue) {
         const float4 nd = node4[ind];
         const uint32_t cnt = ncount[ind];
+        // one round trip per step: both loads are issued here and waited for
+        // together (otherwise the compiler sinks the x, y half of the record
+        // and the count into the branches that use them, three dependent
+        // round trips for an accepted cell)
+        asm volatile("" ::"s"(nd.x), "s"(nd.y), "s"(nd.z), "s"(nd.w), "s"(cnt));
         if (nd.w <= 0.0f) {  // a particle node: w = -(particle nodes left in its run)
             const uint32_t pr = (uint32_t)(-nd.w);
             const uint32_t n = __builtin_amdgcn_readfirstlane(min(max(pr, 1u), end - ind));

@@ -829,6 +829,8 @@ __global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
             while (ind < end && qn < kRunQueue) {
                 const float4 nd = node4[ind];
                 const uint32_t cnt = ncount[ind];
+                // one round trip per step, as in walk_partials
+                asm volatile("" ::"s"(nd.x), "s"(nd.y), "s"(nd.z), "s"(nd.w), "s"(cnt));
                 if (nd.w <= 0.0f) {  // a particle node: w = -(particle nodes left in its run)
                     const uint32_t pr = (uint32_t)(-nd.w);
                     const uint32_t n = __builtin_amdgcn_readfirstlane(min(max(pr, 1u), end - ind));
@@ -1073,7 +1075,7 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, unsigned long long *__restrict__ gvis,
     unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk,
-    const uint8_t *__restrict__ heavy) {
+    const uint8_t *__restrict__ heavy, uint32_t xcd) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
     __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1094,7 +1096,8 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
                                              theta2, kv, kp, gvis, gacc, nullptr, gchunk);
         }
     } else {
-        const uint32_t g = blockIdx.x * 4 + w;
+        const uint32_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+        const uint32_t g = b * 4 + w;
         if (g >= groups) return;  // whole wave beyond the last group
         walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
                                          pos_out, vel_out, acc_dbg, na, eps2, theta2, kv,
@@ -1115,11 +1118,12 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     const uint32_t *__restrict__ ncount, BhCounters *__restrict__ ctr,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
     uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp,
-    unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc) {
+    unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc, uint32_t xcd) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t g = blockIdx.x * 4 + w;
+    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+    const uint32_t g = b * 4 + w;
     if (g >= na / 32) return;  // whole wave beyond the last group
     // node count is bounded by the node arrays; every step advances the walk
     const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
@@ -1405,7 +1409,8 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #define GSIM_TRAV(D, S, P)                                                                 \
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
-        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk, heavy)
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk, heavy, \
+        s->xcd_order)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
             else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }
@@ -1424,11 +1429,11 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         if (dbg)
             bh_traverse32_kernel<true><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc);
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order);
         else
             bh_traverse32_kernel<false><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc);
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order);
     }
     return hipGetLastError();
 }

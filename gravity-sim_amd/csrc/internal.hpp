@@ -175,6 +175,9 @@ struct gsim_sim {
     uint32_t trav_split = 2;           // static: heavy list above this x mean chunk count;
                                        // balanced: split above this x mean cost (0: never)
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
+    uint32_t xcd_order = 1;            // static traversals: each XCD takes runs of this
+                                       // many consecutive workgroups (xcd_block; <= 1:
+                                       // round-robin)
     uint32_t ds_rcp = 0;               // direct sum: 1/r2 from the transcendental unit
     // state
     float2 *pos[2] = {}, *vel[2] = {};

@@ -50,6 +50,7 @@ TUNE_DS_RCP, TUNE_BH_WAVES_PER_SIMD, TUNE_BH_TRAVERSAL = 1, 2, 3
 TUNE_DS_SYMMETRIC, TUNE_DS_SYM_COLS, TUNE_BH_SEGMENTS, TUNE_BH_SPLIT = 4, 5, 6, 7
 TUNE_BH_ADAPTIVE_BOX = 8
 TUNE_BH_LEAF_BITS = 9
+TUNE_BH_XCD_ORDER = 10
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
           4: "NO_DEVICE", 5: "IO", 6: "CAPACITY"}

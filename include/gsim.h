@@ -190,7 +190,7 @@ typedef enum gsim_tuning {
                                          only non-finite positions are killed; cell
                                          radius (h*h)*2 / 4^d.  Changes the topology,
                                          so results differ from the reference's.        */
-    GSIM_TUNE_BH_LEAF_BITS = 9        /* 10 (the reference's 1024^2 leaf grid, the
+    GSIM_TUNE_BH_LEAF_BITS = 9,       /* 10 (the reference's 1024^2 leaf grid, the
                                          default) or 15, beyond the reference: leaves
                                          on a 32768^2 grid at depth 15, so a dense knot
                                          is no longer one bucket that every group
@@ -200,6 +200,15 @@ typedef enum gsim_tuning {
                                          the reference's.  Combines with the adaptive
                                          box.  Reallocates the node arrays (synchronous;
                                          they grow by up to 2.5 x Na entries).          */
+    GSIM_TUNE_BH_XCD_ORDER = 10       /* 0..4096: the static traversals hand each of
+                                         the 8 XCDs (each with its own L2) runs of this
+                                         many consecutive workgroups instead of every
+                                         eighth one, so spatially adjacent groups share
+                                         an L2 (one contiguous range per XCD once the
+                                         runs reach an eighth of the launch); 0 or 1 =
+                                         round-robin, the default: measured at 500k,
+                                         runs of 2-32 change nothing and contiguous
+                                         ranges are 16% slower.  W = 32 and 64.         */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 

@@ -528,3 +528,21 @@ def test_w32_handle_refuses_w64_launch_knobs():
     with gsim.Simulation(p, "barnes-hut", group_width=64) as s:
         for knob in knobs:
             s.set_tuning(knob, 2)
+
+
+@pytest.mark.parametrize("W", [32, 64])
+def test_bh_xcd_group_order_is_bit_identical(W):
+    """Handing each XCD runs of consecutive workgroups changes only where a
+    group runs: states and visit/accept counts are bit-identical over steps,
+    for run lengths that do and do not divide the launch."""
+    p = gsim.generate("galaxy-collision", 40000, 120.0, 1.0, 10.0, seed=210)
+    out = []
+    for order in (1, 3, 16, 4096):
+        with gsim.Simulation(p, "barnes-hut", group_width=W, accuracy_parameter=0.8) as s:
+            s.set_tuning(gsim.TUNE_BH_XCD_ORDER, order)
+            s.step(3)
+            out.append((s.read(), tuple(int(x) for x in s.dump(gsim.DUMP_COUNTERS))))
+    a, ca = out[0]
+    for b, cb in out[1:]:
+        assert ca == cb
+        assert np.array_equal(a.pos, b.pos) and np.array_equal(a.vel, b.vel)
