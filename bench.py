@@ -225,7 +225,10 @@ def bench_batched(steps: int, serial_steps: int = 20) -> dict:
     batch = gsim.Batch(cap, max_systems=len(systems))
     for p, algo, kw in systems:
         batch.add(p, algo, **kw)
-    waves = batch.autotune(steps_per_trial=10)
+    t0 = time.perf_counter()
+    batch.autotune(steps_per_trial=5)
+    tune_s = time.perf_counter() - t0
+    picks = batch.tuning()
     batch.step(2)
     batch.synchronize()
     v0, a0 = batch.visits(), batch.accepts()
@@ -278,7 +281,11 @@ def bench_batched(steps: int, serial_steps: int = 20) -> dict:
                                f"exponential disk (radius 100), N < 8192 direct sum ({n_ds}), "
                                f"else Barnes-Hut theta=0.7 ({len(systems) - n_ds}); one hipGraph "
                                f"replay per step", "steps": steps,
-                   "one_by_one_steps": serial_steps, "waves_per_simd": waves},
+                   "one_by_one_steps": serial_steps,
+                   "autotuned": dict(picks, seconds=tune_s,
+                                     method="coordinate descent over waves per SIMD, sort "
+                                            "tile, run queue, priority step and carry; each "
+                                            "candidate 2 x 5 steps from a restored snapshot")},
     }
 
 

@@ -228,7 +228,7 @@ gsim_status gsim_batch_create(int32_t device, size_t capacity_particles, uint32_
     b->node_capacity = 3 * cap + 64;
     b->partial_capacity = 4 * cap;
     b->item_capacity = cap / 8 + cap / 64 + 16 * (size_t)max_systems + 64;
-    b->tiles_cap = (cap + gsim::kBatchSortTile - 1) / gsim::kBatchSortTile;
+    b->tiles_cap = (cap + gsim::kBatchSortTileMin - 1) / gsim::kBatchSortTileMin;
     b->blocks_cap = (cap + 255) / 256;
     b->free_ranges[0] = (uint32_t)cap;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
@@ -483,7 +483,7 @@ gsim_status gsim_batch_query(const gsim_batch *b, gsim_batch_info *out) {
     out->node_capacity = b->node_capacity;
     out->nodes_reserved = b->nodes_reserved;
     out->steps = b->steps;
-    out->waves_per_simd = b->waves_per_simd;
+    out->waves_per_simd = b->tune.waves_per_simd;
     return GSIM_OK;
 }
 
@@ -495,12 +495,45 @@ size_t gsim_batch_bytes_per_particle(void) {
 gsim_status gsim_batch_set_waves(gsim_batch *b, uint32_t waves_per_simd) {
     gsim::clear_error();
     if (!b) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_set_waves: NULL handle");
-    if (waves_per_simd < 1 || waves_per_simd > 8)
+    gsim_batch_tuning t;
+    (void)gsim_batch_get_tuning(b, &t);
+    t.waves_per_simd = waves_per_simd;
+    return gsim_batch_set_tuning(b, &t);
+}
+
+gsim_status gsim_batch_get_tuning(const gsim_batch *b, gsim_batch_tuning *out) {
+    gsim::clear_error();
+    if (!b || !out) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_get_tuning: NULL argument");
+    out->waves_per_simd = b->tune.waves_per_simd;
+    out->sort_tile = b->tune.sort_rounds * (uint32_t)gsim::kSortThreads;
+    out->run_queue = b->tune.run_queue;
+    out->prio_step = b->tune.prio_step;
+    out->prio_carry = b->tune.prio_carry;
+    return GSIM_OK;
+}
+
+gsim_status gsim_batch_set_tuning(gsim_batch *b, const gsim_batch_tuning *t) {
+    gsim::clear_error();
+    if (!b || !t) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_set_tuning: NULL argument");
+    if (t->waves_per_simd < 1 || t->waves_per_simd > 8)
         return fail(GSIM_ERROR_INVALID_ARGUMENT, "waves per SIMD must be in [1, 8]");
-    if (waves_per_simd != b->waves_per_simd) {
+    if (t->sort_tile != 2048 && t->sort_tile != 3072 && t->sort_tile != 4096)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "sort tile must be 2048, 3072 or 4096 keys");
+    if (t->run_queue < 1 || t->run_queue > gsim::kRunQueue)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "run queue must be in [1, %u]", gsim::kRunQueue);
+    if (t->prio_step > 4096 || t->prio_carry > 4096)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "priority step and carry must be in [0, 4096]");
+    gsim::BatchTune n;
+    n.waves_per_simd = t->waves_per_simd;
+    n.sort_rounds = t->sort_tile / (uint32_t)gsim::kSortThreads;
+    n.run_queue = t->run_queue;
+    n.prio_step = t->prio_step;
+    n.prio_carry = t->prio_carry;
+    if (std::memcmp(&n, &b->tune, sizeof(n)) != 0) {
+        GSIM_HIP(hipSetDevice(b->device));
         GSIM_HIP(hipStreamSynchronize(b->stream));
-        destroy_graphs(b);
-        b->waves_per_simd = waves_per_simd;
+        destroy_graphs(b);  // the launch parameters are baked into the graphs
+        b->tune = n;
     }
     return GSIM_OK;
 }
@@ -508,7 +541,7 @@ gsim_status gsim_batch_set_waves(gsim_batch *b, uint32_t waves_per_simd) {
 gsim_status gsim_batch_autotune(gsim_batch *b, uint32_t steps_per_trial, uint32_t *chosen) {
     gsim::clear_error();
     if (!b) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_batch_autotune: NULL handle");
-    if (steps_per_trial == 0) steps_per_trial = 3;
+    if (steps_per_trial == 0) steps_per_trial = 5;
     GSIM_HIP(hipSetDevice(b->device));
     if (b->dirty) {
         gsim_status st = upload_membership(b);
@@ -549,35 +582,67 @@ gsim_status gsim_batch_autotune(gsim_batch *b, uint32_t steps_per_trial, uint32_
     if (e == hipSuccess) e = hipMemcpy(snap_pos, b->d.pos[cur0], sizeof(float2) * cap, hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemcpy(snap_vel, b->d.vel[cur0], sizeof(float2) * cap, hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemcpy(snap_mass, b->d.mass, sizeof(float) * cap, hipMemcpyDeviceToDevice);
-    uint32_t best = b->waves_per_simd;
-    double best_ms = 1e300;
     gsim_status st = GSIM_OK;
-    const uint32_t candidates[] = {2, 4, 6, 8};
-    for (uint32_t w : candidates) {
-        if (e != hipSuccess || st != GSIM_OK) break;
-        st = gsim_batch_set_waves(b, w);
-        if (st != GSIM_OK) break;
-        st = launch_steps(b, 1);  // capture + warm
-        if (st != GSIM_OK) break;
-        e = hipStreamSynchronize(b->stream);
-        const auto t0 = std::chrono::steady_clock::now();
-        st = launch_steps(b, steps_per_trial);
-        if (st != GSIM_OK) break;
-        e = hipStreamSynchronize(b->stream);
-        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = w;
+    // One trial: the candidate's graphs are captured and warmed by one step,
+    // then `steps_per_trial` steps are timed, twice, each from the snapshot;
+    // the faster of the two counts.
+    auto trial = [&](const gsim_batch_tuning &t, double *ms_out) -> bool {
+        *ms_out = 1e300;
+        if (e != hipSuccess || st != GSIM_OK) return false;
+        st = gsim_batch_set_tuning(b, &t);
+        for (int rep = 0; rep < 2 && st == GSIM_OK && e == hipSuccess; ++rep) {
+            st = launch_steps(b, 1);
+            if (st != GSIM_OK) break;
+            e = hipStreamSynchronize(b->stream);
+            const auto t0 = std::chrono::steady_clock::now();
+            st = launch_steps(b, steps_per_trial);
+            if (st != GSIM_OK) break;
+            e = hipStreamSynchronize(b->stream);
+            const double ms =
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            *ms_out = std::min(*ms_out, ms);
+            if (e == hipSuccess) e = restore();
         }
-        if (e == hipSuccess) e = restore();
+        return e == hipSuccess && st == GSIM_OK;
+    };
+    // Coordinate descent over the knobs, each over its candidates from the
+    // current best point; a candidate must beat it by 0.5% to replace it.
+    gsim_batch_tuning best;
+    (void)gsim_batch_get_tuning(b, &best);
+    double best_ms = 1e300;
+    bool ok = trial(best, &best_ms);
+    struct Knob {
+        uint32_t gsim_batch_tuning::*field;
+        std::vector<uint32_t> values;
+    };
+    const Knob knobs[] = {
+        {&gsim_batch_tuning::waves_per_simd, {2, 4, 6, 8}},
+        {&gsim_batch_tuning::sort_tile, {2048, 3072, 4096}},
+        {&gsim_batch_tuning::run_queue, {16, 32, 64}},
+        {&gsim_batch_tuning::prio_step, {0, 16, 32, 64}},
+        {&gsim_batch_tuning::prio_carry, {0, 8, 16, 32}},
+    };
+    for (const Knob &k : knobs) {
+        for (uint32_t v : k.values) {
+            if (!ok) break;
+            if (best.*(k.field) == v) continue;
+            gsim_batch_tuning cand = best;
+            cand.*(k.field) = v;
+            double ms = 1e300;
+            ok = trial(cand, &ms);
+            if (ok && ms < best_ms * 0.995) {
+                best = cand;
+                best_ms = ms;
+            }
+        }
     }
-    if (e == hipSuccess && st == GSIM_OK) st = gsim_batch_set_waves(b, best);
+    if (e == hipSuccess && st == GSIM_OK) st = gsim_batch_set_tuning(b, &best);
     (void)hipFree(snap_pos);
     (void)hipFree(snap_vel);
     (void)hipFree(snap_mass);
     if (st != GSIM_OK) return st;
     if (e != hipSuccess) return hip_fail(e, "autotune");
-    if (chosen) *chosen = best;
+    if (chosen) *chosen = best.waves_per_simd;
     return GSIM_OK;
 }
 

@@ -58,10 +58,23 @@ static_assert(offsetof(BatchCounts, visits) == offsetof(BatchCounts, accepts) + 
 
 constexpr uint32_t kUnlimitedSteps = 0xffffffffu;
 
-// Radix tiles of the batched sort: kBatchSortRounds keys per thread (3,072 a
-// tile; 2,048 measured 1.7% slower over the mixed 256-system batch).
-constexpr int kBatchSortRounds = 12;
-constexpr int kBatchSortTile = kBatchSortRounds * kSortThreads;
+// Radix tiles of the batched sort: sort_rounds keys per thread (8, 12 or 16:
+// tiles of 2,048, 3,072 or 4,096 keys; 12 by default, the autotuner picks).
+// The look-back status words are sized for the smallest tile.
+constexpr int kBatchSortRoundsDefault = 12;
+constexpr int kBatchSortTileMin = 8 * kSortThreads;
+
+// Launch parameters of a batched step that never change results (the
+// autotuner's knobs): persistent force-launch width, radix tile size, leaf
+// runs a wave queues before summing them (its LDS run stack), and the walk's
+// issue-priority step and carry divisor (0: off).
+struct BatchTune {
+    uint32_t waves_per_simd = 8;
+    uint32_t sort_rounds = kBatchSortRoundsDefault;
+    uint32_t run_queue = 64;   // <= kRunQueue
+    uint32_t prio_step = 32;   // summed chunks per priority level
+    uint32_t prio_carry = 16;  // last step's chunks / this = the starting level
+};
 
 constexpr uint32_t kBatchRankShift = 21;                   // key = rank << 21 | cell (bit 20 = killed)
 constexpr uint32_t kBatchMaxSystems = 1u << (32 - kBatchRankShift);  // 2048
@@ -134,7 +147,7 @@ struct gsim_batch {
     bool dirty = true;
     int cur = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
-    uint32_t waves_per_simd = 8;
+    gsim::BatchTune tune;
     uint64_t steps = 0;
     // host mirror of the membership arrays
     std::vector<gsim::SysDesc> h_desc;

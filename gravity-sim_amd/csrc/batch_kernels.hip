@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t *__restrict__ keys,
     uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
     RankStats *__restrict__ stats, uint32_t *__restrict__ digit_total,
-    uint32_t *__restrict__ status) {
+    uint32_t *__restrict__ status, uint32_t tile_keys) {
     const uint32_t nb = counts->n_bh, n = counts->n_bh_particles;
     const uint32_t tid = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
     for (uint32_t t = tid; t < nb * 2 * kTiles; t += stride) tile_range[t] = 0;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
     for (uint32_t t = tid; t < 4 * 256; t += stride) digit_total[t] = 0;
     for (uint32_t t = tid; t < kLiveLists; t += stride) counts->n_live[t] = 0;
     // the first radix pass's look-back words (each pass clears the next one's)
-    const uint32_t status_words = (n + kBatchSortTile - 1) / kBatchSortTile * 256;
+    const uint32_t status_words = (n + tile_keys - 1) / tile_keys * 256;
     for (uint32_t t = tid; t < status_words; t += stride) status[t] = 0;
     if (tid == 0) {
         counts->n_cells = 0;
@@ -99,24 +99,26 @@ __global__ __launch_bounds__(256) void bb_keys_kernel(
 // digit_total[pass * 256 + d] += keys of digit d in pass `pass`; the per-wave
 // aggregation matters here: a wave's keys usually share one system, so the
 // upper digits repeat across the wave.
+template <int kR>
 __global__ __launch_bounds__(kSortThreads) void bb_hist_kernel(
     const uint32_t *__restrict__ keys, const BatchCounts *__restrict__ counts,
     uint32_t *__restrict__ digit_total) {
+    constexpr uint32_t kTile = kR * kSortThreads;
     __shared__ uint32_t h[4][256];
     const uint32_t t = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < 4; ++p) h[p][t] = 0;
     __syncthreads();
     const uint32_t n = counts->n_bh_particles;
-    for (uint32_t base = blockIdx.x * kBatchSortTile; base < n; base += gridDim.x * kBatchSortTile) {
-        uint32_t k[kBatchSortRounds];
+    for (uint32_t base = blockIdx.x * kTile; base < n; base += gridDim.x * kTile) {
+        uint32_t k[kR];
 #pragma unroll
-        for (int r = 0; r < kBatchSortRounds; ++r) {
+        for (int r = 0; r < kR; ++r) {
             const uint32_t i = base + r * kSortThreads + t;
             k[r] = i < n ? keys[i] : 0u;
         }
 #pragma unroll
-        for (int r = 0; r < kBatchSortRounds; ++r)
+        for (int r = 0; r < kR; ++r)
             hist_add<4, true>(h, k[r], base + r * kSortThreads + t < n);
     }
     __syncthreads();
@@ -131,14 +133,16 @@ __global__ __launch_bounds__(kSortThreads) void bb_hist_kernel(
 // counts->sort_tile[pass] until the live tiles run out.  status: this pass's
 // [tiles][256] look-back words, zeroed; status_next: the next pass's, zeroed
 // here tile by tile.
+template <int kR>
 __global__ __launch_bounds__(kSortThreads) void bb_onesweep_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out,
     BatchCounts *__restrict__ counts, uint32_t pass, const uint32_t *__restrict__ digit_total,
     uint32_t *__restrict__ status, uint32_t *__restrict__ status_next) {
-    __shared__ SortShared<true, kBatchSortRounds> sh;
+    __shared__ SortShared<true, kR> sh;
+    constexpr uint32_t kTile = kR * kSortThreads;
     const uint32_t n = counts->n_bh_particles;
-    const uint32_t ntiles = (n + kBatchSortTile - 1) / kBatchSortTile;
+    const uint32_t ntiles = (n + kTile - 1) / kTile;
     const uint32_t gbase = digit_base(digit_total + 256 * pass, sh);  // synchronises
     for (;;) {
         if (threadIdx.x == 0) sh.tile = atomicAdd(&counts->sort_tile[pass], 1u);
@@ -468,7 +472,7 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
                               const uint32_t *__restrict__ ncount, float2 *__restrict__ pos_out,
                               float2 *__restrict__ vel_out, uint32_t node_cap,
                               unsigned long long &vis, unsigned long long &acc,
-                              uint32_t *__restrict__ chunks_io) {
+                              uint32_t *__restrict__ chunks_io, const BatchTune &tune) {
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t src = sorted_src[d.sorted_base + g * 64 + lane];
     const bool real = src != kBatchSentinel;
@@ -483,10 +487,11 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     const uint32_t root = __builtin_amdgcn_readfirstlane(st.root);
     Partials P;
     // start at the issue priority this work item reached in the last step
-    const uint32_t prio0 = *chunks_io / kPrioCarryBatch;
+    const uint32_t prio0 = tune.prio_carry ? *chunks_io / tune.prio_carry : 0u;
     uint32_t chunks = 0;
     (void)walk_partials<true>(root, end, stage, queue, node4, ncount, px, py, real, nreal, d.eps2,
-                              d.theta2, 0xfu, P, vis, acc, kPrioStepBatch, prio0, &chunks);
+                              d.theta2, 0xfu, P, vis, acc, tune.prio_step, prio0, &chunks,
+                              tune.run_queue);
     if (lane == 0) *chunks_io = chunks;
     if (real) {
         float ax, ay;
@@ -547,7 +552,8 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     const float2 *__restrict__ vel, const float *__restrict__ mass,
     const float4 *__restrict__ node4, const uint32_t *__restrict__ ncount,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ partial,
-    uint32_t node_cap, unsigned long long *__restrict__ wave_acc, uint32_t *__restrict__ ichunk) {
+    uint32_t node_cap, unsigned long long *__restrict__ wave_acc, uint32_t *__restrict__ ichunk,
+    BatchTune tune) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t w = threadIdx.x >> 6;
@@ -565,7 +571,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             // kernel has already written the system's output)
             if (!live || st.n_in == 0) continue;
             bb_walk_group(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel, node4, ncount,
-                          pos_out, vel_out, node_cap, vis, acc, &ichunk[it]);
+                          pos_out, vel_out, node_cap, vis, acc, &ichunk[it], tune);
         } else if (live) {
             bb_ds_tile(item.y, d, pos, mass, partial);
         }
@@ -653,22 +659,34 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
     hipStream_t st = b->stream;
     const uint32_t g_part = (uint32_t)std::min<size_t>((b->capacity + 255) / 256, 4096);
     const uint32_t live_per_list = (uint32_t)(((size_t)b->max_systems * kTiles + kLiveLists - 1) / kLiveLists);
-    const uint32_t g_tiles = (uint32_t)std::min<size_t>(b->tiles_cap, 2048);
-    const uint32_t g_hist = (uint32_t)std::min<size_t>(b->tiles_cap, 1024);
+    const BatchTune &tune = b->tune;
+    const uint32_t rounds = tune.sort_rounds;
+    if (rounds != 8 && rounds != 12 && rounds != 16) return hipErrorInvalidValue;
+    const uint32_t tile_keys = rounds * kSortThreads;
+    const size_t tiles = (b->capacity + tile_keys - 1) / tile_keys;  // <= tiles_cap
+    const uint32_t g_tiles = (uint32_t)std::min<size_t>(tiles, 2048);
+    const uint32_t g_hist = (uint32_t)std::min<size_t>(tiles, 1024);
     bb_keys_kernel<<<g_part, 256, 0, st>>>(d.desc, d.bh_sys, d.bh_prefix, d.remaining, d.counts, d.pos[in],
                                            d.mass, d.pos[out], d.vel[in], d.vel[out], d.keys[0],
                                            d.idx[0], d.tile_range, d.stats, d.digit_total,
-                                           d.status);
-    bb_hist_kernel<<<g_hist, kSortThreads, 0, st>>>(d.keys[0], d.counts, d.digit_total);
+                                           d.status, tile_keys);
+#define GSIM_BB_SORT(R)                                                                          \
+    do {                                                                                         \
+        bb_hist_kernel<R><<<g_hist, kSortThreads, 0, st>>>(d.keys[0], d.counts, d.digit_total); \
+        for (int pass = 0; pass < 4; ++pass) {                                                   \
+            const int i = pass & 1, o = i ^ 1;                                                   \
+            bb_onesweep_kernel<R><<<g_tiles, kSortThreads, 0, st>>>(                              \
+                d.keys[i], d.idx[i], d.keys[o], d.idx[o], d.counts, (uint32_t)pass,              \
+                d.digit_total, d.status + half * (pass & 1), d.status + half * ((pass + 1) & 1)); \
+        }                                                                                        \
+    } while (0)
     // 4 passes: keys[0] -> keys[1] -> keys[0] -> keys[1] -> keys[0]; look-back
     // words alternate between the two status halves
     const size_t half = 256 * b->tiles_cap;
-    for (int pass = 0; pass < 4; ++pass) {
-        const int i = pass & 1, o = i ^ 1;
-        bb_onesweep_kernel<<<g_tiles, kSortThreads, 0, st>>>(
-            d.keys[i], d.idx[i], d.keys[o], d.idx[o], d.counts, (uint32_t)pass, d.digit_total,
-            d.status + half * (pass & 1), d.status + half * ((pass + 1) & 1));
-    }
+    if (rounds == 8) GSIM_BB_SORT(8);
+    else if (rounds == 12) GSIM_BB_SORT(12);
+    else GSIM_BB_SORT(16);
+#undef GSIM_BB_SORT
     const uint32_t *sk = d.keys[0], *si = d.idx[0];
     const uint32_t g_blk = (uint32_t)std::min<size_t>(b->blocks_cap, 4096);
     bb_prep_kernel<<<g_blk, 256, 0, st>>>(sk, d.counts, d.block_cells, d.block_leaves);
@@ -686,11 +704,12 @@ hipError_t batch_launch_step(gsim_batch *b, int in, int out) {
                                            d.node4, d.ncount, d.top);
     bb_top_kernel<<<std::min<uint32_t>(b->max_systems, 256), 1024, 0, st>>>(
         d.top, d.tile_range, d.sdep, d.cpos, d.counts, d.node4, d.ncount);
-    const uint32_t waves = 1024u * b->waves_per_simd;
+    const uint32_t waves = 1024u * b->tune.waves_per_simd;
     bb_force_kernel<<<waves / 4, 256, 0, st>>>(d.items, d.desc, d.remaining, d.stats, d.counts, d.sorted_src,
                                                d.pos[in], d.vel[in], d.mass, d.node4, d.ncount,
                                                d.pos[out], d.vel[out], d.partial,
-                                               (uint32_t)b->node_capacity, d.wave_acc, d.ichunk);
+                                               (uint32_t)b->node_capacity, d.wave_acc, d.ichunk,
+                                               tune);
     bb_ds_combine_kernel<<<g_part, 256, 0, st>>>(d.desc, d.remaining, d.ds_sys, d.ds_prefix,
                                                  d.counts, d.partial, d.pos[in], d.vel[in],
                                                  d.pos[out], d.vel[out]);

@@ -366,8 +366,6 @@ __device__ __forceinline__ void split_share(const Partials &P, uint32_t k, float
     }
 }
 
-// Particle runs a wave queues in LDS before summing them (kStaged walk).
-constexpr uint32_t kRunQueue = 64;
 
 // The run chunks [queue[0], queue[qn]) of one queue batch, chunk numbers from
 // ch0: each chunk this wave owns ((parts >> (chunk & 3)) & 1) is loaded with
@@ -455,7 +453,9 @@ __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, u
 // 16-byte record tells a cell from a run and gives the run's length.  `parts` (bit k) selects the partials this wave computes; the
 // wave computing partial 0 adds the cells and counts the real-lane visits
 // and acceptances.  kStaged: the walk decides first and queues the particle
-// runs (up to kRunQueue, `queue` is the wave's LDS) -- no decision depends on
+// runs (up to qmax <= kRunQueue, `queue` is the wave's LDS; where the queue
+// is cut changes no sum, as chunks are numbered across runs) -- no decision
+// depends on
 // a force -- then sums the queued runs with the next chunk always in flight
 // (sum_run_queue), raising the wave's issue priority one level per prio_step
 // chunks summed (0: never); the node record, its count and its run length are loaded
@@ -468,16 +468,14 @@ __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, u
 // tail of a launch whose groups are all resident from the start; letting them
 // take issue slots ahead of lighter waves shortens it.  Measured best: 56-72
 // for the single-system 500k two-disk step (traversal -4.5%), 32 for the
-// batched workload (+1.8%); 0 turns it off.
+// batched workload (+1.8%; the batch's step and carry are autotuner knobs,
+// BatchTune); 0 turns it off.
 constexpr uint32_t kPrioStepSingle = 64;
-constexpr uint32_t kPrioStepBatch = 32;
 // A single-system group also starts at level (chunks it summed last step) /
 // kPrioCarry: the heavy groups lead from the first instruction.  Measured on
 // the same workload: 24 -> traversal a further -4% (16: none, 32: -2%, 64: +1%).
-// The batched force launch does the same per work item with kPrioCarryBatch
-// (16 best of 12, 16, 24, 40 on the mixed batch).
+// The batched force launch does the same per work item.
 constexpr uint32_t kPrioCarry = 24;
-constexpr uint32_t kPrioCarryBatch = 16;
 
 __device__ __forceinline__ void set_issue_priority(uint32_t level) {
     if (level >= 3) __builtin_amdgcn_s_setprio(3);
@@ -494,7 +492,7 @@ __device__ __forceinline__ uint32_t walk_partials(
     const uint32_t *__restrict__ ncount, float px, float py, bool real, uint32_t nreal,
     float eps2, float theta2, uint32_t parts, Partials &P, unsigned long long &vis,
     unsigned long long &acc, uint32_t prio_step, uint32_t prio0 = 0,
-    uint32_t *chunks_out = nullptr) {
+    uint32_t *chunks_out = nullptr, uint32_t qmax = kRunQueue) {
     static_assert(kStaged || !kHalf, "half-wave groups need the staged walk");
     const uint32_t lane = threadIdx.x & 63;
     const bool first = (parts & 1u) != 0;
@@ -505,7 +503,7 @@ __device__ __forceinline__ uint32_t walk_partials(
         set_issue_priority(level);
     }
     while (true) {
-      while (ind < end && qn < kRunQueue) {
+      while (ind < end && qn < qmax) {
         const float4 nd = node4[ind];
         const uint32_t cnt = ncount[ind];
         // one round trip per step: both loads are issued here and waited for

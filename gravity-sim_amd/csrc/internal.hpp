@@ -58,6 +58,10 @@ constexpr int kSortTile = kSortThreads * kSortRounds;
 // kSortTile would number more than this.
 constexpr size_t kSortBigTiles = 384;
 
+// Particle runs a walking wave queues in its LDS before summing them (the
+// staged walk's run stack; the batch may use fewer, BatchTune::run_queue).
+constexpr uint32_t kRunQueue = 64;
+
 // Static traversal: at most this many groups per step go to the heavy list
 // (walked by four waves each beside the main launch).
 constexpr uint32_t kMaxHeavy = 512;

@@ -95,6 +95,11 @@ class BatchInfo(C.Structure):
     ]
 
 
+class BatchTuning(C.Structure):
+    _fields_ = [("waves_per_simd", C.c_uint32), ("sort_tile", C.c_uint32),
+                ("run_queue", C.c_uint32), ("prio_step", C.c_uint32), ("prio_carry", C.c_uint32)]
+
+
 class RequestInfo(C.Structure):
     _fields_ = [("n", C.c_uint64), ("steps", C.c_uint64),
                 ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
@@ -155,6 +160,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_batch_bytes_per_particle": (C.c_size_t, []),
         "gsim_batch_set_waves": (st, [vp, C.c_uint32]),
         "gsim_batch_autotune": (st, [vp, C.c_uint32, C.POINTER(C.c_uint32)]),
+        "gsim_batch_get_tuning": (st, [vp, C.POINTER(BatchTuning)]),
+        "gsim_batch_set_tuning": (st, [vp, C.POINTER(BatchTuning)]),
         "gsim_batch_destroy": (st, [vp]),
         "gsim_batch_set_steps": (st, [vp, C.c_uint32, C.c_uint64]),
         "gsim_batch_steps_done": (st, [vp, C.c_uint32, C.POINTER(C.c_uint64)]),
@@ -471,10 +478,25 @@ class Batch:
         _check(_lib.gsim_batch_visits(self._h, C.byref(v)))
         return v.value
 
-    def autotune(self, steps_per_trial: int = 3) -> int:
+    def autotune(self, steps_per_trial: int = 5) -> int:
+        """Tunes every launch knob (see tuning()); returns the chosen waves
+        per SIMD."""
         w = C.c_uint32(0)
         _check(_lib.gsim_batch_autotune(self._h, steps_per_trial, C.byref(w)))
         return w.value
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        _check(_lib.gsim_batch_get_tuning(self._h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in BatchTuning._fields_}
+
+    def set_tuning(self, **kw) -> None:
+        """Sets any of waves_per_simd, sort_tile, run_queue, prio_step,
+        prio_carry; the others keep their values."""
+        cur = self.tuning()
+        cur.update(kw)
+        t = BatchTuning(**cur)
+        _check(_lib.gsim_batch_set_tuning(self._h, C.byref(t)))
 
     def close(self) -> None:
         if getattr(self, "_h", None):

@@ -282,10 +282,28 @@ gsim_status gsim_batch_read(gsim_batch *batch, uint32_t system_id,
 gsim_status gsim_batch_query(const gsim_batch *batch, gsim_batch_info *out);
 /* Device bytes the arena needs per padded particle (for HBM budgeting). */
 size_t gsim_batch_bytes_per_particle(void);
-/* Persistent force-launch width (waves per SIMD, 1..8); results unchanged. */
+/* Launch parameters of a batched step.  None changes any result: systems
+ * stay bit-identical to their alone runs under every setting. */
+typedef struct gsim_batch_tuning {
+    uint32_t waves_per_simd;  /* persistent force-launch width, 1..8 (default 8)      */
+    uint32_t sort_tile;       /* radix sort tile: 2048, 3072 (default) or 4096 keys  */
+    uint32_t run_queue;       /* leaf runs a walking wave queues in its LDS stack
+                                 before summing them, 1..64 (default 64)            */
+    uint32_t prio_step;       /* summed 64-node chunks per issue-priority level of a
+                                 walking wave, 0 = off (default 32)                  */
+    uint32_t prio_carry;      /* a work item starts at level (chunks it summed last
+                                 step) / prio_carry, 0 = off (default 16)            */
+} gsim_batch_tuning;
+gsim_status gsim_batch_get_tuning(const gsim_batch *batch, gsim_batch_tuning *out);
+gsim_status gsim_batch_set_tuning(gsim_batch *batch, const gsim_batch_tuning *tuning);
+/* Persistent force-launch width only (waves per SIMD, 1..8). */
 gsim_status gsim_batch_set_waves(gsim_batch *batch, uint32_t waves_per_simd);
-/* Times each candidate width on the live batch (state is snapshotted and
- * restored after every trial) and keeps the fastest. */
+/* Autotuner: coordinate descent over the gsim_batch_tuning knobs on the live
+ * batch (waves per SIMD, sort tile, run queue, priority step, priority
+ * carry, in that order), each candidate timed twice over steps_per_trial
+ * steps (0: 5) from a snapshot of the state that is restored after every
+ * trial, the faster run counting; keeps the fastest setting (read it back
+ * with gsim_batch_get_tuning).  *chosen_waves_per_simd may be NULL. */
 gsim_status gsim_batch_autotune(gsim_batch *batch, uint32_t steps_per_trial,
                                 uint32_t *chosen_waves_per_simd);
 gsim_status gsim_batch_destroy(gsim_batch *batch);

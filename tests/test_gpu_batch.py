@@ -99,10 +99,42 @@ def test_batch_autotune_leaves_state_unchanged():
         ids = [b.add(p, algo, **kw) for p, algo, kw in systems]
         b.step(2)
         w = b.autotune(steps_per_trial=2)
-        assert 1 <= w <= 8
+        t = b.tuning()
+        assert 1 <= w <= 8 and t["waves_per_simd"] == w
+        assert t["sort_tile"] in (2048, 3072, 4096) and 1 <= t["run_queue"] <= 64
         b.step(3)
         for sid, (p, algo, kw) in zip(ids, systems):
             _same(b.read(sid), _alone(p, algo, kw, 5))
+
+
+@pytest.mark.parametrize("knobs", [
+    {"sort_tile": 2048}, {"sort_tile": 4096}, {"run_queue": 1}, {"run_queue": 7},
+    {"prio_step": 0, "prio_carry": 0}, {"prio_step": 5, "prio_carry": 3},
+    {"waves_per_simd": 1, "sort_tile": 2048, "run_queue": 16}])
+def test_batch_tuning_knobs_change_no_result(knobs):
+    """Every launch knob the autotuner turns leaves each system bit-identical
+    to its alone run (sort tiles, run-queue cuts and issue priority change
+    only how the work is scheduled), including tiles that split systems and
+    queues cut after a single run."""
+    systems = _systems()
+    cap = sum((p.n + 63) // 64 * 64 for p, _, _ in systems)
+    with gsim.Batch(cap, max_systems=16) as b:
+        ids = [b.add(p, algo, **kw) for p, algo, kw in systems]
+        b.set_tuning(**knobs)
+        assert all(b.tuning()[k] == v for k, v in knobs.items())
+        b.step(3)
+        for sid, (p, algo, kw) in zip(ids, systems):
+            _say(f"tuning {knobs}: {algo} n={p.n} {kw}")
+            _same(b.read(sid), _alone(p, algo, kw, 3))
+
+
+def test_batch_tuning_rejects_bad_values():
+    with gsim.Batch(8192, max_systems=2) as b:
+        for bad in ({"sort_tile": 1000}, {"run_queue": 0}, {"run_queue": 65},
+                    {"waves_per_simd": 9}, {"prio_step": 5000}):
+            with pytest.raises(gsim.GsimError) as e:
+                b.set_tuning(**bad)
+            assert e.value.status == 1
 
 
 def test_batch_capacity_is_enforced():
