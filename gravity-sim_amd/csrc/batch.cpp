@@ -122,7 +122,9 @@ gsim_status upload_membership(gsim_batch *b) {
             d.sorted_base = bh_prefix.back();
             bh_sys.push_back(di);
             bh_prefix.push_back(bh_prefix.back() + s.na);
-            for (uint32_t g = 0; g < s.na / 64; ++g) items.push_back(make_uint2(di, g));
+            // one work item per vote group: 64 sorted positions, or 32 at W = 32
+            d.w = s.params.group_width;
+            for (uint32_t g = 0; g < s.na / d.w; ++g) items.push_back(make_uint2(di, g));
         } else {
             uint32_t R = 0, S = 0;
             gsim::ds_pick_shape(s.na, &R, &S);
@@ -296,8 +298,9 @@ gsim_status gsim_batch_add(gsim_batch *b, const gsim_params *p, const float *pos
     if (n == 0) return fail(GSIM_ERROR_INVALID_ARGUMENT, "The simulation must contain at least one particle!");
     if (p->algorithm != GSIM_DIRECT_SUM && p->algorithm != GSIM_BARNES_HUT)
         return fail(GSIM_ERROR_INVALID_ARGUMENT, "A valid simulation algorithm must be given!");
-    if (p->group_width != 0 && p->group_width != 64)
-        return fail(GSIM_ERROR_INVALID_ARGUMENT, "batched systems use W = 64");
+    const uint32_t W = p->group_width ? p->group_width : 64;
+    if (W != 32 && W != 64)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "group_width must be 32 or 64 (got %u)", W);
     const float vals[] = {p->gravitational_const, p->simulation_time, p->simulation_speed,
                           p->softening_len, p->accuracy_parameter};
     for (float v : vals)
@@ -345,7 +348,7 @@ gsim_status gsim_batch_add(gsim_batch *b, const gsim_params *p, const float *pos
     }
     gsim::BatchSystem s;
     s.params = *p;
-    s.params.group_width = 64;
+    s.params.group_width = W;
     s.off = off;
     s.na = (uint32_t)na;
     s.n = (uint32_t)n;

@@ -20,7 +20,8 @@ struct SysDesc {
     float dt, pad0;
     uint32_t bh_rank, sorted_base;   // Barnes-Hut: rank in the global sort, first sorted position
     uint32_t ds_part_off, ds_jlen;   // direct sum: partial-sum offset (float2) and slice length
-    uint32_t ds_slices, pad1;
+    uint32_t ds_slices;
+    uint32_t w;                      // Barnes-Hut vote width: 64 or 32
 };
 
 // Per Barnes-Hut rank, written by the emit kernel every step.

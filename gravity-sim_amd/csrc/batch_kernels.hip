@@ -465,6 +465,13 @@ __global__ __launch_bounds__(1024) void bb_top_kernel(const TileCell *__restrict
 
 // Barnes-Hut group g of one system: the walk of bh_kernels.hip over the
 // system's node range [root, end_slot) with its own constants.
+// kHalf: a group of the W = 32 vote (the published runs' subgroup width), one
+// 32-particle group per wave with sorted position g * 32 + l held by lanes l
+// and l + 32, walked exactly as bh_traverse32_kernel walks it (both halves
+// vote alike; each half sums 32 nodes of every 64-node chunk, and the halves'
+// partials are added before the acceleration), so the result is that
+// kernel's bit for bit.
+template <bool kHalf>
 __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st, float4 *stage,
                               uint2 *queue, const uint32_t *__restrict__ sorted_src,
                               const float2 *__restrict__ pos, const float2 *__restrict__ vel,
@@ -474,9 +481,11 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
                               unsigned long long &vis, unsigned long long &acc,
                               uint32_t *__restrict__ chunks_io, const BatchTune &tune) {
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t src = sorted_src[d.sorted_base + g * 64 + lane];
+    const uint32_t src = kHalf ? sorted_src[d.sorted_base + g * 32 + (lane & 31u)]
+                               : sorted_src[d.sorted_base + g * 64 + lane];
     const bool real = src != kBatchSentinel;
-    const uint32_t nreal = (uint32_t)__popcll(__ballot(real));
+    const uint32_t nreal = kHalf ? (uint32_t)__popcll(__ballot(real) & 0xffffffffull)
+                                 : (uint32_t)__popcll(__ballot(real));
     float px = 0.0f, py = 0.0f;
     if (real) {
         const float2 p = pos[src];
@@ -489,11 +498,21 @@ __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st,
     // start at the issue priority this work item reached in the last step
     const uint32_t prio0 = tune.prio_carry ? *chunks_io / tune.prio_carry : 0u;
     uint32_t chunks = 0;
-    (void)walk_partials<true>(root, end, stage, queue, node4, ncount, px, py, real, nreal, d.eps2,
-                              d.theta2, 0xfu, P, vis, acc, tune.prio_step, prio0, &chunks,
-                              tune.run_queue);
+    (void)walk_partials<true, kHalf>(root, end, stage, queue, node4, ncount, px, py, real, nreal,
+                                     d.eps2, d.theta2, 0xfu, P, vis, acc, tune.prio_step, prio0,
+                                     &chunks, tune.run_queue);
     if (lane == 0) *chunks_io = chunks;
-    if (real) {
+    if (kHalf) {
+        P.x0 += __shfl_xor(P.x0, 32, 64);
+        P.y0 += __shfl_xor(P.y0, 32, 64);
+        P.x1 += __shfl_xor(P.x1, 32, 64);
+        P.y1 += __shfl_xor(P.y1, 32, 64);
+        P.x2 += __shfl_xor(P.x2, 32, 64);
+        P.y2 += __shfl_xor(P.y2, 32, 64);
+        P.x3 += __shfl_xor(P.x3, 32, 64);
+        P.y3 += __shfl_xor(P.y3, 32, 64);
+    }
+    if (real && (!kHalf || lane < 32)) {
         float ax, ay;
         partial_sum(P, ax, ay);
         const float2 v = vel[src];
@@ -570,8 +589,14 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             // no particle inside the domain, or a spent budget (the keys
             // kernel has already written the system's output)
             if (!live || st.n_in == 0) continue;
-            bb_walk_group(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel, node4, ncount,
-                          pos_out, vel_out, node_cap, vis, acc, &ichunk[it], tune);
+            if (d.w == 32)
+                bb_walk_group<true>(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel, node4,
+                                    ncount, pos_out, vel_out, node_cap, vis, acc, &ichunk[it],
+                                    tune);
+            else
+                bb_walk_group<false>(item.y, d, st, stage[w], runq[w], sorted_src, pos, vel,
+                                     node4, ncount, pos_out, vel_out, node_cap, vis, acc,
+                                     &ichunk[it], tune);
         } else if (live) {
             bb_ds_tile(item.y, d, pos, mass, partial);
         }

@@ -242,7 +242,8 @@ gsim_status gsim_ds_step_noalloc(const gsim_params *params, const float *pos_xy,
  * one node array; one persistent launch computes every system's forces), and
  * the whole step is replayed from a captured hipGraph.  Membership can change
  * between steps without re-capturing.  A system's results are bit-identical
- * to stepping it alone with gsim_step (group width 64, default launch shape).
+ * to stepping it alone with gsim_step (the same group width, 32 or 64, and
+ * the default launch shape).
  * Direct-sum systems of 16,384 or more padded particles are refused: they
  * fill the GPU alone and are stepped alone with the symmetric kernel.
  * No counterpart in the reference, which runs one system per process. */

@@ -25,7 +25,9 @@ def _systems():
              ("galaxy-collision", 9000, "barnes-hut", {"accuracy_parameter": 0.7}),
              ("exponential-disk", 20000, "barnes-hut", {"accuracy_parameter": 0.7}),
              ("random", 12345, "barnes-hut", {"accuracy_parameter": 1.0, "simulation_speed": 2.0}),
-             ("galaxy", 40000, "barnes-hut", {"accuracy_parameter": 0.5, "gravitational_const": 0.5})]
+             ("galaxy", 40000, "barnes-hut", {"accuracy_parameter": 0.5, "gravitational_const": 0.5}),
+             ("galaxy-collision", 15000, "barnes-hut", {"accuracy_parameter": 0.7, "group_width": 32}),
+             ("exponential-disk", 4321, "barnes-hut", {"group_width": 32, "simulation_speed": 2.0})]
     for k, (kind, n, algo, kw) in enumerate(specs):
         p = gsim.generate(kind, n, 100.0, 1.0, 10.0, kw.get("gravitational_const", 1.0), seed=100 + k)
         if k == 5:
@@ -142,7 +144,7 @@ def test_batch_capacity_is_enforced():
     with gsim.Batch(8192, max_systems=2) as b:
         b.add(p, "barnes-hut")
         with pytest.raises(gsim.GsimError) as e:
-            b.add(p, "barnes-hut", group_width=32)  # batched systems vote with W = 64
+            b.add(p, "barnes-hut", group_width=48)  # votes are 32 or 64 wide
         assert e.value.status == 1
         with pytest.raises(gsim.GsimError) as e:
             b.add(p, "direct-sum")  # 5056 more padded particles do not fit in 8192

@@ -38,7 +38,8 @@ def _requests():
              ("random", 12000, "barnes-hut", {"accuracy_parameter": 1.0, "simulation_speed": 2.0}, 3),
              ("two-exponential-disks", 9000, "barnes-hut", {}, 17),
              ("galaxy", 2000, "direct-sum", {}, 1),
-             ("random", 40000, "barnes-hut", {"accuracy_parameter": 0.7}, 8)]
+             ("random", 40000, "barnes-hut", {"accuracy_parameter": 0.7}, 8),
+             ("galaxy", 8000, "barnes-hut", {"group_width": 32}, 11)]
     out = []
     for k, (kind, n, algo, kw, steps) in enumerate(specs):
         p = gsim.generate(kind, n, 100.0, 1.0, 10.0, seed=300 + k)
