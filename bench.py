@@ -395,7 +395,7 @@ def bench_serve(seconds: float, budget_gb: float = 200.0, max_active: int = 0,
                                "N log-uniform [2k, 2M], preset random/galaxy/galaxy-collision "
                                "(size 100, masses U[1,10]), Barnes-Hut theta in {0.5,0.7,1.0}, "
                                f"steps U{{10..1000}}, HBM budget {budget_gb:g} GB",
-                   "capacity_particles": st["capacity_particles"],
+                   "capacity_particles": st["capacity_particles"], "arenas": st["arenas"],
                    "max_active_particles": max_active, "deadline_s": deadline_s},
     }
 

@@ -1,12 +1,15 @@
 // Serving: continuous batching of independent simulations on one GPU.
 //
 // Clients submit systems (particles, parameters, a step count) from any
-// thread.  A worker thread owns a gsim_batch sized by the HBM budget; each
-// iteration it admits queued requests first-come first-served while the arena
-// (and the optional active-particle cap) has room, enqueues one batched step
-// for every admitted system, and retires the systems whose step budget is
-// reached: their state is read back, they leave the batch, and the space
-// admits the next requests.  Requests that arrive while others run join at the
+// thread.  A worker thread owns the arenas the HBM budget buys: gsim_batch
+// arenas of at most kBatchMaxParticles each (32-bit node indices inside an
+// arena), each with its own stream, so the arenas' steps also overlap on the
+// device.  Each iteration it admits queued requests first-come first-served
+// into the first arena with room (while the optional active-particle cap
+// allows), enqueues the same number of batched steps on every arena that
+// holds systems, and retires the systems whose step budget is reached: their
+// state is read back, they leave their arena, and the space admits the next
+// requests.  Requests that arrive while others run join at the
 // next step, so their work shares every kernel launch with the running ones.
 // A system's step budget is enforced on the device, so it stops at exactly its
 // step count whatever the host timing; its result is bit-identical to running
@@ -40,7 +43,7 @@ struct Request {
     uint32_t steps = 0;
     std::vector<float> pos, vel, mass;  // input, then the result
     Clock::time_point submitted, admitted, done;
-    uint32_t batch_id = 0;
+    uint32_t arena = 0, batch_id = 0;
     int state = 0;  // 0 queued, 1 running, 2 done, 3 failed
     uint64_t interactions = 0;  // direct sum: N_pad^2 per step (Barnes-Hut counted per batch)
 };
@@ -48,8 +51,8 @@ struct Request {
 }  // namespace
 
 struct gsim_server {
-    gsim_batch *batch = nullptr;
-    size_t capacity = 0, max_active = 0, active_particles = 0;
+    std::vector<gsim_batch *> arenas;
+    size_t capacity = 0, largest_arena = 0, max_active = 0, active_particles = 0;
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -57,7 +60,7 @@ struct gsim_server {
     uint64_t next_id = 1;
     std::deque<Request *> queue;
     std::map<uint64_t, std::unique_ptr<Request>> all;
-    std::map<uint32_t, Request *> running;  // batch id -> request
+    std::map<uint64_t, Request *> running;  // (arena << 32 | batch id) -> request
     std::vector<double> latencies_ms;
     uint64_t completed = 0, failed = 0, batch_steps = 0, ds_interactions = 0;
     Clock::time_point first_submit, last_done;
@@ -81,13 +84,17 @@ void worker_loop(gsim_server *s) {
             const size_t na = padded(r->n);
             if (s->max_active && s->active_particles + na > s->max_active && !s->running.empty())
                 break;
-            uint32_t bid = 0;
-            gsim_status st = gsim_batch_add(s->batch, &r->params, r->pos.data(), r->vel.data(),
-                                            r->mass.data(), r->n, &bid);
+            uint32_t bid = 0, ai = 0;
+            gsim_status st = GSIM_ERROR_CAPACITY;
+            for (; ai < s->arenas.size(); ++ai) {  // the first arena with room
+                st = gsim_batch_add(s->arenas[ai], &r->params, r->pos.data(), r->vel.data(),
+                                    r->mass.data(), r->n, &bid);
+                if (st != GSIM_ERROR_CAPACITY) break;
+            }
             if (st == GSIM_ERROR_CAPACITY && !s->running.empty()) break;  // wait for room
             s->queue.pop_front();
-            if (st != GSIM_OK || gsim_batch_set_steps(s->batch, bid, r->steps) != GSIM_OK) {
-                if (st == GSIM_OK) gsim_batch_remove(s->batch, bid);
+            if (st != GSIM_OK || gsim_batch_set_steps(s->arenas[ai], bid, r->steps) != GSIM_OK) {
+                if (st == GSIM_OK) gsim_batch_remove(s->arenas[ai], bid);
                 r->state = 3;
                 r->done = Clock::now();
                 ++s->failed;
@@ -95,10 +102,11 @@ void worker_loop(gsim_server *s) {
                 s->cv_done.notify_all();
                 continue;
             }
+            r->arena = ai;
             r->batch_id = bid;
             r->state = 1;
             r->admitted = Clock::now();
-            s->running[bid] = r;
+            s->running[(uint64_t)ai << 32 | bid] = r;
             s->active_particles += na;
         }
         if (s->running.empty()) {
@@ -109,14 +117,22 @@ void worker_loop(gsim_server *s) {
         // kChunkSteps at a time, so new arrivals wait at most one chunk), then
         // wait for the device; outside the lock, so clients keep submitting.
         uint64_t chunk = kChunkSteps;
+        std::vector<char> busy(s->arenas.size(), 0);
         for (auto &kv : s->running) {
+            const Request *r = kv.second;
+            busy[r->arena] = 1;
             uint64_t done = 0;
-            if (gsim_batch_steps_done(s->batch, kv.first, &done) == GSIM_OK && kv.second->steps > done)
-                chunk = std::min<uint64_t>(chunk, kv.second->steps - done);
+            if (gsim_batch_steps_done(s->arenas[r->arena], r->batch_id, &done) == GSIM_OK &&
+                r->steps > done)
+                chunk = std::min<uint64_t>(chunk, r->steps - done);
         }
         lk.unlock();
-        gsim_status st = gsim_batch_step(s->batch, (uint32_t)chunk);
-        if (st == GSIM_OK) st = gsim_batch_synchronize(s->batch);
+        // every busy arena's steps go out on its own stream before any wait
+        gsim_status st = GSIM_OK;
+        for (size_t a = 0; a < s->arenas.size() && st == GSIM_OK; ++a)
+            if (busy[a]) st = gsim_batch_step(s->arenas[a], (uint32_t)chunk);
+        for (size_t a = 0; a < s->arenas.size() && st == GSIM_OK; ++a)
+            if (busy[a]) st = gsim_batch_synchronize(s->arenas[a]);
         lk.lock();
         if (st != GSIM_OK) {
             std::snprintf(s->worker_error, sizeof(s->worker_error), "%s", gsim_last_error());
@@ -134,15 +150,18 @@ void worker_loop(gsim_server *s) {
         // retire the systems whose budget is spent
         std::vector<Request *> finished;
         for (auto &kv : s->running) {
+            const Request *r = kv.second;
             uint64_t done = 0;
-            if (gsim_batch_steps_done(s->batch, kv.first, &done) == GSIM_OK && done >= kv.second->steps)
+            if (gsim_batch_steps_done(s->arenas[r->arena], r->batch_id, &done) == GSIM_OK &&
+                done >= r->steps)
                 finished.push_back(kv.second);
         }
         for (Request *r : finished) {
-            const gsim_status rs = gsim_batch_read(s->batch, r->batch_id, r->pos.data(),
-                                                   r->vel.data(), r->mass.data());
-            gsim_batch_remove(s->batch, r->batch_id);
-            s->running.erase(r->batch_id);
+            gsim_batch *b = s->arenas[r->arena];
+            const gsim_status rs = gsim_batch_read(b, r->batch_id, r->pos.data(), r->vel.data(),
+                                                   r->mass.data());
+            gsim_batch_remove(b, r->batch_id);
+            s->running.erase((uint64_t)r->arena << 32 | r->batch_id);
             s->active_particles -= padded(r->n);
             r->done = Clock::now();
             if (rs == GSIM_OK) {
@@ -178,27 +197,48 @@ double percentile(std::vector<double> v, double q) {
 
 extern "C" {
 
-gsim_status gsim_server_create(int32_t device, size_t hbm_budget_bytes, uint32_t max_systems,
-                               size_t max_active_particles, gsim_server **out) {
+gsim_status gsim_server_create_arenas(int32_t device, size_t hbm_budget_bytes,
+                                      uint32_t max_systems, size_t max_active_particles,
+                                      size_t arena_particles, gsim_server **out) {
     gsim::clear_error();
     if (!out) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_server_create: out is NULL");
     *out = nullptr;
     const size_t per = gsim_batch_bytes_per_particle();
-    // the budget buys arena space up to the arena's index limit
-    const size_t capacity = std::min(hbm_budget_bytes / per / 64 * 64, gsim::kBatchMaxParticles);
-    if (capacity < 64) return fail(GSIM_ERROR_INVALID_ARGUMENT, "HBM budget below one 64-particle system");
+    // the budget buys this many padded particles, in arenas of at most the
+    // arena's index limit (or the caller's smaller cap)
+    const size_t total = hbm_budget_bytes / per / 64 * 64;
+    size_t arena = arena_particles ? std::min(arena_particles / 64 * 64, gsim::kBatchMaxParticles)
+                                   : gsim::kBatchMaxParticles;
+    if (total < 64 || arena < 64)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "HBM budget below one 64-particle system");
+    if ((total + arena - 1) / arena > 64)
+        return fail(GSIM_ERROR_INVALID_ARGUMENT, "the budget would need more than 64 arenas");
     gsim_server *s = new (std::nothrow) gsim_server();
     if (!s) return fail(GSIM_ERROR_OUT_OF_MEMORY, "host allocation failed");
-    gsim_status st = gsim_batch_create(device, capacity, max_systems, &s->batch);
-    if (st != GSIM_OK) {
-        delete s;
-        return st;
+    for (size_t left = total; left >= 64;) {
+        const size_t cap = std::min(left, arena);
+        gsim_batch *b = nullptr;
+        gsim_status st = gsim_batch_create(device, cap, max_systems, &b);
+        if (st != GSIM_OK) {
+            for (gsim_batch *x : s->arenas) gsim_batch_destroy(x);
+            delete s;
+            return st;
+        }
+        s->arenas.push_back(b);
+        s->capacity += cap;
+        s->largest_arena = std::max(s->largest_arena, cap);
+        left -= cap;
     }
-    s->capacity = capacity;
     s->max_active = max_active_particles;
     s->worker = std::thread(worker_loop, s);
     *out = s;
     return GSIM_OK;
+}
+
+gsim_status gsim_server_create(int32_t device, size_t hbm_budget_bytes, uint32_t max_systems,
+                               size_t max_active_particles, gsim_server **out) {
+    return gsim_server_create_arenas(device, hbm_budget_bytes, max_systems, max_active_particles,
+                                     0, out);
 }
 
 gsim_status gsim_server_submit(gsim_server *s, const gsim_params *params, const float *pos_xy,
@@ -208,8 +248,9 @@ gsim_status gsim_server_submit(gsim_server *s, const gsim_params *params, const 
     if (!s || !params || !pos_xy || !vel_xy || !mass || !request_id)
         return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_server_submit: NULL argument");
     if (n == 0) return fail(GSIM_ERROR_INVALID_ARGUMENT, "The simulation must contain at least one particle!");
-    if (padded(n) > s->capacity)
-        return fail(GSIM_ERROR_CAPACITY, "a %zu-particle system exceeds the HBM budget", n);
+    if (padded(n) > s->largest_arena)
+        return fail(GSIM_ERROR_CAPACITY, "a %zu-particle system exceeds the HBM budget%s", n,
+                    s->arenas.size() > 1 ? " (its largest arena)" : "");
     if (params->algorithm == GSIM_DIRECT_SUM && padded(n) >= gsim::kDsSymMinParticles)
         return fail(GSIM_ERROR_INVALID_ARGUMENT,
                     "direct-sum systems of %zu or more padded particles are stepped alone (gsim_create)",
@@ -323,11 +364,17 @@ gsim_status gsim_server_query(gsim_server *s, gsim_server_info *out) {
         out->ds_interactions = s->ds_interactions;
         out->capacity_particles = s->capacity;
     }
-    gsim_batch_accepts(s->batch, &accepts);
-    out->bh_interactions = accepts;
     uint64_t visits = 0;
-    gsim_batch_visits(s->batch, &visits);
+    for (gsim_batch *b : s->arenas) {
+        uint64_t a = 0, v = 0;
+        gsim_batch_accepts(b, &a);
+        gsim_batch_visits(b, &v);
+        accepts += a;
+        visits += v;
+    }
+    out->bh_interactions = accepts;
     out->bh_visits = visits;
+    out->arenas = (uint32_t)s->arenas.size();
     return GSIM_OK;
 }
 
@@ -340,7 +387,7 @@ gsim_status gsim_server_destroy(gsim_server *s) {
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    gsim_batch_destroy(s->batch);
+    for (gsim_batch *b : s->arenas) gsim_batch_destroy(b);
     delete s;
     return GSIM_OK;
 }

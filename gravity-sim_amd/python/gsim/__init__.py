@@ -112,7 +112,7 @@ class ServerInfo(C.Structure):
         ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
         ("span_s", C.c_double), ("ds_interactions", C.c_uint64),
         ("bh_interactions", C.c_uint64), ("capacity_particles", C.c_uint64),
-        ("bh_visits", C.c_uint64),
+        ("bh_visits", C.c_uint64), ("arenas", C.c_uint32),
     ]
 
 
@@ -169,6 +169,8 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         "gsim_batch_visits": (st, [vp, C.POINTER(C.c_uint64)]),
         "gsim_server_create": (st, [C.c_int32, C.c_size_t, C.c_uint32, C.c_size_t,
                                     C.POINTER(vp)]),
+        "gsim_server_create_arenas": (st, [C.c_int32, C.c_size_t, C.c_uint32, C.c_size_t,
+                                           C.c_size_t, C.POINTER(vp)]),
         "gsim_server_submit": (st, [vp, C.POINTER(Params), fp, fp, fp, C.c_size_t,
                                     C.c_uint32, C.POINTER(C.c_uint64)]),
         "gsim_server_wait": (st, [vp, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
@@ -524,11 +526,12 @@ class Server:
     latency."""
 
     def __init__(self, hbm_budget_bytes: int, max_systems: int = 1024,
-                 max_active_particles: int = 0, device: int = 0):
+                 max_active_particles: int = 0, device: int = 0, arena_particles: int = 0):
         lib = load_library()
         h = C.c_void_p()
-        _check(lib.gsim_server_create(device, hbm_budget_bytes, max_systems,
-                                      max_active_particles, C.byref(h)))
+        _check(lib.gsim_server_create_arenas(device, hbm_budget_bytes, max_systems,
+                                             max_active_particles, arena_particles,
+                                             C.byref(h)))
         self._h = h
         self.n = {}
 

@@ -332,16 +332,26 @@ typedef struct gsim_server_info {
     double span_s;                 /* first submit -> last completion         */
     uint64_t ds_interactions;      /* direct sum: padded N^2 per step, completed requests */
     uint64_t bh_interactions;      /* Barnes-Hut accepted interactions, all steps so far  */
-    uint64_t capacity_particles;   /* arena capacity the budget bought        */
+    uint64_t capacity_particles;   /* arena capacity the budget bought, all arenas */
     uint64_t bh_visits;            /* Barnes-Hut real-lane node visits, all steps so far  */
+    uint32_t arenas;               /* arenas the budget was split into          */
 } gsim_server_info;
 
-/* hbm_budget_bytes sizes the arena (gsim_batch_bytes_per_particle per padded
- * particle, at most 268,435,392 particles: 32-bit node indices);
- * max_active_particles caps the particles stepping at once (0: the arena
- * capacity is the only limit). */
+/* hbm_budget_bytes buys gsim_batch_bytes_per_particle per padded particle,
+ * split into arenas (batches) of at most 268,435,392 particles each (32-bit
+ * node indices inside an arena): a 200 GB budget is 4 arenas, 913M
+ * particles.  Each arena has its own stream, so their steps overlap; a
+ * request goes to the first arena with room, and must fit in one.
+ * max_systems applies per arena (at most 2048); max_active_particles caps
+ * the particles stepping at once (0: the arenas' capacity is the only
+ * limit). */
 gsim_status gsim_server_create(int32_t device, size_t hbm_budget_bytes, uint32_t max_systems,
                                size_t max_active_particles, gsim_server **out);
+/* The same with each arena capped at arena_particles (0: the 268,435,392
+ * maximum); at most 64 arenas. */
+gsim_status gsim_server_create_arenas(int32_t device, size_t hbm_budget_bytes,
+                                      uint32_t max_systems, size_t max_active_particles,
+                                      size_t arena_particles, gsim_server **out);
 /* Queues one system for `steps` steps; inputs are copied.  GSIM_ERROR_CAPACITY
  * when the system alone exceeds the budget. */
 gsim_status gsim_server_submit(gsim_server *server, const gsim_params *params,

@@ -47,11 +47,12 @@ def _requests():
     return out
 
 
-def _serve(reqs, budget_particles, max_active=0, stagger_s=0.0):
+def _serve(reqs, budget_particles, max_active=0, stagger_s=0.0, arena=0):
     """Submits every request (from a second thread, `stagger_s` apart) and
     returns results in submission order."""
     budget = budget_particles * gsim.Batch.bytes_per_particle()
-    with gsim.Server(budget, max_systems=64, max_active_particles=max_active) as srv:
+    with gsim.Server(budget, max_systems=64, max_active_particles=max_active,
+                     arena_particles=arena) as srv:
         rids = [None] * len(reqs)
 
         def client():
@@ -97,6 +98,28 @@ def test_server_results_match_alone_runs():
     got, _ = _serve(reqs[:4], total, max_active=1)
     for (res, info), w in zip(got, want[:4]):
         _same(res, w)
+
+
+def test_server_spreads_requests_over_arenas():
+    """A budget split into three arenas (each with its own stream, stepped
+    together): requests land in whichever arena has room and every result is
+    still its alone run's, bit for bit; a request larger than the largest
+    arena is refused at submission."""
+    reqs = _requests()
+    want = [_alone(p, algo, kw, steps) for p, algo, kw, steps in reqs]
+    arena = 40960  # the largest request, 40,000 particles, fits one arena
+    _say("serve: three arenas")
+    got, stats = _serve(reqs, 3 * arena, arena=arena, stagger_s=0.002)
+    for (res, info), w, (p, algo, kw, steps) in zip(got, want, reqs):
+        _same(res, w)
+    assert stats["arenas"] == 3 and stats["capacity_particles"] == 3 * arena
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
+    big = gsim.generate("random", arena + 1, 100.0, 1.0, 10.0, seed=12)
+    budget = 3 * arena * gsim.Batch.bytes_per_particle()
+    with gsim.Server(budget, max_systems=8, arena_particles=arena) as srv:
+        with pytest.raises(gsim.GsimError) as e:
+            srv.submit(big, 2, "barnes-hut")
+        assert e.value.status == 6
 
 
 def test_server_edge_cases():
