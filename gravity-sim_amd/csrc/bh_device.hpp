@@ -125,15 +125,12 @@ __device__ __forceinline__ TileCell reduce4(const TileCell &a, const TileCell &b
     return r;
 }
 
-__device__ __forceinline__ void emit_cell(const TileCell &c, int d,
-                                          const int8_t *__restrict__ sdep,
-                                          const uint32_t *__restrict__ cpos,
-                                          float4 *__restrict__ node4,
-                                          uint32_t *__restrict__ ncount,
-                                          float r0 = kFixedR0) {
+// The depth-d cell node c (first particle c.f) at its pre-order slot, given.
+__device__ __forceinline__ void emit_cell_at(const TileCell &c, int d, uint32_t slot,
+                                             float4 *__restrict__ node4,
+                                             uint32_t *__restrict__ ncount,
+                                             float r0 = kFixedR0) {
     if (c.y <= 1u) return;
-    const uint32_t f = c.f;
-    const uint32_t slot = f + cpos[f] + (uint32_t)(d - (int)sdep[f] - 1);
     float px = c.sx, py = c.sy;
     if (c.sm != 0.0f) {  // TreeMoveShader.comp:66-67
         px = px / c.sm;
@@ -141,6 +138,22 @@ __device__ __forceinline__ void emit_cell(const TileCell &c, int d,
     }
     ncount[slot] = c.y;
     node4[slot] = make_float4(px, py, c.sm, cell_radius(d, r0));
+}
+
+// Depth d of the cell starting at sorted position f: slot f + C(f) + (d - s(f) - 1).
+__device__ __forceinline__ uint32_t cell_slot(uint32_t f, int d, const int8_t *__restrict__ sdep,
+                                              const uint32_t *__restrict__ cpos) {
+    return f + cpos[f] + (uint32_t)(d - (int)sdep[f] - 1);
+}
+
+__device__ __forceinline__ void emit_cell(const TileCell &c, int d,
+                                          const int8_t *__restrict__ sdep,
+                                          const uint32_t *__restrict__ cpos,
+                                          float4 *__restrict__ node4,
+                                          uint32_t *__restrict__ ncount,
+                                          float r0 = kFixedR0) {
+    if (c.y <= 1u) return;
+    emit_cell_at(c, d, cell_slot(c.f, d, sdep, cpos), node4, ncount, r0);
 }
 
 // A particle node's fourth word (the reference's radius 0): minus the number
@@ -259,41 +272,54 @@ __device__ __forceinline__ float4 lane_leaf_sum(const float4 *__restrict__ prod,
 }
 
 // A larger leaf [f, e) (wave-uniform bounds) by the whole wave: 64 products
-// per coalesced load (the next chunk in flight while the current one is
-// summed) staged in the wave's LDS as packed 12-byte (m*x, m*y, m) records.
-// Lanes 0, 1 and 2 each sum one component, so the wave issues one add per
-// product; each component's order is the leaf's particle order, from +0.  A
-// full chunk is unrolled so the reads run ahead of the dependent add chain.
+// per coalesced load, four chunks in flight ahead of the one being summed,
+// staged in the wave's LDS component by component (64 m*x, 64 m*y, 64 m).
+// Lanes 0, 1 and 2 each sum one component, reading it 16 bytes at a time, so
+// the wave issues one add per product; each component's order is the leaf's
+// particle order, from +0.  The dependent chain of adds is what remains.
 __device__ __forceinline__ float4 wave_leaf_sum(float4 *stage, const float4 *__restrict__ prod,
                                                 uint32_t f, uint32_t e, uint32_t lane) {
     typedef float v4 __attribute__((ext_vector_type(4)));  // stays in registers
     const v4 *vp = reinterpret_cast<const v4 *>(prod);
-    const v4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     float *st = reinterpret_cast<float *>(stage);
     const uint32_t comp = lane < 3u ? lane : 0u;  // the component this lane sums
+    const v4 *sc = reinterpret_cast<const v4 *>(st + 64 * comp);
     float s = 0.0f;
-    v4 next = f + lane < e ? vp[f + lane] : zero;
-    for (uint32_t base = f; base < e; base += 64) {
-        const uint32_t cnt = min(64u, e - base);
-        st[3 * lane] = next.x;
-        st[3 * lane + 1] = next.y;
-        st[3 * lane + 2] = next.z;
-        __builtin_amdgcn_wave_barrier();
-        const uint32_t nb = base + 64 + lane;
-        next = nb < e ? vp[nb] : zero;
-        if (cnt == 64u) {
+    // loads past the leaf read its last record instead (never summed): no
+    // lane-dependent branch, so each wait is for the one chunk it needs
+    v4 q[4];
 #pragma unroll
-            for (int k = 0; k < 64; k += 16) {
-                float r[16];
+    for (uint32_t u = 0; u < 4; ++u) q[u] = vp[min(f + 64 * u + lane, e - 1)];
+    for (uint32_t base = f; base < e; base += 256) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) r[u] = st[3 * (k + u) + comp];
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t cb = base + 64 * u;
+            if (cb >= e) break;
+            st[lane] = q[u].x;
+            st[64 + lane] = q[u].y;
+            st[128 + lane] = q[u].z;
+            __builtin_amdgcn_wave_barrier();
+            q[u] = vp[min(cb + 256 + lane, e - 1)];
+            const uint32_t cnt = min(64u, e - cb);
+            if (cnt == 64u) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) s = s + r[u];
+                for (int h = 0; h < 16; h += 4) {
+                    v4 r[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) r[j] = sc[h + j];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        s = s + r[j].x;
+                        s = s + r[j].y;
+                        s = s + r[j].z;
+                        s = s + r[j].w;
+                    }
+                }
+            } else {
+                for (uint32_t k = 0; k < cnt; ++k) s = s + st[64 * comp + k];
             }
-        } else {
-            for (uint32_t k = 0; k < cnt; ++k) s = s + st[3 * k + comp];
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
     }
     return make_float4(__shfl(s, 0, 64), __shfl(s, 1, 64), __shfl(s, 2, 64), 0.0f);
 }

@@ -11,19 +11,21 @@
 //   2. sort      stable LSD radix sort of (cell, index), 3 passes of 8 bits,
 //                so a leaf's particles are in ascending index order: one
 //                histogram launch, then one launch per pass (look-back).
-//   3. prep/scan s(k) = depth shared by sorted keys k-1 and k; the cell nodes
+//   3. prep      s(k) = depth shared by sorted keys k-1 and k; the cell nodes
 //                that start at k are the depths in (s(k), s(k+1)], so an
 //                exclusive scan C(k) of their number gives every pre-order
 //                slot: particle k -> k + C(k+1); the depth-d cell starting at
-//                k -> k + C(k) + (d - s(k) - 1).
-//   4. emit      particle nodes, sorted_src, m*p products, leaf records.
+//                k -> k + C(k) + (d - s(k) - 1).  Per-block counts here.
+//   4. emit      the scan (each block adds up the counts before it), particle
+//                nodes, sorted_src, m*p products, leaf records.
 //   5. leafsum   leaf sums in ascending particle index (the canonical order of
 //                the reference's float atomics, InitShader.comp:107-109).
 //   6. tile      one workgroup per depth-5 cell: the 32x32 leaf tile is
 //                reduced in LDS level by level with ((c0+c1)+c2)+c3 exactly as
 //                TreeInitShader.comp:78-83, counts y = sum + [sum > 1]; cells
 //                with y > 1 are written as nodes (TreeMoveShader.comp:60-77).
-//   7. top       one workgroup reduces the 1024 depth-5 cells to the root.
+//   7. top       the last tile workgroup to finish reduces the 1024 depth-5
+//                cells to the root.
 //   8. traverse  groups of W sorted slots walk one path; a node is accepted
 //                iff all W lanes accept it (ForceShader.comp:111-141), then
 //                integrate (ForceShader.comp:145-150).
@@ -115,11 +117,12 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
 }
 
 // kDeep: 30-bit keys of the 32768^2 grid (15-bit leaves) instead of 20-bit cells.
-// One sort tile (kR x kSortThreads particles, kR per thread) per workgroup.
+// One block of kR x kSortThreads particles (kR per thread) per workgroup.
 // The digits of every radix pass are counted into digit_total, this build's
-// half of the totals (zeroed by the previous build's keys kernel, or at
-// allocation), while digit_next, the other half, is zeroed for the next build;
-// the first radix pass's look-back words and counters are zeroed as well.
+// half of the totals (kDigitCopies copies of 1024 words, workgroup b adding
+// into copy b mod kDigitCopies; zeroed by the previous build's keys kernel,
+// or at allocation), while digit_next, the other half, is zeroed for the next
+// build; the radix passes' look-back words and counters are zeroed as well.
 template <bool kBox, bool kDeep, int kR>
 __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
@@ -127,17 +130,19 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
     uint32_t *__restrict__ digit_total, uint32_t *__restrict__ digit_next,
     uint32_t *__restrict__ status, uint32_t status_words, BhCounters *__restrict__ ctr,
-    const float *__restrict__ box) {
+    const float *__restrict__ box, float4 *__restrict__ rec) {
     constexpr int kPasses = kDeep ? 4 : 3;
     __shared__ uint32_t h[kPasses][256];
     const uint32_t t = threadIdx.x;
     const uint32_t g = blockIdx.x * kSortThreads + t, stride = gridDim.x * kSortThreads;
     for (uint32_t j = g; j < 2 * kTiles; j += stride) tile_range[j] = 0;
-    for (uint32_t j = g; j < 4 * 256; j += stride) digit_next[j] = 0;
+    for (uint32_t j = g; j < kDigitCopies * 1024; j += stride) digit_next[j] = 0;
     for (uint32_t j = g; j < status_words; j += stride) status[j] = 0;
     if (g == 0) {
         ctr->next_group = 0;
         ctr->sort_stall = 0;
+        ctr->tiles_busy = 0;
+        ctr->tiles_done = 0;
     }
     if (g < 4) ctr->sort_tile[g] = 0;
 #pragma unroll
@@ -145,10 +150,12 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     __syncthreads();
     const uint32_t base = blockIdx.x * (kR * kSortThreads);
     float2 pr[kR];  // all loads first: the kill below stores to pos
+    float mr[kR];
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
         pr[r] = i < na ? pos[i] : make_float2(0.0f, 0.0f);
+        mr[r] = i < na ? mass[i] : 0.0f;
     }
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
@@ -170,20 +177,26 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
         }
         keys[i] = c;
         idx[i] = i;
+        // the particle record for emit's gather: one 16-byte read there
+        // instead of two (a killed particle's record is never read)
+        rec[i] = make_float4(p.x, p.y, mr[r], 0.0f);
         hist_add<kPasses, false>(h, c, true);
     }
     __syncthreads();
+    uint32_t *const copy = digit_total + (blockIdx.x % kDigitCopies) * 1024;
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) {
         const uint32_t n = h[p][t];
-        if (n) atomicAdd(&digit_total[p * 256 + t], n);
+        if (n) atomicAdd(&copy[p * 256 + t], n);
     }
 }
 
 // ---- 2. stable LSD radix sort, one sweep per pass (radix_device.hpp) -----------
 
-// One pass, one tile per workgroup: digit (key >> shift) & 255.  digit_total:
-// this pass's 256 totals; status: this pass's [tiles][256] words, zeroed;
+// One pass, one tile per workgroup: digit (key >> shift) & 255, with the
+// two-level look-back (grouped_prefix).  digit_total: this pass's 256 totals,
+// split over kDigitCopies copies 1024 words apart; status: this pass's
+// [tiles][256] tile words followed by its [groups][256] group words, zeroed;
 // tile_ctr: zeroed.  Tiles of kR x kSortThreads keys.
 template <int kR>
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
@@ -194,20 +207,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
     __shared__ SortShared<false, kR> sh;
     if (threadIdx.x == 0) sh.tile = atomicAdd(tile_ctr, 1u);
     sort_tile_clear(sh);
-    const uint32_t gbase = digit_base(digit_total, sh);  // synchronises
-    sort_tile<false>(keys_in, idx_in, keys_out, idx_out, n, shift, sh.tile, gbase, status, stall, sh);
-}
-
-// Exclusive scan of one array per workgroup (blockIdx.x selects a, b or c),
-// in place.
-__global__ __launch_bounds__(1024) void scan_exclusive_kernel(uint32_t *__restrict__ a,
-                                                               uint32_t *__restrict__ b,
-                                                               uint32_t *__restrict__ c,
-                                                               uint32_t count) {
-    __shared__ uint32_t wsum[16];
-    uint32_t *data = blockIdx.x == 0 ? a : blockIdx.x == 1 ? b : c;
-    if (!data) return;
-    scan_inplace_1024(data, count, wsum);
+    const uint32_t gbase = digit_base(digit_total, sh, kDigitCopies, 1024u);  // synchronises
+    sort_tile<false, kR, true>(keys_in, idx_in, keys_out, idx_out, n, shift, sh.tile, gbase,
+                               status, stall, sh, gridDim.x, status + (size_t)gridDim.x * 256);
 }
 
 // ---- 3./4. slot scan and emission --------------------------------------------
@@ -268,9 +270,12 @@ __global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict
     }
 }
 
+// block_cells, block_leaves (and block_gran): bh_prep_kernel's per-block
+// counts; each block adds up those of the blocks before it (integer sums, a
+// few loads per thread) instead of a separate scan launch.
 __global__ __launch_bounds__(256) void bh_emit_kernel(
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ sidx,
-    const float2 *__restrict__ pos, const float *__restrict__ mass, uint32_t na, int bits,
+    const float4 *__restrict__ rec, uint32_t na, int bits,
     const uint32_t *__restrict__ block_cells, const uint32_t *__restrict__ block_leaves,
     int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
     uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
@@ -278,27 +283,56 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     uint32_t *__restrict__ sorted_src, uint32_t *__restrict__ breaker,
     BhCounters *__restrict__ ctr, const uint32_t *__restrict__ block_gran,
     uint32_t *__restrict__ gran_first, uint32_t *__restrict__ gran_key) {
-    __shared__ uint32_t wa[4], wb[4], wc[4];
+    __shared__ uint32_t wa[4], wb[4], wc[4], pa[4], pb[4], pc[4];
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    // the particle's gather does not depend on the scan below: start it first
+    // the particle's gather does not depend on the scans below: start it first
     const uint32_t i = k < na ? sidx[k] : 0u;
-    const float2 p = k < na ? pos[i] : make_float2(0.0f, 0.0f);
-    const float m = k < na ? mass[i] : 0.0f;
+    const float4 r4 = k < na ? rec[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const float2 p = make_float2(r4.x, r4.y);
+    const float m = r4.z;
+    // the counts of the blocks before this one, eight loads per thread at once
+    uint32_t xa = 0, xb = 0, xc = 0;
+    for (uint32_t j0 = threadIdx.x; j0 < blockIdx.x; j0 += 8 * 256) {
+        uint32_t va[8], vb[8], vc[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t j = j0 + 256 * u;
+            const bool in = j < blockIdx.x;
+            va[u] = in ? block_cells[j] : 0u;
+            vb[u] = in ? block_leaves[j] : 0u;
+            vc[u] = in && block_gran ? block_gran[j] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            xa += va[u];
+            xb += vb[u];
+            xc += vc[u];
+        }
+    }
     const PosInfo q = pos_info(keys, na, k, bits);
     uint32_t a = q.ns, b = q.lf;
     wave_scan2(a, b);
     const unsigned long long gb = __ballot(q.g10);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        xa += __shfl_xor(xa, off, 64);
+        xb += __shfl_xor(xb, off, 64);
+        xc += __shfl_xor(xc, off, 64);
+    }
     if (lane == 63) {
         wa[wave] = a;
         wb[wave] = b;
         wc[wave] = (uint32_t)__popcll(gb);
+        pa[wave] = xa;
+        pb[wave] = xb;
+        pc[wave] = xc;
     }
     __syncthreads();
-    uint32_t ea = a - q.ns + block_cells[blockIdx.x];
-    uint32_t eb = b - q.lf + block_leaves[blockIdx.x];
+    uint32_t ea = a - q.ns + ((pa[0] + pa[1]) + (pa[2] + pa[3]));
+    uint32_t eb = b - q.lf + ((pb[0] + pb[1]) + (pb[2] + pb[3]));
     uint32_t ec = (uint32_t)__popcll(gb & ((1ull << lane) - 1ull)) +
-                  (block_gran ? block_gran[blockIdx.x] : 0u);
+                  ((pc[0] + pc[1]) + (pc[2] + pc[3]));
     for (uint32_t w = 0; w < wave; ++w) {
         ea += wa[w];
         eb += wb[w];
@@ -381,7 +415,7 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
     const BhCounters *__restrict__ ctr, float4 *__restrict__ leaf_sum,
     uint32_t *__restrict__ tile_range, const int8_t *__restrict__ sdep,
     float4 *__restrict__ node4, uint32_t *__restrict__ ncount,
-    float4 *__restrict__ nsum, const float *__restrict__ box) {
+    float4 *__restrict__ nsum, const float *__restrict__ box, uint32_t *__restrict__ tiles_busy) {
     __shared__ float4 stage[4][64];
     const uint32_t n_in = ctr->n_in, nl = ctr->n_leaves, ncells = ctr->n_cells;
     const bool deep = nsum != nullptr;
@@ -389,6 +423,46 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
     const uint32_t waves = gridDim.x * 4;
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t wave_id = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w);
+    // the first particle's run length: its loads go out first, its dependent
+    // load (breaker) once the leaves are summed
+    const uint32_t k0 = blockIdx.x * 256 + threadIdx.x;
+    const bool has0 = k0 < n_in;
+    const uint32_t kk0 = has0 ? k0 : 0u;
+    const uint32_t key0 = keys[kk0], cpos0 = cpos[kk0];
+    const uint32_t keyp = keys[kk0 ? kk0 - 1 : 0], keyn = keys[kk0 + 1 < na ? kk0 + 1 : kk0];
+    // larger leaves: leaf L goes to wave (waves - 1 - L) mod waves, so the
+    // largest ones (dense cores, low L when few) avoid the first waves, which
+    // hold the small leaves; neighbouring dense leaves go to different waves;
+    // each is streamed by the whole wave; the 10-bit leaf's cell node is
+    // written here (15-bit leaves: its depth-15 cell)
+    const uint32_t rw = waves - 1u - wave_id;
+    for (uint32_t i0 = 0; rw + i0 * waves < nl; i0 += 64) {
+        const uint32_t L = rw + (i0 + lane) * waves;
+        uint32_t f = 0, e = 0;
+        if (L < nl) {
+            e = min(leaf_first[L + 1], n_in);
+            f = min(leaf_first[L], e);
+        }
+        unsigned long long big = __ballot(L < nl && e - f > kSmallLeaf);
+        while (big) {
+            const uint32_t j = (uint32_t)__builtin_ctzll(big);
+            big &= big - 1;
+            const uint32_t bf = __builtin_amdgcn_readfirstlane(__shfl(f, j, 64));
+            const uint32_t be = __builtin_amdgcn_readfirstlane(__shfl(e, j, 64));
+            const uint32_t slot = deep ? 0u : cell_slot(bf, (int)kTreeDepth, sdep, cpos);
+            const float4 r = wave_leaf_sum(stage[w], prod, bf, be, lane);
+            if (lane == 0) {
+                const uint32_t nb = be - bf;
+                if (deep) {
+                    emit_deep_leaf(r, bf, nb, sdep, cpos, node4, ncount, nsum, r0);
+                } else {
+                    leaf_sum[rw + (i0 + j) * waves] = make_float4(r.x, r.y, r.z, __uint_as_float(nb));
+                    emit_cell_at(TileCell{nb, nb + 1u, bf, 0u, r.x, r.y, r.z, 0.0f},
+                                 (int)kTreeDepth, slot, node4, ncount, r0);
+                }
+            }
+        }
+    }
     // small leaves: each wave takes 64 consecutive leaves, one per lane
     for (uint32_t base = wave_id * 64; base < nl; base += waves * 64) {
         const uint32_t L = base + lane;
@@ -401,38 +475,36 @@ __global__ __launch_bounds__(256) void bh_leaf_kernel(
         const bool small = valid && e - f <= kSmallLeaf;
         if (small) {
             const float4 r = lane_leaf_sum(prod, f, e);
-            if (deep) emit_deep_leaf(r, f, e - f, sdep, cpos, node4, ncount, nsum, r0);
-            else leaf_sum[L] = make_float4(r.x, r.y, r.z, __uint_as_float(e - f));
+            const uint32_t nb = e - f;
+            if (deep) {
+                emit_deep_leaf(r, f, nb, sdep, cpos, node4, ncount, nsum, r0);
+            } else {
+                leaf_sum[L] = make_float4(r.x, r.y, r.z, __uint_as_float(nb));
+                emit_cell(TileCell{nb, nb + (nb >= 2u ? 1u : 0u), f, 0u, r.x, r.y, r.z, 0.0f},
+                          (int)kTreeDepth, sdep, cpos, node4, ncount, r0);
+            }
         }
         if (valid && !deep) {
             const uint32_t T = leaf_key[L] >> 10;
-            if (L == 0 || (leaf_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
+            if (L == 0 || (leaf_key[L - 1] >> 10) != T) {
+                tile_range[2 * T] = L;
+                atomicAdd(tiles_busy, 1u);
+            }
             if (L + 1 == nl || (leaf_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
         }
     }
-    // larger leaves: leaf L goes to wave L mod waves (neighbouring dense
-    // leaves to different waves), streamed by the whole wave
-    for (uint32_t i0 = 0; wave_id + i0 * waves < nl; i0 += 64) {
-        const uint32_t L = wave_id + (i0 + lane) * waves;
-        uint32_t f = 0, e = 0;
-        if (L < nl) {
-            e = min(leaf_first[L + 1], n_in);
-            f = min(leaf_first[L], e);
-        }
-        unsigned long long big = __ballot(L < nl && e - f > kSmallLeaf);
-        while (big) {
-            const uint32_t j = (uint32_t)__builtin_ctzll(big);
-            big &= big - 1;
-            const uint32_t bf = __builtin_amdgcn_readfirstlane(__shfl(f, j, 64));
-            const uint32_t be = __builtin_amdgcn_readfirstlane(__shfl(e, j, 64));
-            const float4 r = wave_leaf_sum(stage[w], prod, bf, be, lane);
-            if (lane == 0) {
-                if (deep) emit_deep_leaf(r, bf, be - bf, sdep, cpos, node4, ncount, nsum, r0);
-                else leaf_sum[wave_id + (i0 + j) * waves] = make_float4(r.x, r.y, r.z, __uint_as_float(be - bf));
-            }
-        }
+    uint32_t slot0 = 0, end0 = 0;
+    if (has0) {
+        // pos_info(keys, na, k0, bits) from the loads above
+        const uint32_t killed = 1u << (2 * bits);
+        const int sk = k0 == 0 ? -1 : shared_depth_bits(keyp, key0, bits);
+        const int sk1 = k0 + 1 < na && keyn < killed ? shared_depth_bits(key0, keyn, bits) : -1;
+        const uint32_t c1 = cpos0 + (sk1 > sk ? (uint32_t)(sk1 - sk) : 0u);  // C(k+1)
+        slot0 = k0 + c1;
+        end0 = c1 < ncells ? breaker[c1] : n_in;
     }
-    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < n_in; k += gridDim.x * 256) {
+    if (k0 < n_in) set_run_length(node4, slot0, end0 > k0 ? end0 - k0 : 1u);
+    for (uint32_t k = k0 + gridDim.x * 256; k < n_in; k += gridDim.x * 256) {
         const PosInfo q = pos_info(keys, na, k, bits);
         const uint32_t c1 = cpos[k] + q.ns;  // C(k+1)
         const uint32_t run_end = c1 < ncells ? breaker[c1] : n_in;
@@ -535,7 +607,8 @@ __global__ __launch_bounds__(256) void bh_gran_kernel(
     const BhCounters *__restrict__ ctr, const uint32_t *__restrict__ ncount,
     const float4 *__restrict__ nsum, const uint32_t *__restrict__ gran_first,
     const uint32_t *__restrict__ gran_key, float4 *__restrict__ gran_sum,
-    uint32_t *__restrict__ gran_y, uint32_t *__restrict__ tile_range) {
+    uint32_t *__restrict__ gran_y, uint32_t *__restrict__ tile_range,
+    uint32_t *__restrict__ tiles_busy) {
     const uint32_t n_in = ctr->n_in, n_nodes = ctr->n_nodes, ng = ctr->n_gran;
     for (uint32_t L = blockIdx.x * 256 + threadIdx.x; L < ng; L += gridDim.x * 256) {
         const uint32_t k = gran_first[L];
@@ -552,7 +625,10 @@ __global__ __launch_bounds__(256) void bh_gran_kernel(
             gran_y[L] = 1u;
         }
         const uint32_t T = gran_key[L] >> 10;
-        if (L == 0 || (gran_key[L - 1] >> 10) != T) tile_range[2 * T] = L;
+        if (L == 0 || (gran_key[L - 1] >> 10) != T) {
+            tile_range[2 * T] = L;
+            atomicAdd(tiles_busy, 1u);
+        }
         if (L + 1 == ng || (gran_key[L + 1] >> 10) != T) tile_range[2 * T + 1] = L + 1;
     }
 }
@@ -565,34 +641,44 @@ __device__ __forceinline__ int tile_level_offset(int d) {
     return d == 9 ? 0 : d == 8 ? 256 : d == 7 ? 320 : d == 6 ? 336 : 340;
 }
 
+// The last occupied tile's workgroup to finish (a device-scope counter,
+// zeroed by the keys kernel, against the number of occupied tiles the leaf
+// pass counted) also reduces the 1024 depth-5 cells to the root and writes
+// depths 4..0, so the top needs no launch of its own.  Only occupied tiles
+// write their record to `top` (fenced before they count); empty tiles return
+// at once, and the last workgroup takes their empty record from tile_range.
 __global__ __launch_bounds__(256) void bh_tile_kernel(
     const uint32_t *__restrict__ tile_range, const uint32_t *__restrict__ leaf_first,
     const uint32_t *__restrict__ leaf_key, const float4 *__restrict__ leaf_sum,
     const int8_t *__restrict__ sdep, const uint32_t *__restrict__ cpos,
     float4 *__restrict__ node4, uint32_t *__restrict__ ncount, TileCell *__restrict__ top,
-    const float *__restrict__ box, const uint32_t *__restrict__ leaf_y) {
+    const float *__restrict__ box, const uint32_t *__restrict__ leaf_y,
+    BhCounters *__restrict__ ctr) {
     // leaf_y (15-bit leaves): the depth-10 records are granules whose subtree
     // size is given; otherwise leaves of n particles, y = n + [n >= 2]
     __shared__ TileCell l10[1024];
     __shared__ TileCell up[341];
+    __shared__ uint32_t last;
     const uint32_t T = blockIdx.x, t = threadIdx.x;
     const uint32_t lb = tile_range[2 * T], le = tile_range[2 * T + 1];
-    if (lb == le) {
-        if (t == 0) top[T] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
-        return;
-    }
+    if (lb == le) return;  // the whole workgroup: an empty tile
+    const float r0 = box ? box[3] : kFixedR0;
     for (uint32_t c = t; c < 1024; c += 256)
         l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
     __syncthreads();
-    const float r0 = box ? box[3] : kFixedR0;
-    for (uint32_t L = lb + t; L < le; L += 256) {
+    // a tile holds at most 1024 leaves (granules): four per thread, loaded
+    // together; a 10-bit leaf's own cell node was written with its sum
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t L = lb + t + 256 * u;
+        if (L >= le) break;
         const uint32_t c = leaf_key[L] & 1023u;
         const float4 s = leaf_sum[L];
         const uint32_t n = __float_as_uint(s.w);
         const uint32_t y = leaf_y ? leaf_y[L] : n + (n >= 2u ? 1u : 0u);
         TileCell cell{n, y, leaf_first[L], 0u, s.x, s.y, s.z, 0.0f};
         l10[c] = cell;
-        emit_cell(cell, 10, sdep, cpos, node4, ncount, r0);
+        if (leaf_y) emit_cell(cell, 10, sdep, cpos, node4, ncount, r0);
     }
     __syncthreads();
     {
@@ -612,38 +698,36 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
         const int d = c < 256 ? 9 : c < 320 ? 8 : c < 336 ? 7 : c < 340 ? 6 : 5;
         emit_cell(up[c], d, sdep, cpos, node4, ncount, r0);
     }
-    if (t == 0) top[T] = up[340];
-}
-
-// Level offsets inside the top workgroup's LDS: depth 5 at 0 (1024 cells),
-// 4 at 1024 (256), 3 at 1280 (64), 2 at 1344 (16), 1 at 1360 (4), 0 at 1364.
-__device__ __forceinline__ int top_level_offset(int d) {
-    return d == 5 ? 0 : d == 4 ? 1024 : d == 3 ? 1280 : d == 2 ? 1344 : d == 1 ? 1360 : 1364;
-}
-
-__global__ __launch_bounds__(1024) void bh_top_kernel(const TileCell *__restrict__ top,
-                                                      const int8_t *__restrict__ sdep,
-                                                      const uint32_t *__restrict__ cpos,
-                                                      float4 *__restrict__ node4,
-                                                      uint32_t *__restrict__ ncount,
-                                                      const float *__restrict__ box) {
-    __shared__ TileCell lv[1365];
-    const float r0 = box ? box[3] : kFixedR0;
-    const uint32_t t = threadIdx.x;
-    lv[t] = top[t];
+    if (t == 0) {
+        top[T] = up[340];
+        __threadfence();  // top[T] before the count (release)
+        last = atomicAdd(&ctr->tiles_done, 1u) + 1u == ctr->tiles_busy ? 1u : 0u;
+    }
     __syncthreads();
-    for (int d = 4; d >= 0; --d) {
+    if (!last) return;
+    __threadfence();  // every tile's top[] after the count (acquire)
+    // depths 4..0 in `up` at the offsets of depths 9..5
+    TileCell c4[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t q = 4 * t + u;
+        c4[u] = tile_range[2 * q] != tile_range[2 * q + 1]
+                    ? top[q]
+                    : TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    up[t] = reduce4(c4[0], c4[1], c4[2], c4[3]);
+    __syncthreads();
+    for (int d = 3; d >= 0; --d) {
         const uint32_t cnt = 1u << (2 * d);
-        const int o = top_level_offset(d), oc = top_level_offset(d + 1);
+        const int o = tile_level_offset(d + 5), oc = tile_level_offset(d + 6);
         if (t < cnt)
-            lv[o + t] = reduce4(lv[oc + 4 * t], lv[oc + 4 * t + 1], lv[oc + 4 * t + 2],
-                                lv[oc + 4 * t + 3]);
+            up[o + t] = reduce4(up[oc + 4 * t], up[oc + 4 * t + 1], up[oc + 4 * t + 2],
+                                up[oc + 4 * t + 3]);
         __syncthreads();
     }
-    if (t < 341) {
-        const uint32_t c = 1024 + t;
-        const int d = c < 1280 ? 4 : c < 1344 ? 3 : c < 1360 ? 2 : c < 1364 ? 1 : 0;
-        emit_cell(lv[c], d, sdep, cpos, node4, ncount, r0);
+    for (uint32_t c = t; c < 341; c += 256) {
+        const int d = c < 256 ? 4 : c < 320 ? 3 : c < 336 ? 2 : c < 340 ? 1 : 0;
+        emit_cell(up[c], d, sdep, cpos, node4, ncount, r0);
     }
 }
 
@@ -1204,14 +1288,16 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
         GSIM_ALLOC(b->keys[k], na);
         GSIM_ALLOC(b->idx[k], na);
     }
-    GSIM_ALLOC(b->status, 4 * 256 * b->sort_blocks);
-    GSIM_ALLOC(b->digit_total, 2 * 4 * 256);
+    // four passes of tile words and group words
+    GSIM_ALLOC(b->status, 4 * 256 * (b->sort_blocks + (b->sort_blocks + kLookGroup - 1) / kLookGroup));
+    GSIM_ALLOC(b->digit_total, 2 * kDigitCopies * 1024);
     GSIM_ALLOC(b->breaker, b->cap);
     GSIM_ALLOC(b->block_cells, b->prep_blocks);
     GSIM_ALLOC(b->block_leaves, b->prep_blocks);
     GSIM_ALLOC(b->sdep, na + 1);
     GSIM_ALLOC(b->cpos, na);
     GSIM_ALLOC(b->prod, na);
+    GSIM_ALLOC(b->rec, na);
     GSIM_ALLOC(b->leaf_first, na + 1);
     GSIM_ALLOC(b->leaf_key, na);
     GSIM_ALLOC(b->leaf_sum, na);
@@ -1239,7 +1325,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->gpre, na / 64 + 1);
 #undef GSIM_ALLOC
     if (e == hipSuccess) e = hipMemset(b->ctr, 0, sizeof(BhCounters));
-    if (e == hipSuccess) e = hipMemset(b->digit_total, 0, sizeof(uint32_t) * 2 * 4 * 256);
+    if (e == hipSuccess) e = hipMemset(b->digit_total, 0, sizeof(uint32_t) * 2 * kDigitCopies * 1024);
     if (e == hipSuccess) e = hipMemset(b->gcost, 0, sizeof(uint32_t) * (na / 64));
     if (e == hipSuccess) e = hipMemset(b->gchunk, 0, sizeof(uint32_t) * (na / 64 + 1));
     if (e == hipSuccess) e = hipMemset(b->gvis, 0, sizeof(unsigned long long) * (na / 32 + 1));
@@ -1254,7 +1340,7 @@ void bh_free(BhBuffers *b) {
         (void)hipFree(b->keys[k]);
         (void)hipFree(b->idx[k]);
     }
-    void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
+    void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod, b->rec,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
                     b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->plan, b->heavy, b->gpre, b->gvis, b->gacc, b->box,
@@ -1277,52 +1363,40 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
         bh_box_final_kernel<<<1, 256, 0, s->stream>>>(b.box_part, nparts, b.box);
         box = b.box;
     }
+    // one tile of kSortTile keys per workgroup, in the keys kernel and in
+    // every pass; each pass's status holds its tile words, then its group words
+    const uint32_t nblk = (na + kSortTile - 1) / kSortTile;
+    const uint32_t kblk = nblk;
+    const uint32_t pass_words = (nblk + (nblk + kLookGroup - 1) / kLookGroup) * 256u;
     const int passes = deep ? 4 : 3;
-    // one-sweep passes are fastest with about one tile per CU: the look-back
-    // chain grows with the tile count (500k: 245 tiles of 2048; 1M: 245 of
-    // 4096 beats 489 of 2048 by 14 us a build).  The buffers hold the tiles
-    // of the smaller size.
-    const bool big = (size_t)na > kSortBigTiles * kSortTile;
-    const uint32_t tile_keys = big ? 2u * kSortTile : (uint32_t)kSortTile;
-    const uint32_t nblk = (na + tile_keys - 1) / tile_keys;
-    const uint32_t status_words = (uint32_t)passes * nblk * 256u;
+    const uint32_t status_words = (uint32_t)passes * pass_words;
     // digit totals alternate between two halves by build parity
-    uint32_t *const totals = b.digit_total + 1024 * b.parity;
-    uint32_t *const totals_next = b.digit_total + 1024 * (b.parity ^ 1u);
+    uint32_t *const totals = b.digit_total + kDigitCopies * 1024 * b.parity;
+    uint32_t *const totals_next = b.digit_total + kDigitCopies * 1024 * (b.parity ^ 1u);
     b.parity ^= 1u;
-#define GSIM_KEYS(B, D, R)                                                                 \
-    bh_keys_kernel<B, D, R><<<nblk, kSortThreads, 0, s->stream>>>(                          \
-        pos, mass, pos_out, vel, vel_out, na, b.keys[0], b.idx[0], b.tile_range, totals,     \
-        totals_next, b.status, status_words, b.ctr, box)
-#define GSIM_KEYS_R(R)                                                                     \
-    if (box) { if (deep) GSIM_KEYS(true, true, R); else GSIM_KEYS(true, false, R); }         \
-    else { if (deep) GSIM_KEYS(false, true, R); else GSIM_KEYS(false, false, R); }
-    if (big) { GSIM_KEYS_R(2 * kSortRounds) } else { GSIM_KEYS_R(kSortRounds) }
-#undef GSIM_KEYS_R
+#define GSIM_KEYS(B, D)                                                                       \
+    bh_keys_kernel<B, D, kSortRounds><<<kblk, kSortThreads, 0, s->stream>>>(                   \
+        pos, mass, pos_out, vel, vel_out, na, b.keys[0], b.idx[0], b.tile_range, totals,        \
+        totals_next, b.status, status_words, b.ctr, box, b.rec)
+    if (box) { if (deep) GSIM_KEYS(true, true); else GSIM_KEYS(true, false); }
+    else { if (deep) GSIM_KEYS(false, true); else GSIM_KEYS(false, false); }
 #undef GSIM_KEYS
     // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
     // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
     const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
     for (int pass = 0; pass < passes; ++pass) {
         const int i = in_of[pass], o = out_of[pass];
-        uint32_t *const st = b.status + (size_t)pass * nblk * 256;
-        if (big)
-            radix_onesweep_kernel<2 * kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
-                b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
-                &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
-        else
-            radix_onesweep_kernel<kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
-                b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
-                &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
+        uint32_t *const st = b.status + (size_t)pass * pass_words;
+        radix_onesweep_kernel<kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
+            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
+            &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
     }
     b.sorted_slot = deep ? 2 : 1;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
     const uint32_t pb = (uint32_t)b.prep_blocks;
     bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, bits, b.block_cells, b.block_leaves,
                                               deep ? b.block_gran : nullptr);
-    scan_exclusive_kernel<<<deep ? 3 : 2, 1024, 0, s->stream>>>(b.block_cells, b.block_leaves,
-                                                                b.block_gran, pb);
-    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, na, bits, b.block_cells,
+    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, b.rec, na, bits, b.block_cells,
                                               b.block_leaves, b.sdep, b.cpos, b.prod,
                                               b.leaf_first, b.leaf_key, b.node4, b.ncount,
                                               b.sorted_src, b.breaker, b.ctr,
@@ -1332,7 +1406,7 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     bh_leaf_kernel<<<lb, 256, 0, s->stream>>>(b.leaf_first, b.leaf_key, b.prod, sk, na, bits,
                                               b.cpos, b.breaker, b.ctr, b.leaf_sum, b.tile_range,
                                               b.sdep, b.node4, b.ncount,
-                                              deep ? b.nsum : nullptr, box);
+                                              deep ? b.nsum : nullptr, box, &b.ctr->tiles_busy);
     if (deep) {
         // sparse levels below depth 10, then depth-10 records for the dense path
         for (int d = (int)kDeepBits - 1; d > (int)kTreeDepth; --d)
@@ -1340,16 +1414,15 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
                                                        b.ncount, b.nsum, d, box);
         bh_gran_kernel<<<pb, 256, 0, s->stream>>>(sk, b.sdep, b.cpos, b.prod, b.ctr, b.ncount,
                                                   b.nsum, b.gran_first, b.gran_key, b.leaf_sum,
-                                                  b.gran_y, b.tile_range);
+                                                  b.gran_y, b.tile_range, &b.ctr->tiles_busy);
         bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.gran_first, b.gran_key,
                                                       b.leaf_sum, b.sdep, b.cpos, b.node4,
-                                                      b.ncount, b.top, box, b.gran_y);
+                                                      b.ncount, b.top, box, b.gran_y, b.ctr);
     } else {
         bh_tile_kernel<<<kTiles, 256, 0, s->stream>>>(b.tile_range, b.leaf_first, b.leaf_key,
                                                       b.leaf_sum, b.sdep, b.cpos, b.node4,
-                                                      b.ncount, b.top, box, nullptr);
+                                                      b.ncount, b.top, box, nullptr, b.ctr);
     }
-    bh_top_kernel<<<1, 1024, 0, s->stream>>>(b.top, b.sdep, b.cpos, b.node4, b.ncount, box);
     return hipGetLastError();
 }
 

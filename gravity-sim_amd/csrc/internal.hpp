@@ -6,7 +6,8 @@
 //                   symmetric: row partials float2[tiles*16*64] then column
 //                   partials float2[(sum over groups of B - 16g)*64]
 //   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
-//                   cpos u32[Na], prod float4[Na], leaf records, tile results,
+//                   cpos u32[Na], prod float4[Na], rec float4[Na], leaf records,
+//                   tile results,
 //                   breaker u32[cap], node4 float4[cap] {x, y, mass, radius}
 //                   (a particle node's radius word holds minus its run length),
 //                   ncount u32[cap],
@@ -46,6 +47,8 @@ struct BhCounters {
     uint32_t sort_stall;         // look-back waits that hit their bound (never expected)
     uint32_t n_heavy;            // static traversal: groups on the heavy list this step
     uint32_t sort_tile[4];       // tile counter of each radix pass
+    uint32_t tiles_busy;         // occupied depth-5 tiles
+    uint32_t tiles_done;         // occupied tiles reduced (the last one reduces the top)
 };
 
 // Radix sort passes (single and batched builds): tiles of kSortTile keys,
@@ -54,9 +57,10 @@ struct BhCounters {
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 8;
 constexpr int kSortTile = kSortThreads * kSortRounds;
-// The single-system build sorts tiles of 2 x kSortTile keys when tiles of
-// kSortTile would number more than this.
-constexpr size_t kSortBigTiles = 384;
+// The single-system build counts every pass's digits into kDigitCopies
+// copies of the totals (workgroup b into copy b mod kDigitCopies), so that
+// fewer workgroups add into each word; a pass sums the copies.
+constexpr uint32_t kDigitCopies = 8;
 
 // Particle runs a walking wave queues in its LDS before summing them (the
 // staged walk's run stack; the batch may use fewer, BatchTune::run_queue).
@@ -79,7 +83,7 @@ struct BhBuffers {
     uint32_t bits = 10;              // leaf grid bits per axis the buffers are sized for
     uint32_t *keys[3] = {}, *idx[3] = {};
     uint32_t *status = nullptr;      // [4 * sort_blocks * 256] look-back words per pass
-    uint32_t *digit_total = nullptr; // [2][4 * 256] per-pass digit totals, two halves
+    uint32_t *digit_total = nullptr; // [2][kDigitCopies][4 * 256] per-pass digit totals, two halves
     uint32_t parity = 0;             // the half the next build counts into
     uint32_t *breaker = nullptr;     // [cap] sorted position where cell node #o starts
     uint32_t *block_cells = nullptr; // [prep_blocks] then scanned
@@ -87,6 +91,7 @@ struct BhBuffers {
     int8_t *sdep = nullptr;          // [Na] shared depth with the previous sorted key
     uint32_t *cpos = nullptr;        // [Na] cell nodes starting before sorted position k
     float4 *prod = nullptr;          // [Na] (m*x, m*y, m, -) in sorted order
+    float4 *rec = nullptr;           // [Na] (x, y, m, -) in particle order, for emit's gather
     uint32_t *leaf_first = nullptr;  // [Na+1]
     uint32_t *leaf_key = nullptr;    // [Na]
     float4 *leaf_sum = nullptr;      // [Na] (sx, sy, sm, n as bits)

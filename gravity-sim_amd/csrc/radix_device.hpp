@@ -119,11 +119,14 @@ __device__ __forceinline__ void look_back(const uint32_t *st, int j, uint32_t *e
 }
 
 // Exclusive prefix of this pass's digit totals for digit threadIdx.x
-// (synchronises the workgroup).
+// (synchronises the workgroup).  The totals may be split over `copies`
+// arrays `stride` words apart, whose sum they are.
 template <bool kStage, int kR>
 __device__ __forceinline__ uint32_t digit_base(const uint32_t *__restrict__ digit_total,
-                                               SortShared<kStage, kR> &sh) {
-    const uint32_t total = digit_total[threadIdx.x];
+                                               SortShared<kStage, kR> &sh, uint32_t copies = 1,
+                                               uint32_t stride = 0) {
+    uint32_t total = 0;
+    for (uint32_t c = 0; c < copies; ++c) total += digit_total[c * stride + threadIdx.x];
     uint32_t all = 0;
     return block_scan256(total, &all, sh.wsum) - total;
 }
@@ -136,11 +139,64 @@ __device__ __forceinline__ void sort_tile_clear(SortShared<kStage, kR> &sh) {
     for (int k = 0; k < SortShared<kStage, kR>::kRows; ++k) sh.slot[k * 256 + threadIdx.x] = 0;
 }
 
+// Two-level look-back (kLookGroup tiles to a group): the exclusive prefix of
+// this thread's digit before `tile` (c: the tile's own count) is the sum of
+// the counts of the tiles before it in its group, read at once, plus the
+// exclusive prefix of its group, found by decoupled look-back over the
+// groups' words.  Tile words (st[j * 256]) only ever hold kStA | count; the
+// last tile of each group publishes the group's sum and then its inclusive
+// prefix (gst[g * 256]).  Every wait is for a lower tile, so it ends; waits
+// are bounded as in look_back.  The chain of look-backs is the number of
+// groups long instead of the number of tiles.
+constexpr uint32_t kLookGroup = 16;
+
+__device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, uint32_t tile,
+                                                   uint32_t tiles, uint32_t c,
+                                                   uint32_t *__restrict__ stall) {
+    const uint32_t g = tile / kLookGroup, first = g * kLookGroup;
+    status_store(st + (size_t)tile * 256, kStA | c);
+    uint32_t v[kLookGroup - 1];
+#pragma unroll
+    for (uint32_t q = 0; q + 1 < kLookGroup; ++q)
+        v[q] = first + q < tile ? status_load(st + (size_t)(first + q) * 256) : kStA;
+    uint32_t inner = 0;
+#pragma unroll
+    for (uint32_t q = 0; q + 1 < kLookGroup; ++q) {
+        uint32_t x = v[q];
+#pragma unroll 1
+        for (uint32_t spin = 0; (x >> 30) == 0; ++spin) {
+            if (spin >= kSpinLimit) {
+                atomicAdd(stall, 1u);
+                x = kStA;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            x = status_load(st + (size_t)(first + q) * 256);
+        }
+        inner += x & kStVal;
+    }
+    const bool last = tile + 1 == first + kLookGroup || tile + 1 == tiles;
+    if (last) status_store(gst + (size_t)g * 256, (g ? kStA : kStP) | (inner + c));
+    uint32_t gex = 0;
+    if (g) {
+        int j = (int)g - 1;
+        bool done = false;
+        while (!done) {
+            look_back<kLookWindow>(gst, j, &gex, &done, stall);
+            j -= kLookWindow;
+        }
+        if (last) status_store(gst + (size_t)g * 256, kStP | (gex + inner + c));
+    }
+    return gex + inner;
+}
+
 // Sorts tile `tile` of keys_in/idx_in on digit (key >> shift) & 255 into
 // keys_out/idx_out.  gbase: digit_base() of this thread's digit; status: this
 // pass's [tiles][256] words; sh.slot cleared by sort_tile_clear() before a
 // barrier.  Ends with sh in use: a barrier must precede its next use.
-template <bool kStage, int kR>
+// kGrouped: grouped_prefix instead of the tile-by-tile look-back (tiles: the
+// pass's tile count; gstatus: its [groups][256] words, zeroed).
+template <bool kStage, int kR, bool kGrouped = false>
 __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           const uint32_t *__restrict__ idx_in,
                                           uint32_t *__restrict__ keys_out,
@@ -148,7 +204,8 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           uint32_t shift, uint32_t tile, uint32_t gbase,
                                           uint32_t *__restrict__ status,
                                           uint32_t *__restrict__ stall,
-                                          SortShared<kStage, kR> &sh) {
+                                          SortShared<kStage, kR> &sh, uint32_t tiles = 0,
+                                          uint32_t *__restrict__ gstatus = nullptr) {
     constexpr int kRows = SortShared<kStage, kR>::kRows;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint32_t base = tile * SortShared<kStage, kR>::kTile;
@@ -182,16 +239,20 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
         sh.first[t] = first;
     }
     uint32_t *const st = status + t;  // this digit's word of tile j: st[j * 256]
-    status_store(st + (size_t)tile * 256, (tile ? kStA : kStP) | c);
     uint32_t excl = 0;
-    if (tile) {
-        int j = (int)tile - 1;
-        bool done = false;
-        while (!done) {
-            look_back<kLookWindow>(st, j, &excl, &done, stall);
-            j -= kLookWindow;
+    if (kGrouped) {
+        excl = grouped_prefix(st, gstatus + t, tile, tiles, c, stall);
+    } else {
+        status_store(st + (size_t)tile * 256, (tile ? kStA : kStP) | c);
+        if (tile) {
+            int j = (int)tile - 1;
+            bool done = false;
+            while (!done) {
+                look_back<kLookWindow>(st, j, &excl, &done, stall);
+                j -= kLookWindow;
+            }
+            status_store(st + (size_t)tile * 256, kStP | (excl + c));
         }
-        status_store(st + (size_t)tile * 256, kStP | (excl + c));
     }
     sh.adj[t] = gbase + excl - first;
     __syncthreads();
