@@ -122,7 +122,7 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
 // half of the totals (kDigitCopies copies of 1024 words, workgroup b adding
 // into copy b mod kDigitCopies; zeroed by the previous build's keys kernel,
 // or at allocation), while digit_next, the other half, is zeroed for the next
-// build; the radix passes' look-back words and counters are zeroed as well.
+// build; the radix passes' look-back words are zeroed as well.
 template <bool kBox, bool kDeep, int kR>
 __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
@@ -144,7 +144,6 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
         ctr->tiles_busy = 0;
         ctr->tiles_done = 0;
     }
-    if (g < 4) ctr->sort_tile[g] = 0;
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
     __syncthreads();
@@ -197,19 +196,21 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
 // two-level look-back (grouped_prefix).  digit_total: this pass's 256 totals,
 // split over kDigitCopies copies 1024 words apart; status: this pass's
 // [tiles][256] tile words followed by its [groups][256] group words, zeroed;
-// tile_ctr: zeroed.  Tiles of kR x kSortThreads keys.
+// Tiles of kR x kSortThreads keys.
 template <int kR>
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
     uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
     uint32_t shift, const uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status,
-    uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ stall) {
+    uint32_t *__restrict__ stall) {
     __shared__ SortShared<false, kR> sh;
-    if (threadIdx.x == 0) sh.tile = atomicAdd(tile_ctr, 1u);
-    sort_tile_clear(sh);
-    const uint32_t gbase = digit_base(digit_total, sh, kDigitCopies, 1024u);  // synchronises
-    sort_tile<false, kR, true>(keys_in, idx_in, keys_out, idx_out, n, shift, sh.tile, gbase,
-                               status, stall, sh, gridDim.x, status + (size_t)gridDim.x * 256);
+    sort_tile_clear(sh);  // the tile's first barrier follows its ranking
+    // Tile = workgroup index.  A tile only waits for lower tiles, which the
+    // dispatcher starts first, and a pass's workgroups all fit on the GPU at
+    // once, so every wait ends (and each is bounded besides).
+    sort_tile<false, kR, true>(keys_in, idx_in, keys_out, idx_out, n, shift, blockIdx.x, 0u,
+                               status, stall, sh, gridDim.x, status + (size_t)gridDim.x * 256,
+                               digit_total, kDigitCopies, 1024u);
 }
 
 // ---- 3./4. slot scan and emission --------------------------------------------
@@ -1389,7 +1390,7 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
         uint32_t *const st = b.status + (size_t)pass * pass_words;
         radix_onesweep_kernel<kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
             b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
-            &b.ctr->sort_tile[pass], &b.ctr->sort_stall);
+            &b.ctr->sort_stall);
     }
     b.sorted_slot = deep ? 2 : 1;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];

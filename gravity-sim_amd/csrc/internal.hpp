@@ -46,7 +46,6 @@ struct BhCounters {
     uint32_t n_gran;             // 15-bit leaves: occupied depth-10 cells
     uint32_t sort_stall;         // look-back waits that hit their bound (never expected)
     uint32_t n_heavy;            // static traversal: groups on the heavy list this step
-    uint32_t sort_tile[4];       // tile counter of each radix pass
     uint32_t tiles_busy;         // occupied depth-5 tiles
     uint32_t tiles_done;         // occupied tiles reduced (the last one reduces the top)
 };

@@ -91,30 +91,45 @@ __device__ __forceinline__ void hist_add(uint32_t (*h)[256], uint32_t key, bool 
 
 // Adds predecessors j, j-1, ..., j-kW+1 of this digit's chain to *excl until
 // one holds an inclusive prefix (sets *done); predecessors below tile 0 count
-// as an empty prefix.
+// as an empty prefix.  The window is read at once; while a word it needs is
+// not yet published, the whole window is read again (a word only goes from 0
+// to a count to a prefix, and any published form of it is usable), so
+// several late words cost one wait, not one each.
 template <int kW>
 __device__ __forceinline__ void look_back(const uint32_t *st, int j, uint32_t *excl, bool *done,
                                           uint32_t *stall) {
     uint32_t v[kW];
-#pragma unroll
-    for (int q = 0; q < kW; ++q) v[q] = j - q >= 0 ? status_load(st + (size_t)(j - q) * 256) : kStP;
-#pragma unroll
-    for (int q = 0; q < kW; ++q) {
-        if (!*done) {
-            uint32_t x = v[q];
 #pragma unroll 1
-            for (uint32_t spin = 0; (x >> 30) == 0; ++spin) {
-                if (spin >= kSpinLimit) {
-                    atomicAdd(stall, 1u);
-                    x = kStP;
-                    break;
+    for (uint32_t spin = 0;; ++spin) {
+#pragma unroll
+        for (int q = 0; q < kW; ++q)
+            v[q] = j - q >= 0 ? status_load(st + (size_t)(j - q) * 256) : kStP;
+        uint32_t add = 0;
+        bool found = false, blocked = false;
+#pragma unroll
+        for (int q = 0; q < kW; ++q) {
+            if (!found && !blocked) {
+                const uint32_t x = v[q];
+                if ((x >> 30) == 0) {
+                    blocked = true;
+                } else {
+                    add += x & kStVal;
+                    found = (x & kStP) != 0;
                 }
-                __builtin_amdgcn_s_sleep(1);
-                x = status_load(st + (size_t)(j - q) * 256);
             }
-            *excl += x & kStVal;
-            *done = (x & kStP) != 0;
         }
+        if (!blocked) {
+            *excl += add;
+            *done = found;
+            return;
+        }
+        if (spin >= kSpinLimit) {  // never expected: counted, and the prefix taken as is
+            atomicAdd(stall, 1u);
+            *excl += add;
+            *done = true;
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
 }
 
@@ -145,9 +160,10 @@ __device__ __forceinline__ void sort_tile_clear(SortShared<kStage, kR> &sh) {
 // exclusive prefix of its group, found by decoupled look-back over the
 // groups' words.  Tile words (st[j * 256]) only ever hold kStA | count; the
 // last tile of each group publishes the group's sum and then its inclusive
-// prefix (gst[g * 256]).  Every wait is for a lower tile, so it ends; waits
-// are bounded as in look_back.  The chain of look-backs is the number of
-// groups long instead of the number of tiles.
+// prefix (gst[g * 256]); the groups are looked back over kLookGroup at a time.
+// Every wait is for a lower tile, so it ends; waits are bounded as in
+// look_back.  The chain of look-backs is the number of groups long instead
+// of the number of tiles.
 constexpr uint32_t kLookGroup = 16;
 
 __device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, uint32_t tile,
@@ -155,35 +171,40 @@ __device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, 
                                                    uint32_t *__restrict__ stall) {
     const uint32_t g = tile / kLookGroup, first = g * kLookGroup;
     status_store(st + (size_t)tile * 256, kStA | c);
-    uint32_t v[kLookGroup - 1];
-#pragma unroll
-    for (uint32_t q = 0; q + 1 < kLookGroup; ++q)
-        v[q] = first + q < tile ? status_load(st + (size_t)(first + q) * 256) : kStA;
+    // the counts of the tiles before this one in its group, read together
+    // (again together while any is unpublished)
     uint32_t inner = 0;
-#pragma unroll
-    for (uint32_t q = 0; q + 1 < kLookGroup; ++q) {
-        uint32_t x = v[q];
 #pragma unroll 1
-        for (uint32_t spin = 0; (x >> 30) == 0; ++spin) {
-            if (spin >= kSpinLimit) {
-                atomicAdd(stall, 1u);
-                x = kStA;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-            x = status_load(st + (size_t)(first + q) * 256);
+    for (uint32_t spin = 0;; ++spin) {
+        uint32_t v[kLookGroup - 1];
+#pragma unroll
+        for (uint32_t q = 0; q + 1 < kLookGroup; ++q)
+            v[q] = first + q < tile ? status_load(st + (size_t)(first + q) * 256) : kStA;
+        uint32_t sum = 0;
+        bool ready = true;
+#pragma unroll
+        for (uint32_t q = 0; q + 1 < kLookGroup; ++q) {
+            ready = ready && (v[q] >> 30) != 0;
+            sum += v[q] & kStVal;
         }
-        inner += x & kStVal;
+        if (ready || spin >= kSpinLimit) {
+            if (!ready) atomicAdd(stall, 1u);  // never expected
+            inner = sum;
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
     }
     const bool last = tile + 1 == first + kLookGroup || tile + 1 == tiles;
     if (last) status_store(gst + (size_t)g * 256, (g ? kStA : kStP) | (inner + c));
     uint32_t gex = 0;
     if (g) {
+        // groups' sums are published at about the same time, and their
+        // prefixes one after another: read a wide window of them at once
         int j = (int)g - 1;
         bool done = false;
         while (!done) {
-            look_back<kLookWindow>(gst, j, &gex, &done, stall);
-            j -= kLookWindow;
+            look_back<kLookGroup>(gst, j, &gex, &done, stall);
+            j -= (int)kLookGroup;
         }
         if (last) status_store(gst + (size_t)g * 256, kStP | (gex + inner + c));
     }
@@ -195,7 +216,10 @@ __device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, 
 // pass's [tiles][256] words; sh.slot cleared by sort_tile_clear() before a
 // barrier.  Ends with sh in use: a barrier must precede its next use.
 // kGrouped: grouped_prefix instead of the tile-by-tile look-back (tiles: the
-// pass's tile count; gstatus: its [groups][256] words, zeroed).
+// pass's tile count; gstatus: its [groups][256] words, zeroed); with totals
+// (this pass's digit totals, split over `copies` copies `copy_stride` words
+// apart) gbase is not given but computed here, the totals loaded ahead of the
+// keys and scanned once the tile is ranked.
 template <bool kStage, int kR, bool kGrouped = false>
 __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           const uint32_t *__restrict__ idx_in,
@@ -205,10 +229,15 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           uint32_t *__restrict__ status,
                                           uint32_t *__restrict__ stall,
                                           SortShared<kStage, kR> &sh, uint32_t tiles = 0,
-                                          uint32_t *__restrict__ gstatus = nullptr) {
+                                          uint32_t *__restrict__ gstatus = nullptr,
+                                          const uint32_t *__restrict__ totals = nullptr,
+                                          uint32_t copies = 0, uint32_t copy_stride = 0) {
     constexpr int kRows = SortShared<kStage, kR>::kRows;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint32_t base = tile * SortShared<kStage, kR>::kTile;
+    uint32_t total = 0;
+    if (kGrouped && totals)
+        for (uint32_t c = 0; c < copies; ++c) total += totals[c * copy_stride + t];
     uint32_t key[kR], val[kR], rank[kR];
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
@@ -232,6 +261,10 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
         const uint32_t x = sh.slot[k * 256 + t];
         sh.slot[k * 256 + t] = c;
         c += x;
+    }
+    if (kGrouped && totals) {
+        uint32_t all = 0;
+        gbase = block_scan256(total, &all, sh.wsum) - total;  // synchronises
     }
     uint32_t tile_n = 0, first = 0;
     if (kStage) {
