@@ -56,6 +56,10 @@ struct BhCounters {
 constexpr int kSortThreads = 256;
 constexpr int kSortRounds = 8;
 constexpr int kSortTile = kSortThreads * kSortRounds;
+// Tiles per group of the radix passes' two-level look-back; each pass's
+// status words hold its tile words, then one group word set per kLookGroup
+// tiles.
+constexpr uint32_t kLookGroup = 16;
 // The single-system build counts every pass's digits into kDigitCopies
 // copies of the totals (workgroup b into copy b mod kDigitCopies), so that
 // fewer workgroups add into each word; a pass sums the copies.

@@ -163,8 +163,7 @@ __device__ __forceinline__ void sort_tile_clear(SortShared<kStage, kR> &sh) {
 // prefix (gst[g * 256]); the groups are looked back over kLookGroup at a time.
 // Every wait is for a lower tile, so it ends; waits are bounded as in
 // look_back.  The chain of look-backs is the number of groups long instead
-// of the number of tiles.
-constexpr uint32_t kLookGroup = 16;
+// of the number of tiles.  (kLookGroup: internal.hpp.)
 
 __device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, uint32_t tile,
                                                    uint32_t tiles, uint32_t c,
