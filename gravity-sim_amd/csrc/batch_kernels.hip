@@ -468,8 +468,8 @@ __global__ __launch_bounds__(1024) void bb_top_kernel(const TileCell *__restrict
 // kHalf: a group of the W = 32 vote (the published runs' subgroup width), one
 // 32-particle group per wave with sorted position g * 32 + l held by lanes l
 // and l + 32, walked exactly as bh_traverse32_kernel walks it (both halves
-// vote alike; each half sums 32 nodes of every 64-node chunk, and the halves'
-// partials are added before the acceleration), so the result is that
+// vote alike; each half sums its part of every chunk (sum_run_queue), and the
+// halves' partials are added before the acceleration), so the result is that
 // kernel's bit for bit.
 template <bool kHalf>
 __device__ void bb_walk_group(uint32_t g, const SysDesc &d, const RankStats &st, float4 *stage,

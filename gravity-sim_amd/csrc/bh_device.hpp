@@ -400,7 +400,10 @@ __device__ __forceinline__ void split_share(const Partials &P, uint32_t k, float
 // staged in the wave's LDS as packed 12-byte (x, y, m) records,
 // summed node by node from zero and added to its partial.  kHalf: the wave
 // holds one 32-particle group twice (lane l and l + 32 the same particle) and
-// each half sums its 32 nodes of every chunk (nodes 0-31: lanes 0-31).
+// each half sums part of every chunk: lanes 0-31 its first s nodes, lanes
+// 32-63 the rest, s = the chunk's node count halved and rounded up to a
+// multiple of 4 (32 for a full chunk), so a partial chunk costs the wave
+// about half its nodes rather than up to all of them.
 template <bool kHalf = false>
 __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, uint32_t ch0,
                                               float4 *stage, const float4 *__restrict__ node4,
@@ -446,8 +449,9 @@ __device__ __forceinline__ void sum_run_queue(const uint2 *queue, uint32_t qn, u
         float cx = 0.0f, cy = 0.0f;
         uint32_t j = 0, jend = cnt;
         if (kHalf) {
-            j = (lane >> 5) << 5;
-            jend = min(cnt, j + 32u);
+            const uint32_t split = ((cnt + 7u) >> 3) << 2;  // a multiple of 4: float4 reads
+            j = lane < 32u ? 0u : split;
+            jend = lane < 32u ? min(split, cnt) : cnt;
         }
         for (; j + 8 <= jend; j += 8) {
             f4v r[6];

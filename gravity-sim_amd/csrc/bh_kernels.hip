@@ -1193,8 +1193,8 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
 // W = 32 (the published runs' subgroup width): one 32-particle group per
 // wave, particle slot g * 32 + l held by lanes l and l + 32, walked by the
 // staged W = 64 walk (both halves vote alike, so the 64-lane vote is the
-// group's: the same visits and acceptances); each half sums 32 nodes of every
-// 64-node chunk.  Per partial k the halves' sums are added, then the
+// group's: the same visits and acceptances); each half sums its part of every
+// chunk (sum_run_queue).  Per partial k the halves' sums are added, then the
 // acceleration is (((c + p0) + p1) + p2) + p3 as at W = 64.
 template <bool kDebugAccel>
 __global__ __launch_bounds__(256) void bh_traverse32_kernel(
