@@ -134,14 +134,11 @@ __device__ __forceinline__ void look_back(const uint32_t *st, int j, uint32_t *e
 }
 
 // Exclusive prefix of this pass's digit totals for digit threadIdx.x
-// (synchronises the workgroup).  The totals may be split over `copies`
-// arrays `stride` words apart, whose sum they are.
+// (synchronises the workgroup).
 template <bool kStage, int kR>
 __device__ __forceinline__ uint32_t digit_base(const uint32_t *__restrict__ digit_total,
-                                               SortShared<kStage, kR> &sh, uint32_t copies = 1,
-                                               uint32_t stride = 0) {
-    uint32_t total = 0;
-    for (uint32_t c = 0; c < copies; ++c) total += digit_total[c * stride + threadIdx.x];
+                                               SortShared<kStage, kR> &sh) {
+    const uint32_t total = digit_total[threadIdx.x];
     uint32_t all = 0;
     return block_scan256(total, &all, sh.wsum) - total;
 }
@@ -164,7 +161,6 @@ __device__ __forceinline__ void sort_tile_clear(SortShared<kStage, kR> &sh) {
 // Every wait is for a lower tile, so it ends; waits are bounded as in
 // look_back.  The chain of look-backs is the number of groups long instead
 // of the number of tiles.  (kLookGroup: internal.hpp.)
-
 __device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, uint32_t tile,
                                                    uint32_t tiles, uint32_t c,
                                                    uint32_t *__restrict__ stall) {
