@@ -46,14 +46,25 @@ gsim_status hip_fail(hipError_t e, const char *what) {
 
 size_t align_up(size_t n, size_t a) { return (n + a - 1) / a * a; }
 
+// out[perm[slot]] = in[slot] for the first n caller indices, w floats each
+void unpermute(const std::vector<uint32_t> &perm, const float *in, float *out, size_t w, size_t n) {
+    for (size_t slot = 0; slot < perm.size(); ++slot) {
+        const size_t i = perm[slot];
+        if (i >= n) continue;
+        for (size_t c = 0; c < w; ++c) out[i * w + c] = in[slot * w + c];
+    }
+}
+
 void free_sim(gsim_sim *s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int k = 0; k < 2; ++k) {
         (void)hipFree(s->pos[k]);
         (void)hipFree(s->vel[k]);
+        (void)hipFree(s->perm[k]);
     }
     (void)hipFree(s->mass);
+    (void)hipFree(s->mass_next);
     (void)hipFree(s->partial);
     (void)hipFree(s->accel);
     gsim::bh_free(&s->bh);
@@ -141,11 +152,15 @@ gsim_status run_steps(gsim_sim *s, uint32_t count) {
             GSIM_HIP(hipEventRecord(ev[0], s->stream));
         }
         if (bh) {
-            GSIM_HIP(gsim::bh_launch_build(s, s->pos[in], s->mass, s->pos[out], s->vel[in],
-                                           s->vel[out]));
+            // the build writes the next order's masses, permutation and killed
+            // particles; the walk writes the rest of pos[out] and vel[out]
+            GSIM_HIP(gsim::bh_launch_build(s, s->pos[in], s->mass, s->perm[in], s->pos[out],
+                                           s->vel[in], s->vel[out], s->mass_next,
+                                           s->perm[out]));
             if (s->timing) GSIM_HIP(hipEventRecord(ev[1], s->stream));
             GSIM_HIP(gsim::bh_launch_traverse(s, s->pos[in], s->vel[in], s->pos[out],
-                                              s->vel[out]));
+                                              s->vel[out], s->perm[out]));
+            std::swap(s->mass, s->mass_next);
             s->has_tree = true;
         } else {
             GSIM_HIP(gsim::ds_launch_partial(s, s->pos[in], s->mass));
@@ -264,6 +279,11 @@ gsim_status gsim_create(const gsim_params *params, const float *pos_xy, const fl
     }
     if (e == hipSuccess) e = hipMalloc((void **)&s->mass, sizeof(float) * na);
     if (e == hipSuccess) e = hipMalloc((void **)&s->accel, sizeof(float2) * na);
+    if (params->algorithm == GSIM_BARNES_HUT) {
+        for (int k = 0; k < 2 && e == hipSuccess; ++k)
+            e = hipMalloc((void **)&s->perm[k], sizeof(uint32_t) * na);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->mass_next, sizeof(float) * na);
+    }
     if (e != hipSuccess) return bail(fail(GSIM_ERROR_OUT_OF_MEMORY, "device allocation failed: %s",
                                           hipGetErrorString(e)));
 
@@ -279,6 +299,13 @@ gsim_status gsim_create(const gsim_params *params, const float *pos_xy, const fl
     }
     if (e == hipSuccess) e = hipMemcpy(s->mass, hm.data(), sizeof(float) * na, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(s->accel, 0, sizeof(float2) * na);
+    if (e == hipSuccess && s->perm[0]) {
+        // the caller's order to start with
+        std::vector<uint32_t> id(na);
+        for (size_t i = 0; i < na; ++i) id[i] = (uint32_t)i;
+        for (int k = 0; k < 2 && e == hipSuccess; ++k)
+            e = hipMemcpy(s->perm[k], id.data(), sizeof(uint32_t) * na, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) return bail(hip_fail(e, "upload"));
 
     if (params->algorithm == GSIM_DIRECT_SUM) {
@@ -320,9 +347,25 @@ gsim_status gsim_read(gsim_sim *s, float *pos_xy, float *vel_xy, float *mass) {
     if (!s) return fail(GSIM_ERROR_INVALID_ARGUMENT, "gsim_read: NULL handle");
     GSIM_HIP(hipSetDevice(s->device));
     GSIM_HIP(hipStreamSynchronize(s->stream));
-    if (pos_xy) GSIM_HIP(hipMemcpy(pos_xy, s->pos[s->cur], sizeof(float2) * s->n, hipMemcpyDeviceToHost));
-    if (vel_xy) GSIM_HIP(hipMemcpy(vel_xy, s->vel[s->cur], sizeof(float2) * s->n, hipMemcpyDeviceToHost));
-    if (mass) GSIM_HIP(hipMemcpy(mass, s->mass, sizeof(float) * s->n, hipMemcpyDeviceToHost));
+    if (!s->perm[0]) {  // direct sum: the caller's order
+        if (pos_xy) GSIM_HIP(hipMemcpy(pos_xy, s->pos[s->cur], sizeof(float2) * s->n, hipMemcpyDeviceToHost));
+        if (vel_xy) GSIM_HIP(hipMemcpy(vel_xy, s->vel[s->cur], sizeof(float2) * s->n, hipMemcpyDeviceToHost));
+        if (mass) GSIM_HIP(hipMemcpy(mass, s->mass, sizeof(float) * s->n, hipMemcpyDeviceToHost));
+        return GSIM_OK;
+    }
+    // Barnes-Hut: the state is in the last step's sorted order
+    const size_t na = s->na;
+    std::vector<uint32_t> perm(na);
+    GSIM_HIP(hipMemcpy(perm.data(), s->perm[s->cur], sizeof(uint32_t) * na, hipMemcpyDeviceToHost));
+    std::vector<float> tmp(2 * na);
+    const float *src[3] = {(const float *)s->pos[s->cur], (const float *)s->vel[s->cur], s->mass};
+    float *dst[3] = {pos_xy, vel_xy, mass};
+    for (int a = 0; a < 3; ++a) {
+        if (!dst[a]) continue;
+        const size_t w = a < 2 ? 2 : 1;
+        GSIM_HIP(hipMemcpy(tmp.data(), src[a], sizeof(float) * w * na, hipMemcpyDeviceToHost));
+        unpermute(perm, tmp.data(), dst[a], w, s->n);
+    }
     return GSIM_OK;
 }
 
@@ -477,6 +520,13 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
         *written = sizeof(float2) * na;
         if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
         GSIM_HIP(hipMemcpy(buf, s->accel, *written, hipMemcpyDeviceToHost));
+        if (s->perm[0]) {  // Barnes-Hut writes it in the last step's output order
+            std::vector<uint32_t> perm(na);
+            GSIM_HIP(hipMemcpy(perm.data(), s->perm[s->cur], sizeof(uint32_t) * na,
+                               hipMemcpyDeviceToHost));
+            std::vector<float> tmp((const float *)buf, (const float *)buf + 2 * na);
+            unpermute(perm, tmp.data(), (float *)buf, 2, na);
+        }
         return GSIM_OK;
     }
     if (s->params.algorithm != GSIM_BARNES_HUT)
@@ -491,7 +541,7 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
         case GSIM_DUMP_CELLS: {
             *written = sizeof(uint32_t) * na;
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
-            GSIM_HIP(hipMemcpy(buf, s->bh.keys[0], *written, hipMemcpyDeviceToHost));
+            GSIM_HIP(hipMemcpy(buf, s->bh.keys[s->bh.kcells], *written, hipMemcpyDeviceToHost));
             uint32_t *u = (uint32_t *)buf;
             for (size_t i = 0; i < na; ++i)
                 if (u[i] >= (1u << (2 * s->bh.bits))) u[i] = 0xffffffffu;
@@ -512,13 +562,27 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
             const int in = s->steps ? (s->cur ^ 1) : s->cur;
             GSIM_HIP(gsim::bh_launch_morton30(s, s->pos[in], s->bh.keys[2]));
             GSIM_HIP(hipStreamSynchronize(s->stream));
-            GSIM_HIP(hipMemcpy(buf, s->bh.keys[2], *written, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> perm(na), code(na);
+            GSIM_HIP(hipMemcpy(code.data(), s->bh.keys[2], *written, hipMemcpyDeviceToHost));
+            GSIM_HIP(hipMemcpy(perm.data(), s->perm[in], sizeof(uint32_t) * na, hipMemcpyDeviceToHost));
+            uint32_t *u = (uint32_t *)buf;
+            for (size_t slot = 0; slot < na; ++slot) u[perm[slot]] = code[slot];
             return GSIM_OK;
         }
         case GSIM_DUMP_SORTED_SRC:
             *written = sizeof(uint32_t) * na;
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
             GSIM_HIP(hipMemcpy(buf, s->bh.sorted_src, *written, hipMemcpyDeviceToHost));
+            {
+                // slots of the last step's input order -> the caller's indices
+                std::vector<uint32_t> perm(na);
+                const int in = s->steps ? (s->cur ^ 1) : s->cur;
+                GSIM_HIP(hipMemcpy(perm.data(), s->perm[in], sizeof(uint32_t) * na,
+                                   hipMemcpyDeviceToHost));
+                uint32_t *u = (uint32_t *)buf;
+                for (size_t k = 0; k < na; ++k)
+                    if (u[k] < na) u[k] = perm[u[k]];
+            }
             return GSIM_OK;
         case GSIM_DUMP_NODE_TOTAL:
             *written = sizeof(uint32_t);

@@ -123,14 +123,21 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
 // into copy b mod kDigitCopies; zeroed by the previous build's keys kernel,
 // or at allocation), while digit_next, the other half, is zeroed for the next
 // build; the radix passes' look-back words are zeroed as well.
-template <bool kBox, bool kDeep, int kR>
+// The particles are in the last step's sorted order: slot i holds the
+// particle of original index perm[i].  The keys and indices are written at
+// the original index (key[perm[i]] = cell, idx[perm[i]] = i), so the sort
+// runs over original indices, whose order breaks ties in a leaf, and carries
+// each particle's slot.  An out-of-domain particle gets the killed key; its
+// kill (InitShader.comp:110-117) is written by the emit kernel at the slot it
+// sorts to.  kWalk: the last walk (write_next_key) and emit already wrote the
+// keys and indices; only the digits are counted.
+template <bool kBox, bool kDeep, int kR, bool kWalk>
 __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
-    float2 *__restrict__ pos, float *__restrict__ mass, float2 *__restrict__ pos_out,
-    const float2 *__restrict__ vel, float2 *__restrict__ vel_out, uint32_t na,
+    const float2 *__restrict__ pos, const uint32_t *__restrict__ perm, uint32_t na,
     uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
     uint32_t *__restrict__ digit_total, uint32_t *__restrict__ digit_next,
     uint32_t *__restrict__ status, uint32_t status_words, BhCounters *__restrict__ ctr,
-    const float *__restrict__ box, float4 *__restrict__ rec) {
+    const float *__restrict__ box) {
     constexpr int kPasses = kDeep ? 4 : 3;
     __shared__ uint32_t h[kPasses][256];
     const uint32_t t = threadIdx.x;
@@ -148,13 +155,24 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * (kR * kSortThreads);
-    float2 pr[kR];  // all loads first: the kill below stores to pos
-    float mr[kR];
+    if constexpr (kWalk) {
+        uint32_t c[kR];
+#pragma unroll
+        for (int r = 0; r < kR; ++r) {
+            const uint32_t i = base + r * kSortThreads + t;
+            c[r] = i < na ? keys[i] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < kR; ++r)
+            if (base + r * kSortThreads + t < na) hist_add<kPasses, false>(h, c[r], true);
+    } else {
+    float2 pr[kR];
+    uint32_t orig[kR];
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
         pr[r] = i < na ? pos[i] : make_float2(0.0f, 0.0f);
-        mr[r] = i < na ? mass[i] : 0.0f;
+        orig[r] = i < na ? perm[i] : 0u;
     }
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
@@ -164,22 +182,11 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
         uint32_t c;
         if (kDeep) c = kBox ? cell30_box(p.x, p.y, box[0], box[1], box[2]) : cell30(p.x, p.y);
         else c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
-        if (c == 0xffffffffu) {
-            // InitShader.comp:110-117; the reference leaves vel_out stale, here it
-            // keeps the input velocity.
-            const float k = -kSimSize * 2.0f;
-            pos[i] = make_float2(k, k);
-            pos_out[i] = make_float2(k, k);
-            mass[i] = 0.0f;
-            vel_out[i] = vel[i];
-            c = kDeep ? kKilledKeyDeep : kKilledKey;
-        }
-        keys[i] = c;
-        idx[i] = i;
-        // the particle record for emit's gather: one 16-byte read there
-        // instead of two (a killed particle's record is never read)
-        rec[i] = make_float4(p.x, p.y, mr[r], 0.0f);
+        if (c == 0xffffffffu) c = kDeep ? kKilledKeyDeep : kKilledKey;
+        keys[orig[r]] = c;
+        idx[orig[r]] = i;
         hist_add<kPasses, false>(h, c, true);
+    }
     }
     __syncthreads();
     uint32_t *const copy = digit_total + (blockIdx.x % kDigitCopies) * 1024;
@@ -276,7 +283,11 @@ __global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict
 // few loads per thread) instead of a separate scan launch.
 __global__ __launch_bounds__(256) void bh_emit_kernel(
     const uint32_t *__restrict__ keys, const uint32_t *__restrict__ sidx,
-    const float4 *__restrict__ rec, uint32_t na, int bits,
+    const float2 *__restrict__ pos, const float *__restrict__ mass,
+    const float2 *__restrict__ vel, const uint32_t *__restrict__ perm_in,
+    float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float *__restrict__ mass_out,
+    uint32_t *__restrict__ perm_out, uint32_t *__restrict__ nkeys, uint32_t *__restrict__ nidx,
+    uint32_t na, int bits,
     const uint32_t *__restrict__ block_cells, const uint32_t *__restrict__ block_leaves,
     int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
     uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
@@ -286,11 +297,13 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     uint32_t *__restrict__ gran_first, uint32_t *__restrict__ gran_key) {
     __shared__ uint32_t wa[4], wb[4], wc[4], pa[4], pb[4], pc[4];
     const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-    // the particle's gather does not depend on the scans below: start it first
+    // the particle now at sorted position k, from its slot i of the last
+    // step's order (nearly the same order, so nearly sequential reads); k is
+    // its slot in the next order
     const uint32_t i = k < na ? sidx[k] : 0u;
-    const float4 r4 = k < na ? rec[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    const float2 p = make_float2(r4.x, r4.y);
-    const float m = r4.z;
+    const float2 p = k < na ? pos[i] : make_float2(0.0f, 0.0f);
+    const float m = k < na ? mass[i] : 0.0f;
+    const uint32_t orig = k < na ? perm_in[i] : 0u;
     // the counts of the blocks before this one, eight loads per thread at once
     uint32_t xa = 0, xb = 0, xc = 0;
     for (uint32_t j0 = threadIdx.x; j0 < blockIdx.x; j0 += 8 * 256) {
@@ -340,7 +353,18 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         ec += wc[w];
     }
     if (k >= na) return;
+    perm_out[k] = orig;
     if (!q.inside) {
+        // killed (InitShader.comp:110-117): not walked; the reference leaves
+        // vel_out stale, here it keeps the input velocity
+        const float kk = -kSimSize * 2.0f;
+        pos_out[k] = make_float2(kk, kk);
+        vel_out[k] = vel[i];
+        mass_out[k] = 0.0f;
+        if (nkeys) {  // the next build's key, where the walk writes the others
+            nkeys[orig] = bits == (int)kDeepBits ? kKilledKeyDeep : kKilledKey;
+            nidx[orig] = k;
+        }
         sorted_src[k] = na;  // ClearShader.comp:81-82 sentinel
         if (k == 0) {        // nothing inside the domain
             ctr->n_in = 0;
@@ -353,6 +377,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         }
         return;
     }
+    mass_out[k] = m;
     const uint32_t C = ea, L = eb;
     const uint32_t slot = k + C + q.ns;
     // ParticleSortShader.comp:99-107 (canonical order)
@@ -740,6 +765,25 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
 // streamed as a plain direct sum without votes; cell nodes are voted on with
 // r2 = (dx*dx + dy*dy) + eps2 rounded exactly as the reference's shader, so
 // the visited and accepted node sets are the reference's.
+// The next build's sort input, written by the walk on the fixed domain: the
+// cell of the particle's new position (InitShader.comp:73-119), at the
+// particle's caller index o = perm[slot] (keys[o] = cell, idx[o] = slot), so
+// the next build reads cells instead of positions.  keys == nullptr: the next
+// build computes the cells itself (adaptive box, first step).
+struct NextKeys {
+    const uint32_t *perm;  // this step's output order: slot -> caller's index
+    uint32_t *keys, *idx;
+    uint32_t deep;  // 30-bit cells
+};
+
+__device__ __forceinline__ void write_next_key(const NextKeys &nk, uint32_t slot, float2 p) {
+    uint32_t c = nk.deep ? cell30(p.x, p.y) : cell20(p.x, p.y);
+    if (c == 0xffffffffu) c = nk.deep ? kKilledKeyDeep : kKilledKey;
+    const uint32_t o = nk.perm[slot];
+    nk.keys[o] = c;
+    nk.idx[o] = slot;
+}
+
 // One group's walk (all four partials) and integrate.
 template <bool kDebugAccel, bool kStaged>
 __device__ __forceinline__ void walk_group(
@@ -749,11 +793,12 @@ __device__ __forceinline__ void walk_group(
     float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, float eps2,
     float theta2, float kv, float kp, unsigned long long *__restrict__ gvis,
-    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gcost = nullptr,
+    unsigned long long *__restrict__ gacc, const NextKeys &nk, uint32_t *__restrict__ gcost = nullptr,
     uint32_t *__restrict__ gchunk = nullptr, const uint8_t *__restrict__ heavy = nullptr) {
     if (heavy && heavy[g]) return;  // walked by bh_traverse_heavy_kernel
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t src = sorted_src[g * 64 + lane];
+    const uint32_t dst = g * 64 + lane;  // this particle's slot in the next order
+    const uint32_t src = sorted_src[dst];
     const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
@@ -780,9 +825,11 @@ __device__ __forceinline__ void walk_group(
         partial_sum(P, ax, ay);
         const float2 v = vel[src];
         const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
-        pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
-        vel_out[src] = make_float2(nvx, nvy);
-        if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+        const float2 np = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+        pos_out[dst] = np;
+        vel_out[dst] = make_float2(nvx, nvy);
+        if (nk.keys) write_next_key(nk, dst, np);
+        if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
     }
     if (gcost && lane == 0) gcost[g] = cost;
 }
@@ -883,7 +930,7 @@ __global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
     const uint32_t *__restrict__ ncount, float2 *__restrict__ pos_out,
     float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap,
     float eps2, float theta2, float kv, float kp, unsigned long long *__restrict__ gvis,
-    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk) {
+    unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk, NextKeys nk) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[kRunQueue];
     __shared__ uint32_t qs[3];  // queued runs, their first chunk number, walk finished
@@ -894,7 +941,8 @@ __global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
     // node count is bounded by the node arrays; every step advances ind by at
     // least one, so the walk ends
     const uint32_t end = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
-    const uint32_t src = sorted_src[g * 64 + lane];
+    const uint32_t dst = g * 64 + lane;  // this particle's slot in the next order
+    const uint32_t src = sorted_src[dst];
     const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
@@ -973,9 +1021,11 @@ __global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
         ay = ((share[0][lane].y + share[1][lane].y) + share[2][lane].y) + share[3][lane].y;
         const float2 v = vel[src];
         const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
-        pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
-        vel_out[src] = make_float2(nvx, nvy);
-        if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+        const float2 np = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+        pos_out[dst] = np;
+        vel_out[dst] = make_float2(nvx, nvy);
+        if (nk.keys) write_next_key(nk, dst, np);
+        if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
     }
 }
 
@@ -1036,7 +1086,7 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, const uint32_t *__restrict__ gpre, uint32_t *__restrict__ gcost,
     uint32_t split_x, unsigned long long *__restrict__ gvis,
-    unsigned long long *__restrict__ gacc) {
+    unsigned long long *__restrict__ gacc, NextKeys nk) {
     __shared__ float4 stage[16][64];
     __shared__ uint2 runq[16][kRunQueue];
     __shared__ uint32_t seg[4];            // first group, end, next unit, list fill
@@ -1101,7 +1151,8 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         else g = list[h];  // h < nsplit <= kSplitSlots
         g = __builtin_amdgcn_readfirstlane(g);
         const uint32_t parts = !kSplit ? 0xfu : whole ? 0xfu : 1u << part;
-        const uint32_t src = sorted_src[g * 64 + lane];
+        const uint32_t dst = g * 64 + lane;  // this particle's slot in the next order
+        const uint32_t src = sorted_src[dst];
         const bool real = src < na;  // ForceShader.comp:80-92
         float px = 0.0f, py = 0.0f;
         if (real) {
@@ -1141,9 +1192,11 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         if (real) {
             const float2 vv = vel[src];
             const float nvx = vv.x + ax * kv, nvy = vv.y + ay * kv;
-            pos_out[src] = make_float2(px + (vv.x + nvx) * kp, py + (vv.y + nvy) * kp);
-            vel_out[src] = make_float2(nvx, nvy);
-            if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+            const float2 np = make_float2(px + (vv.x + nvx) * kp, py + (vv.y + nvy) * kp);
+            pos_out[dst] = np;
+            vel_out[dst] = make_float2(nvx, nvy);
+            if (nk.keys) write_next_key(nk, dst, np);
+            if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
         }
     }
 }
@@ -1160,7 +1213,7 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, unsigned long long *__restrict__ gvis,
     unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk,
-    const uint8_t *__restrict__ heavy, uint32_t xcd) {
+    const uint8_t *__restrict__ heavy, uint32_t xcd, NextKeys nk) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
     __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1178,7 +1231,7 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
             if (g >= groups) break;
             walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4,
                                              ncount, pos_out, vel_out, acc_dbg, na, eps2,
-                                             theta2, kv, kp, gvis, gacc, nullptr, gchunk);
+                                             theta2, kv, kp, gvis, gacc, nk, nullptr, gchunk);
         }
     } else {
         const uint32_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
@@ -1186,7 +1239,7 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
         if (g >= groups) return;  // whole wave beyond the last group
         walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
                                          pos_out, vel_out, acc_dbg, na, eps2, theta2, kv,
-                                         kp, gvis, gacc, nullptr, gchunk, heavy);
+                                         kp, gvis, gacc, nk, nullptr, gchunk, heavy);
     }
 }
 
@@ -1203,7 +1256,8 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     const uint32_t *__restrict__ ncount, BhCounters *__restrict__ ctr,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
     uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp,
-    unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc, uint32_t xcd) {
+    unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc, uint32_t xcd,
+    NextKeys nk) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1212,7 +1266,8 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     if (g >= na / 32) return;  // whole wave beyond the last group
     // node count is bounded by the node arrays; every step advances the walk
     const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
-    const uint32_t src = sorted_src[g * 32 + (lane & 31u)];
+    const uint32_t dst = g * 32 + (lane & 31u);  // this particle's slot in the next order
+    const uint32_t src = sorted_src[dst];
     const bool real = src < na;  // ForceShader.comp:80-92
     float px = 0.0f, py = 0.0f;
     if (real) {
@@ -1242,9 +1297,11 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
         partial_sum(P, ax, ay);
         const float2 v = vel[src];
         const float nvx = v.x + ax * kv, nvy = v.y + ay * kv;
-        pos_out[src] = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
-        vel_out[src] = make_float2(nvx, nvy);
-        if (kDebugAccel) acc_dbg[src] = make_float2(ax, ay);
+        const float2 np = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
+        pos_out[dst] = np;
+        vel_out[dst] = make_float2(nvx, nvy);
+        if (nk.keys) write_next_key(nk, dst, np);
+        if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
     }
 }
 
@@ -1285,7 +1342,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     do {                                                                        \
         if (e == hipSuccess) e = hipMalloc((void **)&(ptr), sizeof(*(ptr)) * (count)); \
     } while (0)
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         GSIM_ALLOC(b->keys[k], na);
         GSIM_ALLOC(b->idx[k], na);
     }
@@ -1298,7 +1355,6 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->sdep, na + 1);
     GSIM_ALLOC(b->cpos, na);
     GSIM_ALLOC(b->prod, na);
-    GSIM_ALLOC(b->rec, na);
     GSIM_ALLOC(b->leaf_first, na + 1);
     GSIM_ALLOC(b->leaf_key, na);
     GSIM_ALLOC(b->leaf_sum, na);
@@ -1337,11 +1393,11 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
 }
 
 void bh_free(BhBuffers *b) {
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         (void)hipFree(b->keys[k]);
         (void)hipFree(b->idx[k]);
     }
-    void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod, b->rec,
+    void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
                     b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->plan, b->heavy, b->gpre, b->gvis, b->gacc, b->box,
@@ -1350,8 +1406,18 @@ void bh_free(BhBuffers *b) {
     *b = BhBuffers{};
 }
 
-hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_out,
-                           const float2 *vel, float2 *vel_out) {
+// Where this step's walk writes the next build's keys (the buffer the current
+// build does not read), or none when the next build must compute them.
+static NextKeys next_keys(gsim_sim *s, const uint32_t *perm_out) {
+    BhBuffers &b = s->bh;
+    if (s->bh_box || !perm_out) return NextKeys{nullptr, nullptr, nullptr, 0u};
+    const int o = b.kin ^ 3;
+    return NextKeys{perm_out, b.keys[o], b.idx[o], b.bits == kDeepBits ? 1u : 0u};
+}
+
+hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
+                           const uint32_t *perm_in, float2 *pos_out, const float2 *vel,
+                           float2 *vel_out, float *mass_out, uint32_t *perm_out) {
     BhBuffers &b = s->bh;
     const uint32_t na = (uint32_t)s->na;
     const uint32_t g256 = (na + 255) / 256;
@@ -1375,16 +1441,21 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     uint32_t *const totals = b.digit_total + kDigitCopies * 1024 * b.parity;
     uint32_t *const totals_next = b.digit_total + kDigitCopies * 1024 * (b.parity ^ 1u);
     b.parity ^= 1u;
-#define GSIM_KEYS(B, D)                                                                       \
-    bh_keys_kernel<B, D, kSortRounds><<<kblk, kSortThreads, 0, s->stream>>>(                   \
-        pos, mass, pos_out, vel, vel_out, na, b.keys[0], b.idx[0], b.tile_range, totals,        \
-        totals_next, b.status, status_words, b.ctr, box, b.rec)
-    if (box) { if (deep) GSIM_KEYS(true, true); else GSIM_KEYS(true, false); }
-    else { if (deep) GSIM_KEYS(false, true); else GSIM_KEYS(false, false); }
+    // the sort's input: written by the last walk (fixed domain), else here
+    const int kin = b.kin;
+    const bool walk = b.walk_keys && !box;
+    b.kcells = kin;
+#define GSIM_KEYS(B, D, W)                                                                    \
+    bh_keys_kernel<B, D, kSortRounds, W><<<kblk, kSortThreads, 0, s->stream>>>(                \
+        pos, perm_in, na, b.keys[kin], b.idx[kin], b.tile_range, totals, totals_next,         \
+        b.status, status_words, b.ctr, box)
+    if (box) { if (deep) GSIM_KEYS(true, true, false); else GSIM_KEYS(true, false, false); }
+    else if (walk) { if (deep) GSIM_KEYS(false, true, true); else GSIM_KEYS(false, false, true); }
+    else { if (deep) GSIM_KEYS(false, true, false); else GSIM_KEYS(false, false, false); }
 #undef GSIM_KEYS
-    // 20-bit keys: keys[0] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take a
-    // fourth pass -> keys[2].  keys[0] keeps the per-particle cells.
-    const int in_of[4] = {0, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
+    // 20-bit keys: keys[kin] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take
+    // a fourth pass -> keys[2].  keys[kin] keeps the per-particle cells.
+    const int in_of[4] = {kin, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
     for (int pass = 0; pass < passes; ++pass) {
         const int i = in_of[pass], o = out_of[pass];
         uint32_t *const st = b.status + (size_t)pass * pass_words;
@@ -1397,7 +1468,10 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
     const uint32_t pb = (uint32_t)b.prep_blocks;
     bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, bits, b.block_cells, b.block_leaves,
                                               deep ? b.block_gran : nullptr);
-    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, b.rec, na, bits, b.block_cells,
+    const NextKeys nk = next_keys(s, perm_out);
+    bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, vel, perm_in, pos_out, vel_out,
+                                              mass_out, perm_out, nk.keys, nk.idx, na, bits,
+                                              b.block_cells,
                                               b.block_leaves, b.sdep, b.cpos, b.prod,
                                               b.leaf_first, b.leaf_key, b.node4, b.ncount,
                                               b.sorted_src, b.breaker, b.ctr,
@@ -1428,10 +1502,14 @@ hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_ou
 }
 
 hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
-                              float2 *pos_out, float2 *vel_out) {
+                              float2 *pos_out, float2 *vel_out, const uint32_t *perm_out) {
     BhBuffers &b = s->bh;
     const uint32_t na = (uint32_t)s->na;
     const uint32_t grid = (na + 255) / 256;
+    const NextKeys nk = next_keys(s, perm_out);
+    // the next build reads the cells this walk writes
+    b.walk_keys = nk.keys != nullptr;
+    if (b.walk_keys) b.kin ^= 3;
     const bool dbg = (s->debug_flags & GSIM_DEBUG_ACCEL) != 0;
     float2 *acc = dbg ? s->accel : nullptr;
     if (s->W == 64) {
@@ -1443,7 +1521,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     bh_traverse_bal_kernel<D, S><<<P, 1024, 0, s->stream>>>(                                   \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,          \
         (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gpre, b.gcost, s->trav_split,   \
-        b.gvis, b.gacc)
+        b.gvis, b.gacc, nk)
             if (dbg) { if (s->trav_split) GSIM_BAL(true, true); else GSIM_BAL(true, false); }
             else { if (s->trav_split) GSIM_BAL(false, true); else GSIM_BAL(false, false); }
 #undef GSIM_BAL
@@ -1468,7 +1546,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #define GSIM_HEAVY(D)                                                                        \
     bh_traverse_heavy_kernel<D><<<kMaxHeavy, 256, 0, s->side>>>(                              \
         b.plan, b.ctr, b.sorted_src, pos, vel, b.node4, b.ncount, pos_out, vel_out, acc, na, \
-        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk)
+        (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk, nk)
             if (dbg) GSIM_HEAVY(true);
             else GSIM_HEAVY(false);
 #undef GSIM_HEAVY
@@ -1484,7 +1562,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
         (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk, heavy, \
-        s->xcd_order)
+        s->xcd_order, nk)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
             else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }
@@ -1503,11 +1581,13 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         if (dbg)
             bh_traverse32_kernel<true><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order);
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order,
+                nk);
         else
             bh_traverse32_kernel<false><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
-                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order);
+                (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order,
+                nk);
     }
     return hipGetLastError();
 }

@@ -1,12 +1,14 @@
 // Internal state of one simulation handle and the kernel launchers behind the
 // C ABI.  Device layout (HBM), per handle:
 //   particle state  pos float2[Na] x2 (ping-pong), vel float2[Na] x2, mass f32[Na]
-//                   (mass is only written by the Barnes-Hut kill, so one copy)
+//                   (direct sum: one mass copy, in the caller's order; Barnes-Hut:
+//                   two mass copies and perm u32[Na] x2, the state of each set in
+//                   that step's sorted order, perm[k][slot] = caller's index)
 //   direct sum      one-sided: partial float2[S*Na] per-slice partial accelerations;
 //                   symmetric: row partials float2[tiles*16*64] then column
 //                   partials float2[(sum over groups of B - 16g)*64]
 //   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
-//                   cpos u32[Na], prod float4[Na], rec float4[Na], leaf records,
+//                   cpos u32[Na], prod float4[Na], leaf records,
 //                   tile results,
 //                   breaker u32[cap], node4 float4[cap] {x, y, mass, radius}
 //                   (a particle node's radius word holds minus its run length),
@@ -84,7 +86,13 @@ struct TileCell {  // one cell of the dense tile/top reduction
 struct BhBuffers {
     size_t na = 0, cap = 0, sort_blocks = 0, prep_blocks = 0;
     uint32_t bits = 10;              // leaf grid bits per axis the buffers are sized for
-    uint32_t *keys[3] = {}, *idx[3] = {};
+    // keys/idx [0] and [3]: the sort's input, by particle (caller's) index,
+    // alternating between steps (the walk writes the next build's into the
+    // one this build does not read); [1] and [2]: the radix passes' buffers
+    uint32_t *keys[4] = {}, *idx[4] = {};
+    int kin = 0;                     // the next build's input buffer (0 or 3)
+    int kcells = 0;                  // the last build's input buffer
+    bool walk_keys = false;          // keys[kin] was written by the last walk
     uint32_t *status = nullptr;      // [4 * sort_blocks * 256] look-back words per pass
     uint32_t *digit_total = nullptr; // [2][kDigitCopies][4 * 256] per-pass digit totals, two halves
     uint32_t parity = 0;             // the half the next build counts into
@@ -94,7 +102,6 @@ struct BhBuffers {
     int8_t *sdep = nullptr;          // [Na] shared depth with the previous sorted key
     uint32_t *cpos = nullptr;        // [Na] cell nodes starting before sorted position k
     float4 *prod = nullptr;          // [Na] (m*x, m*y, m, -) in sorted order
-    float4 *rec = nullptr;           // [Na] (x, y, m, -) in particle order, for emit's gather
     uint32_t *leaf_first = nullptr;  // [Na+1]
     uint32_t *leaf_key = nullptr;    // [Na]
     float4 *leaf_sum = nullptr;      // [Na] (sx, sy, sm, n as bits)
@@ -195,6 +202,11 @@ struct gsim_sim {
     float2 *pos[2] = {}, *vel[2] = {};
     float *mass = nullptr;
     int cur = 0;  // index of the input set
+    // Barnes-Hut keeps the state in the last step's sorted order: perm[k][slot]
+    // is the caller's index of the particle in slot of set k, and mass_next
+    // receives the next step's masses (swapped with mass after each step)
+    uint32_t *perm[2] = {};
+    float *mass_next = nullptr;
     // direct sum
     float2 *partial = nullptr;
     size_t partial_cap = 0;            // float2 entries allocated
@@ -228,10 +240,11 @@ hipError_t ds_launch_combine(gsim_sim *s, const float2 *pos, const float2 *vel,
 size_t bh_capacity(size_t na, uint32_t bits);
 hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits);
 void bh_free(BhBuffers *b);
-hipError_t bh_launch_build(gsim_sim *s, float2 *pos, float *mass, float2 *pos_out,
-                           const float2 *vel, float2 *vel_out);
+hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
+                           const uint32_t *perm_in, float2 *pos_out, const float2 *vel,
+                           float2 *vel_out, float *mass_out, uint32_t *perm_out);
 hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
-                              float2 *pos_out, float2 *vel_out);
+                              float2 *pos_out, float2 *vel_out, const uint32_t *perm_out);
 hipError_t bh_launch_morton30(gsim_sim *s, const float2 *pos, uint32_t *out);
 
 }  // namespace gsim
