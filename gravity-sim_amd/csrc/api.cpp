@@ -464,6 +464,10 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
             if (value > 1000) return fail(GSIM_ERROR_INVALID_ARGUMENT, "split threshold must be in [0, 1000]");
             s->trav_split = value;
             return GSIM_OK;
+        case GSIM_TUNE_BH_ORDER:
+            if (value > gsim::kOrderMaxBelow) return fail(GSIM_ERROR_INVALID_ARGUMENT, "order threshold too large");
+            s->trav_order = value;
+            return GSIM_OK;
         case GSIM_TUNE_BH_XCD_ORDER:
             if (value > 4096)
                 return fail(GSIM_ERROR_INVALID_ARGUMENT, "XCD run length must be in [0, 4096]");
@@ -541,8 +545,11 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
         case GSIM_DUMP_CELLS: {
             *written = sizeof(uint32_t) * na;
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
-            GSIM_HIP(hipMemcpy(buf, s->bh.keys[s->bh.kcells], *written, hipMemcpyDeviceToHost));
+            std::vector<uint32_t> pairs(2 * na);  // {cell, slot} by particle index
+            GSIM_HIP(hipMemcpy(pairs.data(), s->bh.cells[s->bh.kcells], sizeof(uint32_t) * 2 * na,
+                               hipMemcpyDeviceToHost));
             uint32_t *u = (uint32_t *)buf;
+            for (size_t i = 0; i < na; ++i) u[i] = pairs[2 * i];
             for (size_t i = 0; i < na; ++i)
                 if (u[i] >= (1u << (2 * s->bh.bits))) u[i] = 0xffffffffu;
             return GSIM_OK;
@@ -560,10 +567,10 @@ gsim_status gsim_debug_dump(gsim_sim *s, int32_t what, void *buf, size_t bytes, 
             if (!buf || bytes < *written) return fail(GSIM_ERROR_CAPACITY, "buffer too small");
             // positions the last step read (its input set)
             const int in = s->steps ? (s->cur ^ 1) : s->cur;
-            GSIM_HIP(gsim::bh_launch_morton30(s, s->pos[in], s->bh.keys[2]));
+            GSIM_HIP(gsim::bh_launch_morton30(s, s->pos[in], s->bh.keys[1]));
             GSIM_HIP(hipStreamSynchronize(s->stream));
             std::vector<uint32_t> perm(na), code(na);
-            GSIM_HIP(hipMemcpy(code.data(), s->bh.keys[2], *written, hipMemcpyDeviceToHost));
+            GSIM_HIP(hipMemcpy(code.data(), s->bh.keys[1], *written, hipMemcpyDeviceToHost));
             GSIM_HIP(hipMemcpy(perm.data(), s->perm[in], sizeof(uint32_t) * na, hipMemcpyDeviceToHost));
             uint32_t *u = (uint32_t *)buf;
             for (size_t slot = 0; slot < na; ++slot) u[perm[slot]] = code[slot];

@@ -124,17 +124,17 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
 // or at allocation), while digit_next, the other half, is zeroed for the next
 // build; the radix passes' look-back words are zeroed as well.
 // The particles are in the last step's sorted order: slot i holds the
-// particle of original index perm[i].  The keys and indices are written at
-// the original index (key[perm[i]] = cell, idx[perm[i]] = i), so the sort
-// runs over original indices, whose order breaks ties in a leaf, and carries
-// each particle's slot.  An out-of-domain particle gets the killed key; its
+// particle of original index perm[i].  Cells and slots are written at the
+// original index (cells[perm[i]] = {cell, i}), so the sort runs over original
+// indices, whose order breaks ties in a leaf, and carries each particle's
+// slot.  An out-of-domain particle gets the killed key; its
 // kill (InitShader.comp:110-117) is written by the emit kernel at the slot it
 // sorts to.  kWalk: the last walk (write_next_key) and emit already wrote the
-// keys and indices; only the digits are counted.
+// cells; only the digits are counted.
 template <bool kBox, bool kDeep, int kR, bool kWalk>
 __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     const float2 *__restrict__ pos, const uint32_t *__restrict__ perm, uint32_t na,
-    uint32_t *__restrict__ keys, uint32_t *__restrict__ idx, uint32_t *__restrict__ tile_range,
+    uint2 *__restrict__ cells, uint32_t *__restrict__ tile_range,
     uint32_t *__restrict__ digit_total, uint32_t *__restrict__ digit_next,
     uint32_t *__restrict__ status, uint32_t status_words, BhCounters *__restrict__ ctr,
     const float *__restrict__ box) {
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
 #pragma unroll
         for (int r = 0; r < kR; ++r) {
             const uint32_t i = base + r * kSortThreads + t;
-            c[r] = i < na ? keys[i] : 0u;
+            c[r] = i < na ? cells[i].x : 0u;
         }
 #pragma unroll
         for (int r = 0; r < kR; ++r)
@@ -183,8 +183,7 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
         if (kDeep) c = kBox ? cell30_box(p.x, p.y, box[0], box[1], box[2]) : cell30(p.x, p.y);
         else c = kBox ? cell20_box(p.x, p.y, box[0], box[1], box[2]) : cell20(p.x, p.y);
         if (c == 0xffffffffu) c = kDeep ? kKilledKeyDeep : kKilledKey;
-        keys[orig[r]] = c;
-        idx[orig[r]] = i;
+        cells[orig[r]] = make_uint2(c, i);
         hist_add<kPasses, false>(h, c, true);
     }
     }
@@ -207,8 +206,9 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
 template <int kR>
 __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
     const uint32_t *__restrict__ keys_in, const uint32_t *__restrict__ idx_in,
-    uint32_t *__restrict__ keys_out, uint32_t *__restrict__ idx_out, uint32_t n,
-    uint32_t shift, const uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status,
+    const uint2 *__restrict__ cells_in, uint32_t *__restrict__ keys_out,
+    uint32_t *__restrict__ idx_out, uint32_t n, uint32_t shift,
+    const uint32_t *__restrict__ digit_total, uint32_t *__restrict__ status,
     uint32_t *__restrict__ stall) {
     __shared__ SortShared<false, kR> sh;
     sort_tile_clear(sh);  // the tile's first barrier follows its ranking
@@ -217,7 +217,7 @@ __global__ __launch_bounds__(kSortThreads) void radix_onesweep_kernel(
     // once, so every wait ends (and each is bounded besides).
     sort_tile<false, kR, true>(keys_in, idx_in, keys_out, idx_out, n, shift, blockIdx.x, 0u,
                                status, stall, sh, gridDim.x, status + (size_t)gridDim.x * 256,
-                               digit_total, kDigitCopies, 1024u);
+                               digit_total, kDigitCopies, 1024u, cells_in);
 }
 
 // ---- 3./4. slot scan and emission --------------------------------------------
@@ -286,8 +286,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
     const float2 *__restrict__ pos, const float *__restrict__ mass,
     const float2 *__restrict__ vel, const uint32_t *__restrict__ perm_in,
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float *__restrict__ mass_out,
-    uint32_t *__restrict__ perm_out, uint32_t *__restrict__ nkeys, uint32_t *__restrict__ nidx,
-    uint32_t na, int bits,
+    uint32_t *__restrict__ perm_out, uint2 *__restrict__ ncells, uint32_t na, int bits,
     const uint32_t *__restrict__ block_cells, const uint32_t *__restrict__ block_leaves,
     int8_t *__restrict__ sdep, uint32_t *__restrict__ cpos, float4 *__restrict__ prod,
     uint32_t *__restrict__ leaf_first, uint32_t *__restrict__ leaf_key,
@@ -361,10 +360,8 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         pos_out[k] = make_float2(kk, kk);
         vel_out[k] = vel[i];
         mass_out[k] = 0.0f;
-        if (nkeys) {  // the next build's key, where the walk writes the others
-            nkeys[orig] = bits == (int)kDeepBits ? kKilledKeyDeep : kKilledKey;
-            nidx[orig] = k;
-        }
+        if (ncells)  // the next build's cell, where the walk writes the others
+            ncells[orig] = make_uint2(bits == (int)kDeepBits ? kKilledKeyDeep : kKilledKey, k);
         sorted_src[k] = na;  // ClearShader.comp:81-82 sentinel
         if (k == 0) {        // nothing inside the domain
             ctr->n_in = 0;
@@ -767,21 +764,19 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
 // the visited and accepted node sets are the reference's.
 // The next build's sort input, written by the walk on the fixed domain: the
 // cell of the particle's new position (InitShader.comp:73-119), at the
-// particle's caller index o = perm[slot] (keys[o] = cell, idx[o] = slot), so
-// the next build reads cells instead of positions.  keys == nullptr: the next
+// particle's caller index o = perm[slot] (cells[o] = {cell, slot}), so the
+// next build reads cells instead of positions.  cells == nullptr: the next
 // build computes the cells itself (adaptive box, first step).
 struct NextKeys {
     const uint32_t *perm;  // this step's output order: slot -> caller's index
-    uint32_t *keys, *idx;
+    uint2 *cells;
     uint32_t deep;  // 30-bit cells
 };
 
 __device__ __forceinline__ void write_next_key(const NextKeys &nk, uint32_t slot, float2 p) {
     uint32_t c = nk.deep ? cell30(p.x, p.y) : cell20(p.x, p.y);
     if (c == 0xffffffffu) c = nk.deep ? kKilledKeyDeep : kKilledKey;
-    const uint32_t o = nk.perm[slot];
-    nk.keys[o] = c;
-    nk.idx[o] = slot;
+    nk.cells[nk.perm[slot]] = make_uint2(c, slot);
 }
 
 // One group's walk (all four partials) and integrate.
@@ -828,7 +823,7 @@ __device__ __forceinline__ void walk_group(
         const float2 np = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
         pos_out[dst] = np;
         vel_out[dst] = make_float2(nvx, nvy);
-        if (nk.keys) write_next_key(nk, dst, np);
+        if (nk.cells) write_next_key(nk, dst, np);
         if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
     }
     if (gcost && lane == 0) gcost[g] = cost;
@@ -913,6 +908,70 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
         __syncthreads();  // wsum is rewritten by the next chunk
     }
     if (t == 0) ctr->n_heavy = min(carry, cap);
+}
+
+// The W = 32 launch order: by last step's walk cost (the group's real-lane
+// visits, gvis), heaviest first, in buckets of a quarter of the mean (the
+// last bucket holds everything from 7.75x the mean up) and by index within a
+// bucket, so the dense-core groups start in the launch's first round and the
+// light ones fill its end.  No costs yet (first step): index order.  Which
+// wave walks a group never changes a result.  One workgroup.
+constexpr uint32_t kOrderBuckets = 32;
+__global__ __launch_bounds__(1024) void bh_order_plan_kernel(
+    const unsigned long long *__restrict__ gvis, uint32_t groups, uint32_t below,
+    uint32_t *__restrict__ order) {
+    __shared__ unsigned long long total_sh;
+    __shared__ uint32_t base[kOrderBuckets];
+    __shared__ uint32_t wcnt[16][kOrderBuckets];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    unsigned long long part = 0;
+    for (uint32_t g = t; g < groups; g += 1024) part += gvis[g];
+    if (t == 0) total_sh = 0;
+    if (t < kOrderBuckets) base[t] = 0;
+    __syncthreads();
+    atomicAdd(&total_sh, part);
+    __syncthreads();
+    const unsigned long long total = total_sh;
+    auto bucket = [&](uint32_t g) -> uint32_t {
+        if (total == 0) return 0u;
+        const unsigned long long q = 4ull * gvis[g] * groups / total;
+        if (q < below) return 0u;  // the light groups, in index order after the rest
+        return q >= kOrderBuckets - 1 ? kOrderBuckets - 1 : (uint32_t)q;
+    };
+    for (uint32_t g = t; g < groups; g += 1024) atomicAdd(&base[bucket(g)], 1u);
+    __syncthreads();
+    if (t == 0) {  // bucket starts, heaviest bucket first
+        uint32_t run = 0;
+        for (int b = (int)kOrderBuckets - 1; b >= 0; --b) {
+            const uint32_t c = base[b];
+            base[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t start = 0; start < groups; start += 1024) {
+        const uint32_t g = start + t;
+        const uint32_t q = g < groups ? bucket(g) : kOrderBuckets;
+        uint32_t rank = 0;
+        for (uint32_t b = 0; b < kOrderBuckets; ++b) {
+            const unsigned long long m = __ballot(q == b);
+            if (q == b) rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wcnt[wave][b] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (q < kOrderBuckets) {
+            uint32_t pos = base[q] + rank;
+            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][q];
+            order[pos] = g;
+        }
+        __syncthreads();
+        if (t < kOrderBuckets) {
+            uint32_t c = 0;
+            for (uint32_t w = 0; w < 16; ++w) c += wcnt[w][t];
+            base[t] += c;
+        }
+        __syncthreads();
+    }
 }
 
 // The heavy list, one workgroup of four waves per group, beside the main
@@ -1024,7 +1083,7 @@ __global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
         const float2 np = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
         pos_out[dst] = np;
         vel_out[dst] = make_float2(nvx, nvy);
-        if (nk.keys) write_next_key(nk, dst, np);
+        if (nk.cells) write_next_key(nk, dst, np);
         if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
     }
 }
@@ -1195,7 +1254,7 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
             const float2 np = make_float2(px + (vv.x + nvx) * kp, py + (vv.y + nvy) * kp);
             pos_out[dst] = np;
             vel_out[dst] = make_float2(nvx, nvy);
-            if (nk.keys) write_next_key(nk, dst, np);
+            if (nk.cells) write_next_key(nk, dst, np);
             if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
         }
     }
@@ -1257,13 +1316,15 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     float2 *__restrict__ pos_out, float2 *__restrict__ vel_out, float2 *__restrict__ acc_dbg,
     uint32_t na, uint32_t cap, float eps2, float theta2, float kv, float kp,
     unsigned long long *__restrict__ gvis, unsigned long long *__restrict__ gacc, uint32_t xcd,
-    NextKeys nk) {
+    NextKeys nk, const uint32_t *__restrict__ order) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const uint32_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
-    const uint32_t g = b * 4 + w;
-    if (g >= na / 32) return;  // whole wave beyond the last group
+    const uint32_t u = b * 4 + w;
+    if (u >= na / 32) return;  // whole wave beyond the last group
+    // order (na / 32 entries, a permutation of the groups) or index order
+    const uint32_t g = order ? __builtin_amdgcn_readfirstlane(order[u]) : u;
     // node count is bounded by the node arrays; every step advances the walk
     const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
     const uint32_t dst = g * 32 + (lane & 31u);  // this particle's slot in the next order
@@ -1300,7 +1361,7 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
         const float2 np = make_float2(px + (v.x + nvx) * kp, py + (v.y + nvy) * kp);
         pos_out[dst] = np;
         vel_out[dst] = make_float2(nvx, nvy);
-        if (nk.keys) write_next_key(nk, dst, np);
+        if (nk.cells) write_next_key(nk, dst, np);
         if (kDebugAccel) acc_dbg[dst] = make_float2(ax, ay);
     }
 }
@@ -1342,9 +1403,10 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     do {                                                                        \
         if (e == hipSuccess) e = hipMalloc((void **)&(ptr), sizeof(*(ptr)) * (count)); \
     } while (0)
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 2; ++k) {
         GSIM_ALLOC(b->keys[k], na);
         GSIM_ALLOC(b->idx[k], na);
+        GSIM_ALLOC(b->cells[k], na);
     }
     // four passes of tile words and group words
     GSIM_ALLOC(b->status, 4 * 256 * (b->sort_blocks + (b->sort_blocks + kLookGroup - 1) / kLookGroup));
@@ -1376,6 +1438,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->gcost, na / 64);
     GSIM_ALLOC(b->gchunk, na / 64 + 1);
     GSIM_ALLOC(b->plan, kMaxHeavy);
+    GSIM_ALLOC(b->order, na / 32 + 1);
     GSIM_ALLOC(b->heavy, na / 64 + 1);
     GSIM_ALLOC(b->gvis, na / 32 + 1);  // one slot per group of W = 32 or 64
     GSIM_ALLOC(b->gacc, na / 32 + 1);
@@ -1393,14 +1456,15 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
 }
 
 void bh_free(BhBuffers *b) {
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 2; ++k) {
         (void)hipFree(b->keys[k]);
         (void)hipFree(b->idx[k]);
+        (void)hipFree(b->cells[k]);
     }
     void *ptrs[] = {b->status, b->digit_total, b->breaker, b->block_cells, b->block_leaves, b->sdep, b->cpos, b->prod,
                     b->leaf_first, b->leaf_key, b->leaf_sum, b->tile_range, b->top,
                     b->node4, b->ncount, b->nsum, b->gran_first, b->gran_key, b->gran_y,
-                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->plan, b->heavy, b->gpre, b->gvis, b->gacc, b->box,
+                    b->block_gran, b->sorted_src, b->ctr, b->gcost, b->gchunk, b->plan, b->order, b->heavy, b->gpre, b->gvis, b->gacc, b->box,
                     b->box_part};
     for (void *p : ptrs) (void)hipFree(p);
     *b = BhBuffers{};
@@ -1410,9 +1474,8 @@ void bh_free(BhBuffers *b) {
 // build does not read), or none when the next build must compute them.
 static NextKeys next_keys(gsim_sim *s, const uint32_t *perm_out) {
     BhBuffers &b = s->bh;
-    if (s->bh_box || !perm_out) return NextKeys{nullptr, nullptr, nullptr, 0u};
-    const int o = b.kin ^ 3;
-    return NextKeys{perm_out, b.keys[o], b.idx[o], b.bits == kDeepBits ? 1u : 0u};
+    if (s->bh_box || !perm_out) return NextKeys{nullptr, nullptr, 0u};
+    return NextKeys{perm_out, b.cells[b.kin ^ 1], b.bits == kDeepBits ? 1u : 0u};
 }
 
 hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
@@ -1447,30 +1510,31 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
     b.kcells = kin;
 #define GSIM_KEYS(B, D, W)                                                                    \
     bh_keys_kernel<B, D, kSortRounds, W><<<kblk, kSortThreads, 0, s->stream>>>(                \
-        pos, perm_in, na, b.keys[kin], b.idx[kin], b.tile_range, totals, totals_next,         \
-        b.status, status_words, b.ctr, box)
+        pos, perm_in, na, b.cells[kin], b.tile_range, totals, totals_next, b.status,          \
+        status_words, b.ctr, box)
     if (box) { if (deep) GSIM_KEYS(true, true, false); else GSIM_KEYS(true, false, false); }
     else if (walk) { if (deep) GSIM_KEYS(false, true, true); else GSIM_KEYS(false, false, true); }
     else { if (deep) GSIM_KEYS(false, true, false); else GSIM_KEYS(false, false, false); }
 #undef GSIM_KEYS
-    // 20-bit keys: keys[kin] -> keys[1] -> keys[2] -> keys[1]; 30-bit keys take
-    // a fourth pass -> keys[2].  keys[kin] keeps the per-particle cells.
-    const int in_of[4] = {kin, 1, 2, 1}, out_of[4] = {1, 2, 1, 2};
+    // 20-bit keys: cells[kin] -> keys[0] -> keys[1] -> keys[0]; 30-bit keys take
+    // a fourth pass -> keys[1].  cells[kin] keeps the per-particle cells.
+    const int in_of[4] = {-1, 0, 1, 0}, out_of[4] = {0, 1, 0, 1};
     for (int pass = 0; pass < passes; ++pass) {
         const int i = in_of[pass], o = out_of[pass];
         uint32_t *const st = b.status + (size_t)pass * pass_words;
         radix_onesweep_kernel<kSortRounds><<<nblk, kSortThreads, 0, s->stream>>>(
-            b.keys[i], b.idx[i], b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
+            i < 0 ? nullptr : b.keys[i], i < 0 ? nullptr : b.idx[i], i < 0 ? b.cells[kin] : nullptr,
+            b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
             &b.ctr->sort_stall);
     }
-    b.sorted_slot = deep ? 2 : 1;
+    b.sorted_slot = deep ? 1 : 0;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
     const uint32_t pb = (uint32_t)b.prep_blocks;
     bh_prep_kernel<<<pb, 256, 0, s->stream>>>(sk, na, bits, b.block_cells, b.block_leaves,
                                               deep ? b.block_gran : nullptr);
     const NextKeys nk = next_keys(s, perm_out);
     bh_emit_kernel<<<pb, 256, 0, s->stream>>>(sk, si, pos, mass, vel, perm_in, pos_out, vel_out,
-                                              mass_out, perm_out, nk.keys, nk.idx, na, bits,
+                                              mass_out, perm_out, nk.cells, na, bits,
                                               b.block_cells,
                                               b.block_leaves, b.sdep, b.cpos, b.prod,
                                               b.leaf_first, b.leaf_key, b.node4, b.ncount,
@@ -1508,8 +1572,8 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     const uint32_t grid = (na + 255) / 256;
     const NextKeys nk = next_keys(s, perm_out);
     // the next build reads the cells this walk writes
-    b.walk_keys = nk.keys != nullptr;
-    if (b.walk_keys) b.kin ^= 3;
+    b.walk_keys = nk.cells != nullptr;
+    if (b.walk_keys) b.kin ^= 1;
     const bool dbg = (s->debug_flags & GSIM_DEBUG_ACCEL) != 0;
     float2 *acc = dbg ? s->accel : nullptr;
     if (s->W == 64) {
@@ -1578,16 +1642,22 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         }
     } else {
         const uint32_t g32 = (na / 32 + 3) / 4;  // four 32-particle groups per workgroup
+        const uint32_t *order = nullptr;
+        if (s->trav_order) {
+            bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, na / 32, s->trav_order,
+                                                            b.order);
+            order = b.order;
+        }
         if (dbg)
             bh_traverse32_kernel<true><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
                 (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order,
-                nk);
+                nk, order);
         else
             bh_traverse32_kernel<false><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,
                 (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, s->xcd_order,
-                nk);
+                nk, order);
     }
     return hipGetLastError();
 }

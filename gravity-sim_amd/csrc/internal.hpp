@@ -7,7 +7,8 @@
 //   direct sum      one-sided: partial float2[S*Na] per-slice partial accelerations;
 //                   symmetric: row partials float2[tiles*16*64] then column
 //                   partials float2[(sum over groups of B - 16g)*64]
-//   Barnes-Hut      keys/idx u32[Na] x3, radix histograms, sdep i8[Na+1],
+//   Barnes-Hut      cells uint2[Na] x2 (the sort's input {cell, slot} by particle
+//                   index), keys/idx u32[Na] x2, radix histograms, sdep i8[Na+1],
 //                   cpos u32[Na], prod float4[Na], leaf records,
 //                   tile results,
 //                   breaker u32[cap], node4 float4[cap] {x, y, mass, radius}
@@ -31,6 +32,7 @@ namespace gsim {
 constexpr float kSimSize = 500.0f;
 constexpr uint32_t kTreeSize = 1024;
 constexpr uint32_t kTreeDepth = 10;
+constexpr uint32_t kOrderMaxBelow = 32;  // GSIM_TUNE_BH_ORDER: largest threshold, quarter-means
 constexpr uint32_t kKilledKey = 1u << 20;  // sorts after every 20-bit cell
 // 15-bit leaves (opt-in, beyond the reference): a 32768^2 leaf grid, depth 15,
 // 30-bit keys sorted in four 8-bit passes.
@@ -86,13 +88,14 @@ struct TileCell {  // one cell of the dense tile/top reduction
 struct BhBuffers {
     size_t na = 0, cap = 0, sort_blocks = 0, prep_blocks = 0;
     uint32_t bits = 10;              // leaf grid bits per axis the buffers are sized for
-    // keys/idx [0] and [3]: the sort's input, by particle (caller's) index,
+    // cells: the sort's input {cell, slot} by particle (caller's) index,
     // alternating between steps (the walk writes the next build's into the
-    // one this build does not read); [1] and [2]: the radix passes' buffers
-    uint32_t *keys[4] = {}, *idx[4] = {};
-    int kin = 0;                     // the next build's input buffer (0 or 3)
-    int kcells = 0;                  // the last build's input buffer
-    bool walk_keys = false;          // keys[kin] was written by the last walk
+    // one this build does not read); keys/idx: the radix passes' buffers
+    uint2 *cells[2] = {};
+    uint32_t *keys[2] = {}, *idx[2] = {};
+    int kin = 0;                     // the next build's input (cells[kin])
+    int kcells = 0;                  // the last build's input
+    bool walk_keys = false;          // cells[kin] was written by the last walk
     uint32_t *status = nullptr;      // [4 * sort_blocks * 256] look-back words per pass
     uint32_t *digit_total = nullptr; // [2][kDigitCopies][4 * 256] per-pass digit totals, two halves
     uint32_t parity = 0;             // the half the next build counts into
@@ -127,6 +130,7 @@ struct BhBuffers {
     uint32_t *gchunk = nullptr;      // [Na/64 + 1] chunks each group summed last step (issue priority)
     uint32_t *plan = nullptr;        // [kMaxHeavy] the heavy list
     uint8_t *heavy = nullptr;        // [Na/64 + 1] 1: on the heavy list this step
+    uint32_t *order = nullptr;       // [Na/32] W = 32 launch order, heaviest groups first
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
     // each group's real-lane node visits and acceptances in the last walk,
     // written by the group's wave and summed on a dump (one same-address
@@ -194,6 +198,8 @@ struct gsim_sim {
     uint32_t trav_split = 2;           // static: heavy list above this x mean chunk count;
                                        // balanced: split above this x mean cost (0: never)
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
+    uint32_t trav_order = 0;           // W = 32: walk the groups heaviest first (by last
+                                       // step's visits) instead of in index order
     uint32_t xcd_order = 1;            // static traversals: each XCD takes runs of this
                                        // many consecutive workgroups (xcd_block; <= 1:
                                        // round-robin)

@@ -214,7 +214,8 @@ __device__ __forceinline__ uint32_t grouped_prefix(uint32_t *st, uint32_t *gst, 
 // pass's tile count; gstatus: its [groups][256] words, zeroed); with totals
 // (this pass's digit totals, split over `copies` copies `copy_stride` words
 // apart) gbase is not given but computed here, the totals loaded ahead of the
-// keys and scanned once the tile is ranked.
+// keys and scanned once the tile is ranked.  packed_in: the input as {key,
+// index} pairs instead of keys_in/idx_in.
 template <bool kStage, int kR, bool kGrouped = false>
 __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           const uint32_t *__restrict__ idx_in,
@@ -226,7 +227,8 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
                                           SortShared<kStage, kR> &sh, uint32_t tiles = 0,
                                           uint32_t *__restrict__ gstatus = nullptr,
                                           const uint32_t *__restrict__ totals = nullptr,
-                                          uint32_t copies = 0, uint32_t copy_stride = 0) {
+                                          uint32_t copies = 0, uint32_t copy_stride = 0,
+                                          const uint2 *__restrict__ packed_in = nullptr) {
     constexpr int kRows = SortShared<kStage, kR>::kRows;
     const uint32_t t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const uint32_t base = tile * SortShared<kStage, kR>::kTile;
@@ -237,8 +239,14 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
         const uint32_t i = base + r * kSortThreads + t;
-        key[r] = i < n ? keys_in[i] : 0u;
-        val[r] = i < n ? idx_in[i] : 0u;
+        if (packed_in) {
+            const uint2 kv = i < n ? packed_in[i] : make_uint2(0u, 0u);
+            key[r] = kv.x;
+            val[r] = kv.y;
+        } else {
+            key[r] = i < n ? keys_in[i] : 0u;
+            val[r] = i < n ? idx_in[i] : 0u;
+        }
     }
     const uint64_t below = (1ull << lane) - 1ull;
 #pragma unroll

@@ -51,6 +51,7 @@ TUNE_DS_SYMMETRIC, TUNE_DS_SYM_COLS, TUNE_BH_SEGMENTS, TUNE_BH_SPLIT = 4, 5, 6, 
 TUNE_BH_ADAPTIVE_BOX = 8
 TUNE_BH_LEAF_BITS = 9
 TUNE_BH_XCD_ORDER = 10
+TUNE_BH_ORDER = 11
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "OUT_OF_MEMORY", 3: "DEVICE",
           4: "NO_DEVICE", 5: "IO", 6: "CAPACITY"}

@@ -200,7 +200,7 @@ typedef enum gsim_tuning {
                                          the reference's.  Combines with the adaptive
                                          box.  Reallocates the node arrays (synchronous;
                                          they grow by up to 2.5 x Na entries).          */
-    GSIM_TUNE_BH_XCD_ORDER = 10       /* 0..4096: the static traversals hand each of
+    GSIM_TUNE_BH_XCD_ORDER = 10,      /* 0..4096: the static traversals hand each of
                                          the 8 XCDs (each with its own L2) runs of this
                                          many consecutive workgroups instead of every
                                          eighth one, so spatially adjacent groups share
@@ -209,6 +209,9 @@ typedef enum gsim_tuning {
                                          round-robin, the default: measured at 500k,
                                          runs of 2-32 change nothing and contiguous
                                          ranges are 16% slower.  W = 32 and 64.         */
+    GSIM_TUNE_BH_ORDER = 11           /* 0/1: W = 32 walks the groups heaviest first
+                                         (by the last step's visits) instead of in
+                                         index order                                    */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 
