@@ -71,6 +71,8 @@ void free_sim(gsim_sim *s) {
     for (auto &p : s->pending)
         for (hipEvent_t e : p.e) s->event_pool.push_back(e);
     for (hipEvent_t e : s->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : s->order_ev)
+        if (e) (void)hipEventDestroy(e);
     if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
     if (s->ev_join) (void)hipEventDestroy(s->ev_join);
     if (s->side) (void)hipStreamDestroy(s->side);
@@ -139,6 +141,46 @@ gsim_status ds_configure(gsim_sim *s, bool sym) {
     return ds_reserve_partial(s, s->na * s->ds_S);
 }
 
+// GSIM_TUNE_BH_ORDER = 2: whether this step's walk is a trial walk (and with
+// which order); otherwise the walk keeps the order picked last.
+bool order_trial_step(gsim_sim *s) {
+    if (s->order_mode != 2 || s->W != 32) return false;
+    if (s->order_wait || s->steps < s->order_next || s->order_rec >= gsim::kOrderTrial) return false;
+    if (!s->order_ev[0])
+        for (hipEvent_t &e : s->order_ev)
+            if (hipEventCreate(&e) != hipSuccess) {
+                (void)hipGetLastError();
+                s->order_mode = 0;  // no events: index order
+                s->trav_order = 0;
+                return false;
+            }
+    s->trav_order = (s->order_rec & 1u) ? 0u : 1u;  // heaviest first, index, ...
+    return true;
+}
+
+// Once the trial's last walk has completed (never waiting for it), keeps the
+// faster order: heaviest first only if its walks took under 97% of the index
+// order's.
+void order_poll(gsim_sim *s) {
+    if (!s->order_wait) return;
+    const hipEvent_t last = s->order_ev[2 * gsim::kOrderTrial - 1];
+    const hipError_t q = hipEventQuery(last);
+    (void)hipGetLastError();  // not ready is no error of the step
+    if (q == hipErrorNotReady) return;
+    float ms[2] = {0.0f, 0.0f};
+    bool ok = q == hipSuccess;
+    for (uint32_t i = 0; ok && i < gsim::kOrderTrial; ++i) {
+        float t = 0.0f;
+        ok = hipEventElapsedTime(&t, s->order_ev[2 * i], s->order_ev[2 * i + 1]) == hipSuccess;
+        ms[(i & 1u) ? 0 : 1] += t;
+    }
+    (void)hipGetLastError();
+    s->trav_order = ok && ms[1] < 0.97f * ms[0] ? 1u : 0u;
+    s->order_wait = false;
+    s->order_rec = 0;
+    s->order_next = s->steps + gsim::kOrderPeriod;
+}
+
 gsim_status run_steps(gsim_sim *s, uint32_t count) {
     const bool bh = s->params.algorithm == GSIM_BARNES_HUT;
     for (uint32_t k = 0; k < count; ++k) {
@@ -158,8 +200,16 @@ gsim_status run_steps(gsim_sim *s, uint32_t count) {
                                            s->vel[in], s->vel[out], s->mass_next,
                                            s->perm[out]));
             if (s->timing) GSIM_HIP(hipEventRecord(ev[1], s->stream));
+            const bool trial = order_trial_step(s);
+            if (trial) GSIM_HIP(hipEventRecord(s->order_ev[2 * s->order_rec], s->stream));
             GSIM_HIP(gsim::bh_launch_traverse(s, s->pos[in], s->vel[in], s->pos[out],
                                               s->vel[out], s->perm[out]));
+            if (trial) {
+                GSIM_HIP(hipEventRecord(s->order_ev[2 * s->order_rec + 1], s->stream));
+                if (++s->order_rec == gsim::kOrderTrial) s->order_wait = true;
+            } else {
+                order_poll(s);
+            }
             std::swap(s->mass, s->mass_next);
             s->has_tree = true;
         } else {
@@ -465,8 +515,12 @@ gsim_status gsim_set_tuning(gsim_sim *s, int32_t knob, uint32_t value) {
             s->trav_split = value;
             return GSIM_OK;
         case GSIM_TUNE_BH_ORDER:
-            if (value > gsim::kOrderMaxBelow) return fail(GSIM_ERROR_INVALID_ARGUMENT, "order threshold too large");
-            s->trav_order = value;
+            if (value > 2) return fail(GSIM_ERROR_INVALID_ARGUMENT, "group order must be 0, 1 or 2");
+            s->order_mode = value;
+            s->trav_order = value == 1 ? 1u : 0u;
+            s->order_rec = 0;
+            s->order_wait = false;
+            s->order_next = s->steps + 1;
             return GSIM_OK;
         case GSIM_TUNE_BH_XCD_ORDER:
             if (value > 4096)

@@ -918,8 +918,7 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
 // wave walks a group never changes a result.  One workgroup.
 constexpr uint32_t kOrderBuckets = 32;
 __global__ __launch_bounds__(1024) void bh_order_plan_kernel(
-    const unsigned long long *__restrict__ gvis, uint32_t groups, uint32_t below,
-    uint32_t *__restrict__ order) {
+    const unsigned long long *__restrict__ gvis, uint32_t groups, uint32_t *__restrict__ order) {
     __shared__ unsigned long long total_sh;
     __shared__ uint32_t base[kOrderBuckets];
     __shared__ uint32_t wcnt[16][kOrderBuckets];
@@ -935,7 +934,6 @@ __global__ __launch_bounds__(1024) void bh_order_plan_kernel(
     auto bucket = [&](uint32_t g) -> uint32_t {
         if (total == 0) return 0u;
         const unsigned long long q = 4ull * gvis[g] * groups / total;
-        if (q < below) return 0u;  // the light groups, in index order after the rest
         return q >= kOrderBuckets - 1 ? kOrderBuckets - 1 : (uint32_t)q;
     };
     for (uint32_t g = t; g < groups; g += 1024) atomicAdd(&base[bucket(g)], 1u);
@@ -1644,8 +1642,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         const uint32_t g32 = (na / 32 + 3) / 4;  // four 32-particle groups per workgroup
         const uint32_t *order = nullptr;
         if (s->trav_order) {
-            bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, na / 32, s->trav_order,
-                                                            b.order);
+            bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, na / 32, b.order);
             order = b.order;
         }
         if (dbg)

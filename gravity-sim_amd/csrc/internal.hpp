@@ -32,7 +32,10 @@ namespace gsim {
 constexpr float kSimSize = 500.0f;
 constexpr uint32_t kTreeSize = 1024;
 constexpr uint32_t kTreeDepth = 10;
-constexpr uint32_t kOrderMaxBelow = 32;  // GSIM_TUNE_BH_ORDER: largest threshold, quarter-means
+// GSIM_TUNE_BH_ORDER = 2 (auto): every kOrderPeriod steps, kOrderTrial
+// consecutive W = 32 walks alternate heaviest-first and index order
+constexpr uint32_t kOrderTrial = 4;
+constexpr uint64_t kOrderPeriod = 1024;
 constexpr uint32_t kKilledKey = 1u << 20;  // sorts after every 20-bit cell
 // 15-bit leaves (opt-in, beyond the reference): a 32768^2 leaf grid, depth 15,
 // 30-bit keys sorted in four 8-bit passes.
@@ -198,8 +201,16 @@ struct gsim_sim {
     uint32_t trav_split = 2;           // static: heavy list above this x mean chunk count;
                                        // balanced: split above this x mean cost (0: never)
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
-    uint32_t trav_order = 0;           // W = 32: walk the groups heaviest first (by last
-                                       // step's visits) instead of in index order
+    // W = 32 launch order: order_mode 0 index, 1 heaviest first (by last
+    // step's visits), 2 auto: a trial of kOrderTrial walks alternating the
+    // two, timed by events, keeps the faster once the host sees the trial's
+    // last event complete (polled at each later step; never waited for)
+    uint32_t order_mode = 2;
+    uint32_t trav_order = 0;           // the order the next walk uses (0/1)
+    hipEvent_t order_ev[2 * gsim::kOrderTrial] = {};
+    uint32_t order_rec = 0;            // trial walks recorded
+    bool order_wait = false;           // the trial's events are outstanding
+    uint64_t order_next = 1;           // first step of the next trial
     uint32_t xcd_order = 1;            // static traversals: each XCD takes runs of this
                                        // many consecutive workgroups (xcd_block; <= 1:
                                        // round-robin)

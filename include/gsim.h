@@ -209,9 +209,12 @@ typedef enum gsim_tuning {
                                          round-robin, the default: measured at 500k,
                                          runs of 2-32 change nothing and contiguous
                                          ranges are 16% slower.  W = 32 and 64.         */
-    GSIM_TUNE_BH_ORDER = 11           /* 0/1: W = 32 walks the groups heaviest first
-                                         (by the last step's visits) instead of in
-                                         index order                                    */
+    GSIM_TUNE_BH_ORDER = 11           /* W = 32 group launch order: 0 index order, 1
+                                         heaviest first (by the last step's visits), 2
+                                         auto (the default): every 1,024 steps, four
+                                         walks alternate the two and the faster is kept
+                                         (decided once the host sees them complete;
+                                         never waited for)                              */
 } gsim_tuning;
 gsim_status gsim_set_tuning(gsim_sim *sim, int32_t knob, uint32_t value);
 
