@@ -18,6 +18,15 @@ __device__ __forceinline__ uint32_t part1by1(uint32_t v) {
 // -ffp-contract=off, so each operation rounds as written.
 __device__ __forceinline__ float unit_coord(float p) { return ((p / kSimSize) + 1.0f) * 0.5f; }
 
+// A workgroup barrier that orders LDS only: __syncthreads() also waits for
+// the wave's outstanding global stores, which costs a store's latency at
+// every barrier that follows one.  For barriers that guard LDS data alone.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ uint32_t cell20(float x, float y) {
     const float qx = unit_coord(x) * (float)kTreeSize;
     const float qy = unit_coord(y) * (float)kTreeSize;
@@ -186,7 +195,7 @@ __device__ __forceinline__ void scan_inplace_1024(uint32_t *__restrict__ data, u
             if (lane >= (uint32_t)off) x += y;
         }
         if (lane == 63) wsum[wave] = x;
-        __syncthreads();
+        lds_barrier();
         if (wave == 0) {
             uint32_t w = lane < 16 ? wsum[lane] : 0u;
 #pragma unroll
@@ -196,7 +205,7 @@ __device__ __forceinline__ void scan_inplace_1024(uint32_t *__restrict__ data, u
             }
             if (lane < 16) wsum[lane] = w;
         }
-        __syncthreads();
+        lds_barrier();
         uint32_t run = carry + (wave ? wsum[wave - 1] : 0u) + (x - s);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -204,7 +213,7 @@ __device__ __forceinline__ void scan_inplace_1024(uint32_t *__restrict__ data, u
             run += v[u];
         }
         carry += wsum[15];
-        __syncthreads();  // wsum is rewritten by the next chunk
+        lds_barrier();  // wsum is rewritten by the next chunk
     }
 }
 
@@ -219,11 +228,11 @@ __device__ __forceinline__ uint32_t block_scan256(uint32_t x, uint32_t *total,
         if (lane >= (uint32_t)off) x += u;
     }
     if (lane == 63) wsum[wave] = x;
-    __syncthreads();
+    lds_barrier();
     uint32_t before = 0;
     for (uint32_t w = 0; w < wave; ++w) before += wsum[w];
     *total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
+    lds_barrier();
     return x + before;
 }
 

@@ -68,7 +68,7 @@ __global__ __launch_bounds__(256) void bh_box_partial_kernel(const float2 *__res
     }
     minmax4_wave(x0, x1, y0, y1);
     if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = make_float4(x0, x1, y0, y1);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         float4 r = wv[0];
         for (int w = 1; w < 4; ++w) {
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(256) void bh_box_final_kernel(const float4 *__restr
     }
     minmax4_wave(x0, x1, y0, y1);
     if ((threadIdx.x & 63) == 0) wv[threadIdx.x >> 6] = make_float4(x0, x1, y0, y1);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x != 0) return;
     for (int w = 1; w < 4; ++w) {
         wv[0].x = fminf(wv[0].x, wv[w].x);
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
     }
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) h[p][t] = 0;
-    __syncthreads();
+    lds_barrier();
     const uint32_t base = blockIdx.x * (kR * kSortThreads);
     if constexpr (kWalk) {
         uint32_t c[kR];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(kSortThreads) void bh_keys_kernel(
         hist_add<kPasses, false>(h, c, true);
     }
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t *const copy = digit_total + (blockIdx.x % kDigitCopies) * 1024;
 #pragma unroll
     for (int p = 0; p < kPasses; ++p) {
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void bh_prep_kernel(const uint32_t *__restrict
         wb[wave] = b;
         wc[wave] = c;
     }
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) {
         block_cells[blockIdx.x] = wa[0] + wa[1] + wa[2] + wa[3];
         block_leaves[blockIdx.x] = wb[0] + wb[1] + wb[2] + wb[3];
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void bh_emit_kernel(
         pb[wave] = xb;
         pc[wave] = xc;
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t ea = a - q.ns + ((pa[0] + pa[1]) + (pa[2] + pa[3]));
     uint32_t eb = b - q.lf + ((pb[0] + pb[1]) + (pb[2] + pb[3]));
     uint32_t ec = (uint32_t)__popcll(gb & ((1ull << lane) - 1ull)) +
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
     const float r0 = box ? box[3] : kFixedR0;
     for (uint32_t c = t; c < 1024; c += 256)
         l10[c] = TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
-    __syncthreads();
+    lds_barrier();
     // a tile holds at most 1024 leaves (granules): four per thread, loaded
     // together; a 10-bit leaf's own cell node was written with its sum
 #pragma unroll
@@ -703,19 +703,19 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
         l10[c] = cell;
         if (leaf_y) emit_cell(cell, 10, sdep, cpos, node4, ncount, r0);
     }
-    __syncthreads();
+    lds_barrier();
     {
         const uint32_t c = t;  // 256 depth-9 cells
         up[c] = reduce4(l10[4 * c], l10[4 * c + 1], l10[4 * c + 2], l10[4 * c + 3]);
     }
-    __syncthreads();
+    lds_barrier();
     for (int d = 8; d >= 5; --d) {
         const uint32_t cnt = 1u << (2 * (d - 5));
         const int o = tile_level_offset(d), oc = tile_level_offset(d + 1);
         if (t < cnt)
             up[o + t] = reduce4(up[oc + 4 * t], up[oc + 4 * t + 1], up[oc + 4 * t + 2],
                                 up[oc + 4 * t + 3]);
-        __syncthreads();
+        lds_barrier();
     }
     for (uint32_t c = t; c < 341; c += 256) {
         const int d = c < 256 ? 9 : c < 320 ? 8 : c < 336 ? 7 : c < 340 ? 6 : 5;
@@ -726,6 +726,8 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
         __threadfence();  // top[T] before the count (release)
         last = atomicAdd(&ctr->tiles_done, 1u) + 1u == ctr->tiles_busy ? 1u : 0u;
     }
+    // a full barrier: thread 0's acquire of the other tiles' top[] is handed
+    // on to the whole workgroup
     __syncthreads();
     if (!last) return;
     __threadfence();  // every tile's top[] after the count (acquire)
@@ -739,14 +741,14 @@ __global__ __launch_bounds__(256) void bh_tile_kernel(
                     : TileCell{0u, 0u, 0xffffffffu, 0u, 0.0f, 0.0f, 0.0f, 0.0f};
     }
     up[t] = reduce4(c4[0], c4[1], c4[2], c4[3]);
-    __syncthreads();
+    lds_barrier();
     for (int d = 3; d >= 0; --d) {
         const uint32_t cnt = 1u << (2 * d);
         const int o = tile_level_offset(d + 5), oc = tile_level_offset(d + 6);
         if (t < cnt)
             up[o + t] = reduce4(up[oc + 4 * t], up[oc + 4 * t + 1], up[oc + 4 * t + 2],
                                 up[oc + 4 * t + 3]);
-        __syncthreads();
+        lds_barrier();
     }
     for (uint32_t c = t; c < 341; c += 256) {
         const int d = c < 256 ? 4 : c < 320 ? 3 : c < 336 ? 2 : c < 340 ? 1 : 0;
@@ -851,9 +853,9 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
     for (uint32_t g = t; g < groups; g += 1024) part += gchunk[g];
     if (t == 0) total_sh = 0;
     if (t < 64) hist[t] = 0;
-    __syncthreads();
+    lds_barrier();
     atomicAdd(&total_sh, part);
-    __syncthreads();
+    lds_barrier();
     const unsigned long long total = total_sh;
     const bool any = total > 0 && split_x > 0 && cap > 0;
     auto quarter = [&](uint32_t g) -> uint32_t {  // q as above, 64 below the multiple
@@ -866,7 +868,7 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
             const uint32_t q = quarter(g);
             if (q < 64) atomicAdd(&hist[q], 1u);
         }
-    __syncthreads();
+    lds_barrier();
     if (t == 0) {
         uint32_t qt = 64;  // 64: nothing listed
         if (any) {
@@ -879,7 +881,7 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
         }
         qt_sh = qt;
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t qt = qt_sh;
     uint32_t carry = 0;  // heavy groups before this chunk of 1024
     for (uint32_t base = 0; base < groups; base += 1024) {
@@ -894,7 +896,7 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
             if (lane >= (uint32_t)off) x += y;
         }
         if (lane == 63) wsum[wave] = x;
-        __syncthreads();
+        lds_barrier();
         uint32_t before = carry + (h ? x - 1u : x);
         uint32_t chunk_total = 0;
         for (uint32_t w = 0; w < 16; ++w) {
@@ -905,7 +907,7 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
         if (listed) plan[before] = g;
         if (g < groups) heavy[g] = listed ? 1u : 0u;
         carry += chunk_total;
-        __syncthreads();  // wsum is rewritten by the next chunk
+        lds_barrier();  // wsum is rewritten by the next chunk
     }
     if (t == 0) ctr->n_heavy = min(carry, cap);
 }
@@ -917,58 +919,103 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
 // light ones fill its end.  No costs yet (first step): index order.  Which
 // wave walks a group never changes a result.  One workgroup.
 constexpr uint32_t kOrderBuckets = 32;
+constexpr uint32_t kOrderAhead = 8;
+
+// The lanes of the wave whose 5-bit bucket equals this lane's (five ballots,
+// whatever the number of distinct buckets); q = kOrderBuckets: no group.
+__device__ __forceinline__ unsigned long long bucket_peers(uint32_t q) {
+    unsigned long long m = __ballot(q < kOrderBuckets);
+#pragma unroll
+    for (int bit = 0; bit < 5; ++bit) {
+        const unsigned long long b = __ballot((q >> bit) & 1u);
+        m &= ((q >> bit) & 1u) ? b : ~b;
+    }
+    return m;
+}
 __global__ __launch_bounds__(1024) void bh_order_plan_kernel(
     const unsigned long long *__restrict__ gvis, uint32_t groups, uint32_t *__restrict__ order) {
-    __shared__ unsigned long long total_sh;
+    __shared__ unsigned long long wsum[16];
     __shared__ uint32_t base[kOrderBuckets];
     __shared__ uint32_t wcnt[16][kOrderBuckets];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const unsigned long long below = (1ull << lane) - 1ull;
     unsigned long long part = 0;
-    for (uint32_t g = t; g < groups; g += 1024) part += gvis[g];
-    if (t == 0) total_sh = 0;
+    for (uint32_t g0 = t; g0 < groups; g0 += kOrderAhead * 1024) {
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderAhead; ++u) {
+            const uint32_t g = g0 + u * 1024;
+            part += g < groups ? gvis[g] : 0ull;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if (lane == 0) wsum[wave] = part;
     if (t < kOrderBuckets) base[t] = 0;
-    __syncthreads();
-    atomicAdd(&total_sh, part);
-    __syncthreads();
-    const unsigned long long total = total_sh;
+    lds_barrier();
+    unsigned long long total = 0;
+    for (uint32_t w = 0; w < 16; ++w) total += wsum[w];
+    // bucket = visits in quarters of the mean (float: any order is correct)
+    const float scale = total ? 4.0f * (float)groups / (float)total : 0.0f;
     auto bucket = [&](uint32_t g) -> uint32_t {
-        if (total == 0) return 0u;
-        const unsigned long long q = 4ull * gvis[g] * groups / total;
-        return q >= kOrderBuckets - 1 ? kOrderBuckets - 1 : (uint32_t)q;
+        if (g >= groups) return kOrderBuckets;  // no group
+        const float q = (float)gvis[g] * scale;
+        return q >= (float)(kOrderBuckets - 1) ? kOrderBuckets - 1 : (uint32_t)q;
     };
-    for (uint32_t g = t; g < groups; g += 1024) atomicAdd(&base[bucket(g)], 1u);
-    __syncthreads();
+    // Each wave handles 64 consecutive groups at a time; kOrderAhead rounds'
+    // loads are issued together (the loop is otherwise one load latency per
+    // round).
+    for (uint32_t start = wave * 64; start < groups; start += kOrderAhead * 1024) {
+        uint32_t qa[kOrderAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderAhead; ++u) qa[u] = bucket(start + u * 1024 + lane);
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderAhead; ++u) {
+            const uint32_t q = qa[u];
+            const unsigned long long m = bucket_peers(q);
+            // the lowest lane of each bucket adds the bucket's lanes
+            if (q < kOrderBuckets && (m & below) == 0ull)
+                atomicAdd(&base[q], (uint32_t)__popcll(m));
+        }
+    }
+    lds_barrier();
     if (t == 0) {  // bucket starts, heaviest bucket first
         uint32_t run = 0;
-        for (int b = (int)kOrderBuckets - 1; b >= 0; --b) {
-            const uint32_t c = base[b];
-            base[b] = run;
+        for (int k = (int)kOrderBuckets - 1; k >= 0; --k) {
+            const uint32_t c = base[k];
+            base[k] = run;
             run += c;
         }
     }
-    __syncthreads();
-    for (uint32_t start = 0; start < groups; start += 1024) {
-        const uint32_t g = start + t;
-        const uint32_t q = g < groups ? bucket(g) : kOrderBuckets;
-        uint32_t rank = 0;
-        for (uint32_t b = 0; b < kOrderBuckets; ++b) {
-            const unsigned long long m = __ballot(q == b);
-            if (q == b) rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            if (lane == 0) wcnt[wave][b] = (uint32_t)__popcll(m);
+    lds_barrier();
+    for (uint32_t start0 = 0; start0 < groups; start0 += kOrderAhead * 1024) {
+        uint32_t qa[kOrderAhead];
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderAhead; ++u) qa[u] = bucket(start0 + u * 1024 + t);
+#pragma unroll
+        for (uint32_t u = 0; u < kOrderAhead; ++u) {
+            const uint32_t start = start0 + u * 1024;
+            if (start >= groups) break;  // uniform over the workgroup
+            const uint32_t q = qa[u];
+            if (lane < kOrderBuckets) wcnt[wave][lane] = 0;
+            const unsigned long long m = bucket_peers(q);
+            const uint32_t rank = (uint32_t)__popcll(m & below);
+            // the lowest lane of each bucket writes the wave's count (after
+            // the zeroing above: one wave's LDS writes complete in order)
+            if (q < kOrderBuckets && rank == 0) wcnt[wave][q] = (uint32_t)__popcll(m);
+            lds_barrier();
+            if (q < kOrderBuckets) {
+                uint32_t pos = base[q] + rank;
+                for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][q];
+                order[pos] = start + t;
+            }
+            lds_barrier();
+            if (t < kOrderBuckets) {
+                uint32_t c = 0;
+                for (uint32_t w = 0; w < 16; ++w) c += wcnt[w][t];
+                base[t] += c;
+            }
+            lds_barrier();
         }
-        __syncthreads();
-        if (q < kOrderBuckets) {
-            uint32_t pos = base[q] + rank;
-            for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][q];
-            order[pos] = g;
-        }
-        __syncthreads();
-        if (t < kOrderBuckets) {
-            uint32_t c = 0;
-            for (uint32_t w = 0; w < 16; ++w) c += wcnt[w][t];
-            base[t] += c;
-        }
-        __syncthreads();
     }
 }
 
@@ -1054,18 +1101,18 @@ __global__ __launch_bounds__(256) void bh_traverse_heavy_kernel(
                 qs[2] = ind >= end ? 1u : 0u;
             }
         }
-        __syncthreads();
+        lds_barrier();
         const uint32_t qn = __builtin_amdgcn_readfirstlane(qs[0]);
         const uint32_t qch = __builtin_amdgcn_readfirstlane(qs[1]);
         const uint32_t done = __builtin_amdgcn_readfirstlane(qs[2]);
         if (qn) sum_run_queue(runq, qn, qch, stage[w], node4, px, py, eps2, 1u << w, P);
-        __syncthreads();  // runq and qs are rewritten by the next batch
+        lds_barrier();  // runq and qs are rewritten by the next batch
         if (done) break;
     }
     float ax, ay;
     split_share(P, w, ax, ay);
     share[w][lane] = make_float2(ax, ay);
-    __syncthreads();
+    lds_barrier();
     __builtin_amdgcn_s_setprio(0);
     if (w != 0) return;
     if (lane == 0) {
@@ -1099,12 +1146,12 @@ __global__ __launch_bounds__(1024) void bh_cost_scan_kernel(const uint32_t *__re
     uint32_t sum = 0;
     for (uint32_t g = lo; g < hi; ++g) sum += gcost[g] + 1u;
     part[t] = sum;
-    __syncthreads();
+    lds_barrier();
     for (uint32_t off = 1; off < 1024; off <<= 1) {
         const uint32_t v = t >= off ? part[t - off] : 0u;
-        __syncthreads();
+        lds_barrier();
         part[t] += v;
-        __syncthreads();
+        lds_barrier();
     }
     uint32_t run = t ? part[t - 1] : 0u;
     for (uint32_t g = lo; g < hi; ++g) {
@@ -1171,7 +1218,7 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
         seg[3] = 0;
     }
     if (threadIdx.x < kSplitSlots) hdone[threadIdx.x] = 0;
-    __syncthreads();
+    lds_barrier();
     const uint32_t lo = seg[0], hi = seg[1];
     const uint32_t count = hi - lo, listed = min(count, kSegList);
     const uint32_t i = threadIdx.x;  // one listed group per thread (kSegList == blockDim)
@@ -1184,13 +1231,13 @@ __global__ __launch_bounds__(1024) void bh_traverse_bal_kernel(
             split = true;
         }
     }
-    __syncthreads();
+    lds_barrier();
     const uint32_t nsplit = min(seg[3], kSplitSlots);
-    __syncthreads();
+    lds_barrier();
     if (threadIdx.x == 0) seg[3] = nsplit;
-    __syncthreads();
+    lds_barrier();
     if (i < listed && !split) list[atomicAdd(&seg[3], 1u)] = lo + i;
-    __syncthreads();
+    lds_barrier();
     const uint32_t total_nodes = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
     const uint32_t units = 4 * nsplit + (count - nsplit);
     while (true) {

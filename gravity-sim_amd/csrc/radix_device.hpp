@@ -257,7 +257,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
         rank[r] = (uint32_t)__popcll(m & below);
         if (valid && rank[r] == 0) sh.slot[(r * (kSortThreads / 64) + wave) * 256 + d] = (uint32_t)__popcll(m);
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t c = 0;  // keys of digit t before row k, then the tile's count
 #pragma unroll
     for (int k = 0; k < kRows; ++k) {
@@ -291,7 +291,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
         }
     }
     sh.adj[t] = gbase + excl - first;
-    __syncthreads();
+    lds_barrier();
     if (!kStage) {
 #pragma unroll
         for (int r = 0; r < kR; ++r) {
@@ -317,7 +317,7 @@ __device__ __forceinline__ void sort_tile(const uint32_t *__restrict__ keys_in,
             sh.stage_val[at] = val[r];
         }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int r = 0; r < kR; ++r) {
         const uint32_t at = r * kSortThreads + t;
