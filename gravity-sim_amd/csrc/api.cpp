@@ -158,8 +158,8 @@ bool order_trial_step(gsim_sim *s) {
     return true;
 }
 
-// Once the trial's last walk has completed (never waiting for it), keeps the
-// faster order: heaviest first only if its walks took under 97% of the index
+// Once the trial's last step has completed (never waiting for it), keeps the
+// faster order: heaviest first only if its steps took under 97% of the index
 // order's.
 void order_poll(gsim_sim *s) {
     if (!s->order_wait) return;
@@ -194,14 +194,16 @@ gsim_status run_steps(gsim_sim *s, uint32_t count) {
             GSIM_HIP(hipEventRecord(ev[0], s->stream));
         }
         if (bh) {
+            // the walk's group order is fixed before the build, which plans it;
+            // a trial step is timed whole (the plan runs beside the build)
+            const bool trial = order_trial_step(s);
+            if (trial) GSIM_HIP(hipEventRecord(s->order_ev[2 * s->order_rec], s->stream));
             // the build writes the next order's masses, permutation and killed
             // particles; the walk writes the rest of pos[out] and vel[out]
             GSIM_HIP(gsim::bh_launch_build(s, s->pos[in], s->mass, s->perm[in], s->pos[out],
                                            s->vel[in], s->vel[out], s->mass_next,
                                            s->perm[out]));
             if (s->timing) GSIM_HIP(hipEventRecord(ev[1], s->stream));
-            const bool trial = order_trial_step(s);
-            if (trial) GSIM_HIP(hipEventRecord(s->order_ev[2 * s->order_rec], s->stream));
             GSIM_HIP(gsim::bh_launch_traverse(s, s->pos[in], s->vel[in], s->pos[out],
                                               s->vel[out], s->perm[out]));
             if (trial) {

@@ -912,17 +912,19 @@ __global__ __launch_bounds__(1024) void bh_heavy_plan_kernel(const uint32_t *__r
     if (t == 0) ctr->n_heavy = min(carry, cap);
 }
 
-// The W = 32 launch order: by last step's walk cost (the group's real-lane
-// visits, gvis), heaviest first, in buckets of a quarter of the mean (the
-// last bucket holds everything from 7.75x the mean up) and by index within a
-// bucket, so the dense-core groups start in the launch's first round and the
-// light ones fill its end.  No costs yet (first step): index order.  Which
-// wave walks a group never changes a result.  One workgroup.
+// The W = 32 launch order, by workgroup (four consecutive groups, the launch
+// unit): by last step's walk cost (the four groups' real-lane visits, gvis),
+// heaviest first, in buckets of a quarter of the mean (the last bucket holds
+// everything from 7.75x the mean up) and by index within a bucket, so the
+// dense-core groups start in the launch's first round and the light ones
+// fill its end.  No costs yet (first step): index order.  Which wave walks a
+// group never changes a result.  One workgroup: it is bound by its own
+// instruction count, which ordering units of four groups keeps small.
 constexpr uint32_t kOrderBuckets = 32;
-constexpr uint32_t kOrderAhead = 8;
+constexpr uint32_t kOrderAhead = 4;
 
 // The lanes of the wave whose 5-bit bucket equals this lane's (five ballots,
-// whatever the number of distinct buckets); q = kOrderBuckets: no group.
+// whatever the number of distinct buckets); q = kOrderBuckets: no unit.
 __device__ __forceinline__ unsigned long long bucket_peers(uint32_t q) {
     unsigned long long m = __ballot(q < kOrderBuckets);
 #pragma unroll
@@ -932,90 +934,100 @@ __device__ __forceinline__ unsigned long long bucket_peers(uint32_t q) {
     }
     return m;
 }
+
 __global__ __launch_bounds__(1024) void bh_order_plan_kernel(
     const unsigned long long *__restrict__ gvis, uint32_t groups, uint32_t *__restrict__ order) {
     __shared__ unsigned long long wsum[16];
-    __shared__ uint32_t base[kOrderBuckets];
-    __shared__ uint32_t wcnt[16][kOrderBuckets];
+    __shared__ uint32_t base[2][kOrderBuckets];
+    __shared__ uint32_t wcnt[2][16][kOrderBuckets];
     const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t units = (groups + 3) / 4;
     const unsigned long long below = (1ull << lane) - 1ull;
-    unsigned long long part = 0;
-    for (uint32_t g0 = t; g0 < groups; g0 += kOrderAhead * 1024) {
+    // a unit's cost, clamped to 32 bits (a float conversion of one word)
+    auto unit_cost = [&](uint32_t u, unsigned long long &sum) -> uint32_t {
+        unsigned long long v[4];
 #pragma unroll
-        for (uint32_t u = 0; u < kOrderAhead; ++u) {
-            const uint32_t g = g0 + u * 1024;
-            part += g < groups ? gvis[g] : 0ull;
+        for (uint32_t k = 0; k < 4; ++k) {
+            const uint32_t g = 4 * u + k;
+            v[k] = gvis[g < groups ? g : groups - 1];  // unconditional: loads stay together
         }
-    }
+        unsigned long long c = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) c += 4 * u + k < groups ? v[k] : 0ull;
+        sum += u < units ? c : 0ull;
+        return c > 0xffffffffull ? 0xffffffffu : (uint32_t)c;
+    };
+    uint32_t cost[kOrderAhead];  // this thread's units t + 1024 i, i < kOrderAhead
+    unsigned long long part = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kOrderAhead; ++i) cost[i] = unit_cost(t + 1024 * i, part);
+    for (uint32_t u = t + 1024 * kOrderAhead; u < units; u += 1024) (void)unit_cost(u, part);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
     if (lane == 0) wsum[wave] = part;
-    if (t < kOrderBuckets) base[t] = 0;
+    if (t < kOrderBuckets) base[0][t] = 0;
     lds_barrier();
     unsigned long long total = 0;
+#pragma unroll
     for (uint32_t w = 0; w < 16; ++w) total += wsum[w];
-    // bucket = visits in quarters of the mean (float: any order is correct)
-    const float scale = total ? 4.0f * (float)groups / (float)total : 0.0f;
-    auto bucket = [&](uint32_t g) -> uint32_t {
-        if (g >= groups) return kOrderBuckets;  // no group
-        const float q = (float)gvis[g] * scale;
+    // bucket = cost in quarters of the mean unit cost (float: any order is
+    // a correct one)
+    const float scale = total ? 4.0f * (float)units / (float)total : 0.0f;
+    auto bucket_of = [&](uint32_t u, uint32_t c) -> uint32_t {
+        if (u >= units) return kOrderBuckets;  // no unit
+        const float q = (float)c * scale;
         return q >= (float)(kOrderBuckets - 1) ? kOrderBuckets - 1 : (uint32_t)q;
     };
-    // Each wave handles 64 consecutive groups at a time; kOrderAhead rounds'
-    // loads are issued together (the loop is otherwise one load latency per
-    // round).
-    for (uint32_t start = wave * 64; start < groups; start += kOrderAhead * 1024) {
-        uint32_t qa[kOrderAhead];
-#pragma unroll
-        for (uint32_t u = 0; u < kOrderAhead; ++u) qa[u] = bucket(start + u * 1024 + lane);
-#pragma unroll
-        for (uint32_t u = 0; u < kOrderAhead; ++u) {
-            const uint32_t q = qa[u];
-            const unsigned long long m = bucket_peers(q);
-            // the lowest lane of each bucket adds the bucket's lanes
-            if (q < kOrderBuckets && (m & below) == 0ull)
-                atomicAdd(&base[q], (uint32_t)__popcll(m));
-        }
+    auto bucket = [&](uint32_t u, uint32_t i) -> uint32_t {
+        if (i < kOrderAhead) return bucket_of(u, cost[i]);
+        unsigned long long dummy = 0;
+        return bucket_of(u, unit_cost(u, dummy));
+    };
+    // histogram: the lowest lane of each bucket in the wave adds its lanes
+    for (uint32_t i = 0; 1024 * i < units; ++i) {
+        const uint32_t q = bucket(t + 1024 * i, i);
+        const unsigned long long m = bucket_peers(q);
+        if (q < kOrderBuckets && (m & below) == 0ull) atomicAdd(&base[0][q], (uint32_t)__popcll(m));
     }
     lds_barrier();
     if (t == 0) {  // bucket starts, heaviest bucket first
         uint32_t run = 0;
         for (int k = (int)kOrderBuckets - 1; k >= 0; --k) {
-            const uint32_t c = base[k];
-            base[k] = run;
+            const uint32_t c = base[0][k];
+            base[0][k] = run;
             run += c;
         }
     }
     lds_barrier();
-    for (uint32_t start0 = 0; start0 < groups; start0 += kOrderAhead * 1024) {
-        uint32_t qa[kOrderAhead];
+    // placement, 1024 units a round; wcnt and base alternate by round, so
+    // two barriers a round suffice
+    for (uint32_t i = 0; 1024 * i < units; ++i) {
+        const uint32_t p = i & 1u;
+        const uint32_t u = t + 1024 * i;
+        const uint32_t q = bucket(u, i);
+        if (lane < kOrderBuckets) wcnt[p][wave][lane] = 0;
+        const unsigned long long m = bucket_peers(q);
+        const uint32_t rank = (uint32_t)__popcll(m & below);
+        // the lowest lane of each bucket writes the wave's count (after the
+        // zeroing above: one wave's LDS writes complete in order)
+        if (q < kOrderBuckets && rank == 0) wcnt[p][wave][q] = (uint32_t)__popcll(m);
+        lds_barrier();
+        if (q < kOrderBuckets) {
+            uint32_t c[16];
 #pragma unroll
-        for (uint32_t u = 0; u < kOrderAhead; ++u) qa[u] = bucket(start0 + u * 1024 + t);
+            for (uint32_t w = 0; w < 16; ++w) c[w] = wcnt[p][w][q];  // all at once
+            uint32_t pos = base[p][q] + rank;
 #pragma unroll
-        for (uint32_t u = 0; u < kOrderAhead; ++u) {
-            const uint32_t start = start0 + u * 1024;
-            if (start >= groups) break;  // uniform over the workgroup
-            const uint32_t q = qa[u];
-            if (lane < kOrderBuckets) wcnt[wave][lane] = 0;
-            const unsigned long long m = bucket_peers(q);
-            const uint32_t rank = (uint32_t)__popcll(m & below);
-            // the lowest lane of each bucket writes the wave's count (after
-            // the zeroing above: one wave's LDS writes complete in order)
-            if (q < kOrderBuckets && rank == 0) wcnt[wave][q] = (uint32_t)__popcll(m);
-            lds_barrier();
-            if (q < kOrderBuckets) {
-                uint32_t pos = base[q] + rank;
-                for (uint32_t w = 0; w < wave; ++w) pos += wcnt[w][q];
-                order[pos] = start + t;
-            }
-            lds_barrier();
-            if (t < kOrderBuckets) {
-                uint32_t c = 0;
-                for (uint32_t w = 0; w < 16; ++w) c += wcnt[w][t];
-                base[t] += c;
-            }
-            lds_barrier();
+            for (uint32_t w = 0; w < 16; ++w) pos += w < wave ? c[w] : 0u;
+            order[pos] = u;
         }
+        if (t < kOrderBuckets) {
+            uint32_t c = base[p][t];
+#pragma unroll
+            for (uint32_t w = 0; w < 16; ++w) c += wcnt[p][w][t];
+            base[p ^ 1u][t] = c;
+        }
+        lds_barrier();
     }
 }
 
@@ -1365,11 +1377,11 @@ __global__ __launch_bounds__(256) void bh_traverse32_kernel(
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const uint32_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
-    const uint32_t u = b * 4 + w;
-    if (u >= na / 32) return;  // whole wave beyond the last group
-    // order (na / 32 entries, a permutation of the groups) or index order
-    const uint32_t g = order ? __builtin_amdgcn_readfirstlane(order[u]) : u;
+    const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x, xcd);
+    // order: a permutation of the gridDim.x workgroups' units
+    const uint32_t b = order ? __builtin_amdgcn_readfirstlane(order[b0]) : b0;
+    const uint32_t g = b * 4 + w;
+    if (g >= na / 32) return;  // whole wave beyond the last group
     // node count is bounded by the node arrays; every step advances the walk
     const uint32_t total = __builtin_amdgcn_readfirstlane(min(ctr->n_nodes, cap));
     const uint32_t dst = g * 32 + (lane & 31u);  // this particle's slot in the next order
@@ -1483,7 +1495,7 @@ hipError_t bh_alloc(BhBuffers *b, size_t na, uint32_t bits) {
     GSIM_ALLOC(b->gcost, na / 64);
     GSIM_ALLOC(b->gchunk, na / 64 + 1);
     GSIM_ALLOC(b->plan, kMaxHeavy);
-    GSIM_ALLOC(b->order, na / 32 + 1);
+    GSIM_ALLOC(b->order, na / 128 + 1);
     GSIM_ALLOC(b->heavy, na / 64 + 1);
     GSIM_ALLOC(b->gvis, na / 32 + 1);  // one slot per group of W = 32 or 64
     GSIM_ALLOC(b->gacc, na / 32 + 1);
@@ -1571,6 +1583,19 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
             i < 0 ? nullptr : b.keys[i], i < 0 ? nullptr : b.idx[i], i < 0 ? b.cells[kin] : nullptr,
             b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
             &b.ctr->sort_stall);
+    }
+    if (s->W == 32 && s->trav_order && s->side) {
+        // the walk's launch order from the last walk's costs, planned on the
+        // side stream beside the rest of the build (joined before the walk);
+        // forked after the radix passes, whose look-back chains a busy CU
+        // would lengthen
+        hipError_t e = hipEventRecord(s->ev_fork, s->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->ev_fork, 0);
+        if (e != hipSuccess) return e;
+        bh_order_plan_kernel<<<1, 1024, 0, s->side>>>(b.gvis, na / 32, b.order);
+        e = hipEventRecord(s->ev_join, s->side);
+        if (e != hipSuccess) return e;
+        b.order_planned = true;
     }
     b.sorted_slot = deep ? 1 : 0;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
@@ -1688,7 +1713,12 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     } else {
         const uint32_t g32 = (na / 32 + 3) / 4;  // four 32-particle groups per workgroup
         const uint32_t *order = nullptr;
-        if (s->trav_order) {
+        if (b.order_planned) {  // planned beside the build
+            const hipError_t e = hipStreamWaitEvent(s->stream, s->ev_join, 0);
+            if (e != hipSuccess) return e;
+            order = b.order;
+            b.order_planned = false;
+        } else if (s->trav_order) {
             bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, na / 32, b.order);
             order = b.order;
         }

@@ -33,7 +33,7 @@ constexpr float kSimSize = 500.0f;
 constexpr uint32_t kTreeSize = 1024;
 constexpr uint32_t kTreeDepth = 10;
 // GSIM_TUNE_BH_ORDER = 2 (auto): every kOrderPeriod steps, kOrderTrial
-// consecutive W = 32 walks alternate heaviest-first and index order
+// consecutive W = 32 steps alternate heaviest-first and index order
 constexpr uint32_t kOrderTrial = 4;
 constexpr uint64_t kOrderPeriod = 1024;
 constexpr uint32_t kKilledKey = 1u << 20;  // sorts after every 20-bit cell
@@ -133,7 +133,8 @@ struct BhBuffers {
     uint32_t *gchunk = nullptr;      // [Na/64 + 1] chunks each group summed last step (issue priority)
     uint32_t *plan = nullptr;        // [kMaxHeavy] the heavy list
     uint8_t *heavy = nullptr;        // [Na/64 + 1] 1: on the heavy list this step
-    uint32_t *order = nullptr;       // [Na/32] W = 32 launch order, heaviest groups first
+    uint32_t *order = nullptr;       // [Na/128] W = 32 launch order of the workgroups
+    bool order_planned = false;      // the plan of this step runs on the side stream
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
     // each group's real-lane node visits and acceptances in the last walk,
     // written by the group's wave and summed on a dump (one same-address
@@ -187,7 +188,7 @@ struct gsim_sim {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;        // Barnes-Hut: the heavy list's walks, beside the main one
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // W = 64 heavy list; W = 32 order plan
     size_t n = 0, na = 0;
     // FP32 constants squared/multiplied on the host as the reference does
     // (DirectSimulation.cpp:179, BarnesHutSimulation.cpp:575-578).
@@ -202,13 +203,13 @@ struct gsim_sim {
                                        // balanced: split above this x mean cost (0: never)
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
     // W = 32 launch order: order_mode 0 index, 1 heaviest first (by last
-    // step's visits), 2 auto: a trial of kOrderTrial walks alternating the
+    // step's visits), 2 auto: a trial of kOrderTrial steps alternating the
     // two, timed by events, keeps the faster once the host sees the trial's
     // last event complete (polled at each later step; never waited for)
     uint32_t order_mode = 2;
     uint32_t trav_order = 0;           // the order the next walk uses (0/1)
     hipEvent_t order_ev[2 * gsim::kOrderTrial] = {};
-    uint32_t order_rec = 0;            // trial walks recorded
+    uint32_t order_rec = 0;            // trial steps recorded
     bool order_wait = false;           // the trial's events are outstanding
     uint64_t order_next = 1;           // first step of the next trial
     uint32_t xcd_order = 1;            // static traversals: each XCD takes runs of this

@@ -212,7 +212,7 @@ typedef enum gsim_tuning {
     GSIM_TUNE_BH_ORDER = 11           /* W = 32 group launch order: 0 index order, 1
                                          heaviest first (by the last step's visits), 2
                                          auto (the default): every 1,024 steps, four
-                                         walks alternate the two and the faster is kept
+                                         steps alternate the two and the faster is kept
                                          (decided once the host sees them complete;
                                          never waited for)                              */
 } gsim_tuning;
