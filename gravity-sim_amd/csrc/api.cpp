@@ -144,6 +144,7 @@ gsim_status ds_configure(gsim_sim *s, bool sym) {
 // GSIM_TUNE_BH_ORDER = 2: whether this step's walk is a trial walk (and with
 // which order); otherwise the walk keeps the order picked last.
 bool order_trial_step(gsim_sim *s) {
+    s->order_fresh = false;
     if (s->order_mode != 2 || s->W != 32) return false;
     if (s->order_wait || s->steps < s->order_next || s->order_rec >= gsim::kOrderTrial) return false;
     if (!s->order_ev[0])
@@ -155,12 +156,14 @@ bool order_trial_step(gsim_sim *s) {
                 return false;
             }
     s->trav_order = (s->order_rec & 1u) ? 0u : 1u;  // heaviest first, index, ...
+    s->order_fresh = true;
     return true;
 }
 
 // Once the trial's last step has completed (never waiting for it), keeps the
-// faster order: heaviest first only if its steps took under 97% of the index
-// order's.
+// faster order.  The trial's heaviest-first steps plan afresh each time,
+// where later steps plan every kOrderReplan steps: the comparison leans
+// toward index order by most of a plan's cost.
 void order_poll(gsim_sim *s) {
     if (!s->order_wait) return;
     const hipEvent_t last = s->order_ev[2 * gsim::kOrderTrial - 1];
@@ -175,7 +178,7 @@ void order_poll(gsim_sim *s) {
         ms[(i & 1u) ? 0 : 1] += t;
     }
     (void)hipGetLastError();
-    s->trav_order = ok && ms[1] < 0.97f * ms[0] ? 1u : 0u;
+    s->trav_order = ok && ms[1] < ms[0] ? 1u : 0u;
     s->order_wait = false;
     s->order_rec = 0;
     s->order_next = s->steps + gsim::kOrderPeriod;

@@ -1584,11 +1584,13 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
             b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
             &b.ctr->sort_stall);
     }
-    if (s->W == 32 && s->trav_order && s->side) {
+    if (s->W == 32 && s->trav_order && s->side &&
+        (!b.order_valid || s->order_fresh || s->steps % kOrderReplan == 0)) {
         // the walk's launch order from the last walk's costs, planned on the
         // side stream beside the rest of the build (joined before the walk);
         // forked after the radix passes, whose look-back chains a busy CU
-        // would lengthen
+        // would lengthen.  Costs change slowly, so the order is re-planned
+        // every kOrderReplan steps (and for every trial step).
         hipError_t e = hipEventRecord(s->ev_fork, s->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->ev_fork, 0);
         if (e != hipSuccess) return e;
@@ -1596,6 +1598,7 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
         e = hipEventRecord(s->ev_join, s->side);
         if (e != hipSuccess) return e;
         b.order_planned = true;
+        b.order_valid = s->steps > 0;  // the first step's plan has no costs yet
     }
     b.sorted_slot = deep ? 1 : 0;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
@@ -1718,8 +1721,11 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
             if (e != hipSuccess) return e;
             order = b.order;
             b.order_planned = false;
+        } else if (s->trav_order && b.order_valid) {
+            order = b.order;  // planned at an earlier step
         } else if (s->trav_order) {
             bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, na / 32, b.order);
+            b.order_valid = s->steps > 0;
             order = b.order;
         }
         if (dbg)

@@ -36,6 +36,7 @@ constexpr uint32_t kTreeDepth = 10;
 // consecutive W = 32 steps alternate heaviest-first and index order
 constexpr uint32_t kOrderTrial = 4;
 constexpr uint64_t kOrderPeriod = 1024;
+constexpr uint64_t kOrderReplan = 8;  // steps between plans of the order
 constexpr uint32_t kKilledKey = 1u << 20;  // sorts after every 20-bit cell
 // 15-bit leaves (opt-in, beyond the reference): a 32768^2 leaf grid, depth 15,
 // 30-bit keys sorted in four 8-bit passes.
@@ -135,6 +136,7 @@ struct BhBuffers {
     uint8_t *heavy = nullptr;        // [Na/64 + 1] 1: on the heavy list this step
     uint32_t *order = nullptr;       // [Na/128] W = 32 launch order of the workgroups
     bool order_planned = false;      // the plan of this step runs on the side stream
+    bool order_valid = false;        // order holds a plan (a permutation of the units)
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
     // each group's real-lane node visits and acceptances in the last walk,
     // written by the group's wave and summed on a dump (one same-address
@@ -208,6 +210,7 @@ struct gsim_sim {
     // last event complete (polled at each later step; never waited for)
     uint32_t order_mode = 2;
     uint32_t trav_order = 0;           // the order the next walk uses (0/1)
+    bool order_fresh = false;          // this step is a trial step: plan afresh
     hipEvent_t order_ev[2 * gsim::kOrderTrial] = {};
     uint32_t order_rec = 0;            // trial steps recorded
     bool order_wait = false;           // the trial's events are outstanding
