@@ -145,7 +145,7 @@ gsim_status ds_configure(gsim_sim *s, bool sym) {
 // which order); otherwise the walk keeps the order picked last.
 bool order_trial_step(gsim_sim *s) {
     s->order_fresh = false;
-    if (s->order_mode != 2 || s->W != 32) return false;
+    if (s->order_mode != 2 || !gsim::order_launch(s)) return false;
     if (s->order_wait || s->steps < s->order_next || s->order_rec >= gsim::kOrderTrial) return false;
     if (!s->order_ev[0])
         for (hipEvent_t &e : s->order_ev)

@@ -1329,7 +1329,8 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
     float2 *__restrict__ acc_dbg, uint32_t na, uint32_t cap, float eps2, float theta2,
     float kv, float kp, unsigned long long *__restrict__ gvis,
     unsigned long long *__restrict__ gacc, uint32_t *__restrict__ gchunk,
-    const uint8_t *__restrict__ heavy, uint32_t xcd, NextKeys nk) {
+    const uint8_t *__restrict__ heavy, uint32_t xcd, NextKeys nk,
+    const uint32_t *__restrict__ order) {
     __shared__ float4 stage[kStaged ? 4 : 1][64];
     __shared__ uint2 runq[kStaged ? 4 : 1][kStaged ? kRunQueue : 1];
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1350,7 +1351,9 @@ __global__ __launch_bounds__(256) void bh_traverse64_kernel(
                                              theta2, kv, kp, gvis, gacc, nk, nullptr, gchunk);
         }
     } else {
-        const uint32_t b = xcd_block(blockIdx.x, gridDim.x, xcd);
+        const uint32_t b0 = xcd_block(blockIdx.x, gridDim.x, xcd);
+        // order: a permutation of the gridDim.x workgroups' units
+        const uint32_t b = order ? __builtin_amdgcn_readfirstlane(order[b0]) : b0;
         const uint32_t g = b * 4 + w;
         if (g >= groups) return;  // whole wave beyond the last group
         walk_group<kDebugAccel, kStaged>(g, total, st, rq, sorted_src, pos, vel, node4, ncount,
@@ -1584,7 +1587,7 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
             b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
             &b.ctr->sort_stall);
     }
-    if (s->W == 32 && s->trav_order && s->side &&
+    if (s->trav_order && s->side && order_launch(s) &&
         (!b.order_valid || s->order_fresh || s->steps % kOrderReplan == 0)) {
         // the walk's launch order from the last walk's costs, planned on the
         // side stream beside the rest of the build (joined before the walk);
@@ -1594,7 +1597,7 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
         hipError_t e = hipEventRecord(s->ev_fork, s->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->ev_fork, 0);
         if (e != hipSuccess) return e;
-        bh_order_plan_kernel<<<1, 1024, 0, s->side>>>(b.gvis, na / 32, b.order);
+        bh_order_plan_kernel<<<1, 1024, 0, s->side>>>(b.gvis, na / s->W, b.order);
         e = hipEventRecord(s->ev_join, s->side);
         if (e != hipSuccess) return e;
         b.order_planned = true;
@@ -1636,6 +1639,24 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
                                                       b.ncount, b.top, box, nullptr, b.ctr);
     }
     return hipGetLastError();
+}
+
+// The static walk's workgroup order (null: index order): planned beside the
+// build (joined here), kept from an earlier step, or planned now.  A failed
+// join leaves index order.
+static const uint32_t *launch_order(gsim_sim *s, uint32_t groups) {
+    BhBuffers &b = s->bh;
+    if (b.order_planned) {
+        b.order_planned = false;
+        if (hipStreamWaitEvent(s->stream, s->ev_join, 0) != hipSuccess) return nullptr;
+        return b.order;
+    }
+    if (!s->trav_order) return nullptr;
+    if (!b.order_valid) {
+        bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, groups, b.order);
+        b.order_valid = s->steps > 0;
+    }
+    return b.order;
 }
 
 hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
@@ -1689,6 +1710,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
 #undef GSIM_HEAVY
             heavy = b.heavy;
         }
+        const uint32_t *order = persistent ? nullptr : launch_order(s, groups);
         if (persistent) {
             // waves_per_simd waves on each of the 1024 SIMDs, at most one per group
             uint32_t waves = 1024u * s->trav_waves_per_simd;
@@ -1699,7 +1721,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
     bh_traverse64_kernel<D, S, P><<<g, 256, 0, s->stream>>>(                               \
         b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na, \
         (uint32_t)b.cap, s->eps2, s->theta2, s->kv, s->kp, b.gvis, b.gacc, b.gchunk, heavy, \
-        s->xcd_order, nk)
+        s->xcd_order, nk, order)
         if (dbg) {
             if (persistent) { if (staged) GSIM_TRAV(true, true, true); else GSIM_TRAV(true, false, true); }
             else { if (staged) GSIM_TRAV(true, true, false); else GSIM_TRAV(true, false, false); }
@@ -1715,19 +1737,7 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         }
     } else {
         const uint32_t g32 = (na / 32 + 3) / 4;  // four 32-particle groups per workgroup
-        const uint32_t *order = nullptr;
-        if (b.order_planned) {  // planned beside the build
-            const hipError_t e = hipStreamWaitEvent(s->stream, s->ev_join, 0);
-            if (e != hipSuccess) return e;
-            order = b.order;
-            b.order_planned = false;
-        } else if (s->trav_order && b.order_valid) {
-            order = b.order;  // planned at an earlier step
-        } else if (s->trav_order) {
-            bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, na / 32, b.order);
-            b.order_valid = s->steps > 0;
-            order = b.order;
-        }
+        const uint32_t *order = launch_order(s, na / 32);
         if (dbg)
             bh_traverse32_kernel<true><<<g32, 256, 0, s->stream>>>(
                 b.sorted_src, pos, vel, b.node4, b.ncount, b.ctr, pos_out, vel_out, acc, na,

@@ -134,7 +134,7 @@ struct BhBuffers {
     uint32_t *gchunk = nullptr;      // [Na/64 + 1] chunks each group summed last step (issue priority)
     uint32_t *plan = nullptr;        // [kMaxHeavy] the heavy list
     uint8_t *heavy = nullptr;        // [Na/64 + 1] 1: on the heavy list this step
-    uint32_t *order = nullptr;       // [Na/128] W = 32 launch order of the workgroups
+    uint32_t *order = nullptr;       // [Na/128] static walk's launch order of the workgroups
     bool order_planned = false;      // the plan of this step runs on the side stream
     bool order_valid = false;        // order holds a plan (a permutation of the units)
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
@@ -204,7 +204,7 @@ struct gsim_sim {
     uint32_t trav_split = 2;           // static: heavy list above this x mean chunk count;
                                        // balanced: split above this x mean cost (0: never)
     uint32_t bh_box = 0;               // adaptive bounding box instead of the fixed domain
-    // W = 32 launch order: order_mode 0 index, 1 heaviest first (by last
+    // Static walk's launch order: order_mode 0 index, 1 heaviest first (by last
     // step's visits), 2 auto: a trial of kOrderTrial steps alternating the
     // two, timed by events, keeps the faster once the host sees the trial's
     // last event complete (polled at each later step; never waited for)
@@ -267,5 +267,9 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
 hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
                               float2 *pos_out, float2 *vel_out, const uint32_t *perm_out);
 hipError_t bh_launch_morton30(gsim_sim *s, const float2 *pos, uint32_t *out);
+// Whether the handle's walk is a static launch of four-group workgroups,
+// which GSIM_TUNE_BH_ORDER can reorder (W = 32, or W = 64 neither persistent
+// nor balanced).
+inline bool order_launch(const gsim_sim *s) { return s->W == 32 || (s->trav_mode & 5u) == 0; }
 
 }  // namespace gsim

@@ -209,7 +209,9 @@ typedef enum gsim_tuning {
                                          round-robin, the default: measured at 500k,
                                          runs of 2-32 change nothing and contiguous
                                          ranges are 16% slower.  W = 32 and 64.         */
-    GSIM_TUNE_BH_ORDER = 11           /* W = 32 group launch order: 0 index order, 1
+    GSIM_TUNE_BH_ORDER = 11           /* static walk's launch order (W = 32, and W =
+                                         64 neither persistent nor balanced), by
+                                         workgroup of four groups: 0 index order, 1
                                          heaviest first (by the last step's visits), 2
                                          auto (the default): every 1,024 steps, four
                                          steps alternate the two and the faster is kept

@@ -548,21 +548,22 @@ def test_bh_xcd_group_order_is_bit_identical(W):
         assert np.array_equal(a.pos, b.pos) and np.array_equal(a.vel, b.vel)
 
 
-def test_bh_heaviest_first_order_is_bit_identical():
-    """The W = 32 launch order -- index, heaviest first by last step's
-    visits, and auto (a timed trial of both over steps 1-4), combined with
-    XCD runs -- changes only which wave walks a group: states and
-    visit/accept counts are bit-identical over steps (the first step has no
-    costs yet and runs in index order)."""
+@pytest.mark.parametrize("W", [32, 64])
+def test_bh_heaviest_first_order_is_bit_identical(W):
+    """The static walk's workgroup order -- index, heaviest first by last
+    step's visits (re-planned every 8 steps), and auto (a timed trial of both
+    over steps 1-4), combined with XCD runs -- changes only which wave walks
+    a group: states and visit/accept counts are bit-identical over steps
+    (the first step has no costs yet and runs in index order)."""
     p = gsim.generate("galaxy-collision", 40000, 120.0, 1.0, 10.0, seed=211)
     out = []
     for order, xcd in ((0, 1), (1, 1), (2, 1), (1, 16)):
-        with gsim.Simulation(p, "barnes-hut", group_width=32, accuracy_parameter=0.8) as s:
+        with gsim.Simulation(p, "barnes-hut", group_width=W, accuracy_parameter=0.8) as s:
             s.set_tuning(gsim.TUNE_BH_ORDER, order)
             s.set_tuning(gsim.TUNE_BH_XCD_ORDER, xcd)
             s.step(5)
             s.synchronize()
-            s.step(3)  # auto: the trial has completed; its pick walks these
+            s.step(5)  # auto: the trial has completed; its pick walks these
             out.append((s.read(), tuple(int(x) for x in s.dump(gsim.DUMP_COUNTERS))))
     a, ca = out[0]
     for b, cb in out[1:]:
