@@ -48,11 +48,12 @@ DS_SYM_ISSUED_PER_PAIR = 13.3125
 # x 2 + WRITE_SIZE x 1 (factors from a 256 MiB copy), 2 x 2,417 + 48,950 KiB;
 # profiles/ds50k_bh500k_counters.md
 DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
-# HBM bytes per traversal launch at 500,032 particles (same counters), by
-# group width: bh_traverse64_kernel (one wave per group, staged runs)
-# 2 x 75,874 + 33,931 KiB; bh_traverse32_kernel (one 32-particle group per
-# wave) 2 x 82,620 + 33,770 KiB
-BH_TRAFFIC_BYTES = {64: (2 * 75874 + 33931) * 1024, 32: (2 * 82620 + 33770) * 1024}
+# HBM bytes per traversal launch at 500,032 particles (same counters, median
+# over the steady steps, state in sorted order), by group width:
+# bh_traverse64_kernel (one wave per group, staged runs) 2 x 43,485 + 25,531
+# KiB; bh_traverse32_kernel (one 32-particle group per wave) 2 x 44,198 +
+# 25,172 KiB
+BH_TRAFFIC_BYTES = {64: (2 * 43485 + 25531) * 1024, 32: (2 * 44198 + 25172) * 1024}
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
