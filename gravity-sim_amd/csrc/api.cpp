@@ -197,8 +197,8 @@ gsim_status run_steps(gsim_sim *s, uint32_t count) {
             GSIM_HIP(hipEventRecord(ev[0], s->stream));
         }
         if (bh) {
-            // the walk's group order is fixed before the build, which plans it;
-            // a trial step is timed whole (the plan runs beside the build)
+            // the walk's workgroup order is fixed for the whole step; a trial
+            // step is timed whole
             const bool trial = order_trial_step(s);
             if (trial) GSIM_HIP(hipEventRecord(s->order_ev[2 * s->order_rec], s->stream));
             // the build writes the next order's masses, permutation and killed

@@ -1587,22 +1587,6 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
             b.keys[o], b.idx[o], na, 8u * pass, totals + 256 * pass, st,
             &b.ctr->sort_stall);
     }
-    if (s->trav_order && s->side && order_launch(s) &&
-        (!b.order_valid || s->order_fresh || s->steps % kOrderReplan == 0)) {
-        // the walk's launch order from the last walk's costs, planned on the
-        // side stream beside the rest of the build (joined before the walk);
-        // forked after the radix passes, whose look-back chains a busy CU
-        // would lengthen.  Costs change slowly, so the order is re-planned
-        // every kOrderReplan steps (and for every trial step).
-        hipError_t e = hipEventRecord(s->ev_fork, s->stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(s->side, s->ev_fork, 0);
-        if (e != hipSuccess) return e;
-        bh_order_plan_kernel<<<1, 1024, 0, s->side>>>(b.gvis, na / s->W, b.order);
-        e = hipEventRecord(s->ev_join, s->side);
-        if (e != hipSuccess) return e;
-        b.order_planned = true;
-        b.order_valid = s->steps > 0;  // the first step's plan has no costs yet
-    }
     b.sorted_slot = deep ? 1 : 0;
     const uint32_t *sk = b.keys[b.sorted_slot], *si = b.idx[b.sorted_slot];
     const uint32_t pb = (uint32_t)b.prep_blocks;
@@ -1641,20 +1625,17 @@ hipError_t bh_launch_build(gsim_sim *s, const float2 *pos, const float *mass,
     return hipGetLastError();
 }
 
-// The static walk's workgroup order (null: index order): planned beside the
-// build (joined here), kept from an earlier step, or planned now.  A failed
-// join leaves index order.
+// The static walk's workgroup order (null: index order), re-planned from the
+// last walk's costs every kOrderReplan steps (costs change slowly) and on
+// every trial step, on the walk's own stream: a plan on the side stream
+// beside the build measured slower (the cross-stream fork and join cost more
+// than the plan).
 static const uint32_t *launch_order(gsim_sim *s, uint32_t groups) {
     BhBuffers &b = s->bh;
-    if (b.order_planned) {
-        b.order_planned = false;
-        if (hipStreamWaitEvent(s->stream, s->ev_join, 0) != hipSuccess) return nullptr;
-        return b.order;
-    }
     if (!s->trav_order) return nullptr;
-    if (!b.order_valid) {
+    if (!b.order_valid || s->order_fresh || s->steps % kOrderReplan == 0) {
         bh_order_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gvis, groups, b.order);
-        b.order_valid = s->steps > 0;
+        b.order_valid = s->steps > 0;  // the first step's plan has no costs yet
     }
     return b.order;
 }

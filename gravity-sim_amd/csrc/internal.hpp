@@ -135,7 +135,6 @@ struct BhBuffers {
     uint32_t *plan = nullptr;        // [kMaxHeavy] the heavy list
     uint8_t *heavy = nullptr;        // [Na/64 + 1] 1: on the heavy list this step
     uint32_t *order = nullptr;       // [Na/128] static walk's launch order of the workgroups
-    bool order_planned = false;      // the plan of this step runs on the side stream
     bool order_valid = false;        // order holds a plan (a permutation of the units)
     uint32_t *gpre = nullptr;        // [Na/64 + 1]
     // each group's real-lane node visits and acceptances in the last walk,
@@ -190,7 +189,7 @@ struct gsim_sim {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t side = nullptr;        // Barnes-Hut: the heavy list's walks, beside the main one
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;  // W = 64 heavy list; W = 32 order plan
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t n = 0, na = 0;
     // FP32 constants squared/multiplied on the host as the reference does
     // (DirectSimulation.cpp:179, BarnesHutSimulation.cpp:575-578).
