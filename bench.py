@@ -50,10 +50,10 @@ DS_SYM_ISSUED_PER_PAIR = 13.3125
 DS_TRAFFIC_BYTES = (2 * 2417 + 48950) * 1024
 # HBM bytes per traversal launch at 500,032 particles (same counters, median
 # over the steady steps, state in sorted order), by group width:
-# bh_traverse64_kernel (one wave per group, staged runs) 2 x 44,910 + 25,500
-# KiB; bh_traverse32_kernel (one 32-particle group per wave) 2 x 44,260 +
-# 25,070 KiB
-BH_TRAFFIC_BYTES = {64: (2 * 44910 + 25500) * 1024, 32: (2 * 44260 + 25070) * 1024}
+# bh_traverse64_kernel (one wave per group, staged runs) 2 x 44,890 + 25,530
+# KiB; bh_traverse32_kernel (one 32-particle group per wave) 2 x 44,310 +
+# 25,060 KiB
+BH_TRAFFIC_BYTES = {64: (2 * 44890 + 25530) * 1024, 32: (2 * 44310 + 25060) * 1024}
 BH_FLOP_VISIT, BH_FLOP_ACCEPT = 7, 8     # 2 sub + 2 FMA + theta-mul; rsqrt + 3 mul + 2 FMA
 BH_LANEOPS_VISIT, BH_LANEOPS_ACCEPT = 6, 6
 REF_DS_MS_50K = 3.602             # reference README.md:18 (RTX 3070), ms/step at 50,000
