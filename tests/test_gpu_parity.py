@@ -572,3 +572,37 @@ def test_bh_heaviest_first_order_is_bit_identical(W):
     with gsim.Simulation(p, "barnes-hut", group_width=32) as s:
         with pytest.raises(gsim.GsimError):
             s.set_tuning(gsim.TUNE_BH_ORDER, 3)
+
+
+@pytest.mark.parametrize("W,bits", [(32, 10), (64, 10), (32, 15)])
+def test_bh_walk_written_cells_match_positions(W, bits):
+    """From the second step on the walk writes the next build's sort input
+    (each particle's cell at its caller index) from the positions it
+    integrates, and the build only counts digits.  The cells that build
+    sorted equal the cells of the positions read back before it, the 30-bit
+    codes of those positions extend them, and the sorted order maps back to
+    the caller's indices as the oracle's build of the same positions does."""
+    p = gsim.generate("two-exponential-disks", 20000, 33.0, 1.0, 10.0, 1.0, seed=212)
+    with gsim.Simulation(p, "barnes-hut", group_width=W) as s:
+        if bits != 10:
+            s.set_tuning(gsim.TUNE_BH_LEAF_BITS, bits)
+        s.step(3)
+        st = s.read()
+        s.step(1)
+        cells = s.dump(gsim.DUMP_CELLS)[: p.n]
+        if bits == 10:
+            m30 = s.dump(gsim.DUMP_MORTON30)[: p.n]
+            t = s.tree()
+    if bits == 10:
+        assert np.array_equal(cells, H.cells_numpy(st.pos))
+        ok = cells != 0xFFFFFFFF
+        assert np.array_equal(m30[ok] >> np.uint32(10), cells[ok])
+        src = t["sorted_src"]
+        real = src[src < p.n]
+        # the sorted order by cell, ties by caller index, over the real particles
+        order = np.lexsort((np.arange(p.n), cells))
+        order = order[cells[order] != 0xFFFFFFFF]
+        assert np.array_equal(real, order)
+    else:
+        want = np.array([O.cell_bits(float(x), float(y), None, 15) for x, y in st.pos], np.uint32)
+        assert np.array_equal(cells, want)
