@@ -294,7 +294,7 @@ size_t gsim_batch_bytes_per_particle(void);
 /* Launch parameters of a batched step.  None changes any result: systems
  * stay bit-identical to their alone runs under every setting. */
 typedef struct gsim_batch_tuning {
-    uint32_t waves_per_simd;  /* persistent force-launch width, 1..8 (default 8)      */
+    uint32_t waves_per_simd;  /* persistent force-launch width, 1..8 (default 6)      */
     uint32_t sort_tile;       /* radix sort tile: 2048, 3072 (default) or 4096 keys  */
     uint32_t run_queue;       /* leaf runs a walking wave queues in its LDS stack
                                  before summing them, 1..64 (default 64)            */
