@@ -37,6 +37,11 @@ struct RankStats {
 // one counter for all of them serialises thousands of same-address atomics.
 constexpr uint32_t kLiveLists = 64;
 
+// The force launch's work list is dealt to kForceQueues interleaved queues
+// (item i to queue i mod kForceQueues); the waves of queue q take its items
+// one at a time from a counter, so a wave that drew light items takes more.
+constexpr uint32_t kForceQueues = 64;
+
 struct BatchCounts {
     uint32_t n_bh;            // Barnes-Hut systems (ranks)
     uint32_t n_bh_particles;  // sum of their padded counts = sorted positions
@@ -50,6 +55,7 @@ struct BatchCounts {
     uint32_t sort_stall;      // radix look-back waits that hit their bound (never expected)
     uint32_t sort_tile[4];    // tile counter of each radix pass
     uint32_t n_live[kLiveLists];  // (system, depth-5 tile) pairs holding leaves, per list
+    uint32_t qnext[kForceQueues];  // force launch: items taken from each queue (zeroed each step)
     unsigned long long accepts;  // Barnes-Hut real-lane acceptances, all steps
     unsigned long long visits;   // Barnes-Hut real-lane node visits, all steps
 };

@@ -575,12 +575,24 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
     BatchTune tune) {
     __shared__ float4 stage[4][64];
     __shared__ uint2 runq[4][kRunQueue];
-    const uint32_t w = threadIdx.x >> 6;
-    const uint32_t waves = gridDim.x * 4;
+    const uint32_t w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t nitems = counts->n_items;
+    // this wave's queue (kForceQueues queues, each shared by the waves of
+    // one residue): items are taken one at a time, the next one's ticket
+    // drawn before the current item is walked
+    const uint32_t q = (blockIdx.x * 4 + w) % kForceQueues;
+    uint32_t *const qctr = &counts->qnext[q];
+    auto draw = [&]() -> uint32_t {
+        uint32_t j = 0;
+        if (lane == 0) j = atomicAdd(qctr, 1u);
+        return __builtin_amdgcn_readfirstlane(__shfl(j, 0, 64));
+    };
     unsigned long long acc = 0, vis = 0;
-    for (uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + w); it < nitems;
-         it += waves) {
+    uint32_t next = draw();
+    while (true) {
+        const uint32_t it = q + kForceQueues * next;
+        if (it >= nitems) break;  // the queue is drained (every later ticket is beyond too)
+        next = draw();
         const uint2 item = items[it];
         const SysDesc d = desc[item.x];
         const bool live = remaining[item.x] != 0u;
@@ -601,7 +613,7 @@ __global__ __launch_bounds__(256) void bb_force_kernel(
             bb_ds_tile(item.y, d, pos, mass, partial);
         }
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
         wave_acc[blockIdx.x * 4 + w] = acc;
         wave_acc[kMaxForceWaves + blockIdx.x * 4 + w] = vis;
     }
@@ -615,6 +627,8 @@ __global__ __launch_bounds__(256) void bb_finish_kernel(uint32_t *__restrict__ r
                                                         const unsigned long long *__restrict__ wave_acc,
                                                         uint32_t nwaves) {
     __shared__ unsigned long long part[2][4];
+    // the force launch's queue counters, for the next step
+    if (blockIdx.x == 0 && threadIdx.x < kForceQueues) counts->qnext[threadIdx.x] = 0u;
     for (uint32_t s = blockIdx.x * 256 + threadIdx.x; s < counts->n_sys; s += gridDim.x * 256) {
         const uint32_t r = remaining[s];
         if (r != 0u && r != kUnlimitedSteps) remaining[s] = r - 1u;
