@@ -1675,8 +1675,13 @@ hipError_t bh_launch_traverse(gsim_sim *s, const float2 *pos, const float2 *vel,
         // resident from the start the side workgroups only find room at the
         // end and lengthen the launch (two disks at 500k: +5%), so they stay
         // off there.
+        // The heavy list and the heaviest-first launch order (below) answer
+        // the same tail; with the order active the list only lengthened the
+        // walk (1M, W = 64: two disks 0.91 -> 0.95, galaxy collision 1.21 ->
+        // 1.23 ms per step), so it runs with index order only.
         const uint8_t *heavy = nullptr;
-        if (!persistent && staged && s->trav_split && s->side && groups > kHeavyMinGroups) {
+        if (!persistent && staged && s->trav_split && s->side && groups > kHeavyMinGroups &&
+            !s->trav_order) {
             bh_heavy_plan_kernel<<<1, 1024, 0, s->stream>>>(b.gchunk, groups, s->trav_split,
                                                             kMaxHeavy, b.plan, b.heavy, b.ctr);
             hipError_t e = hipEventRecord(s->ev_fork, s->stream);
