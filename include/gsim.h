@@ -180,9 +180,10 @@ typedef enum gsim_tuning {
                                          (default 512, two per CU)                      */
     GSIM_TUNE_BH_SPLIT = 7,           /* groups whose last walk reaches this multiple
                                          of the mean are walked by 4 waves: in the
-                                         default traversal (chunk count, the heaviest
-                                         512 at most, beside the main launch) and the
-                                         balanced one (cost); 0 = never; default 2     */
+                                         default traversal in index order (chunk
+                                         count, the heaviest 512 at most, beside the
+                                         main launch) and the balanced one (cost);
+                                         0 = never; default 2                          */
     GSIM_TUNE_BH_ADAPTIVE_BOX = 8,    /* 0/1, beyond the reference and off by default:
                                          each step's tree spans the bounding square of
                                          the particles (centre c, half-size h = extent
