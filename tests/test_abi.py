@@ -173,3 +173,23 @@ def test_no_cpu_fallback_without_device():
         return
     sim.close()
     pytest.skip("a GPU is visible; covered by the gpu tests")
+
+
+def test_python_constants_mirror_the_header():
+    """Every GSIM_TUNE_*, GSIM_DUMP_* and GSIM_DEBUG_* value in include/gsim.h
+    has a Python constant of the same value (the ctypes layer passes these
+    integers straight to the C ABI), and the Python side defines no value the
+    header lacks."""
+    text = open(os.path.join(ROOT, "include", "gsim.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    header = {}
+    for name, val in re.findall(r"\bGSIM_((?:TUNE|DUMP)_\w+)\s*=\s*(\d+)", text):
+        header[name] = int(val)
+    for name, val in re.findall(r"#define\s+GSIM_(DEBUG_\w+)\s+(\d+)u?", text):
+        header[name] = int(val)
+    assert len([k for k in header if k.startswith("TUNE_")]) >= 11
+    assert len([k for k in header if k.startswith("DUMP_")]) >= 11
+    for name, val in header.items():
+        assert getattr(gsim, name) == val, name
+    py = {k for k in dir(gsim) if re.fullmatch(r"(TUNE|DUMP|DEBUG)_[A-Z0-9_]+", k)}
+    assert py == set(header), sorted(py ^ set(header))
